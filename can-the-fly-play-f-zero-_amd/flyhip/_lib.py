@@ -1,0 +1,108 @@
+"""ctypes binding of ``libflyhip.so`` (the C ABI in include/flyhip.h).
+
+torch is imported first so that its HIP runtime (SONAME libamdhip64.so.7) is the one the
+library binds to: device pointers from torch tensors and from the engine then live in one
+runtime.  There is no CPU fallback: a missing library raises at import time of the engine.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+import torch  # noqa: F401  (loads the HIP runtime the library links against)
+
+from .build import LIB_PATH
+
+FZ_ABI_VERSION = 1
+FZ_OK, FZ_E_ARG, FZ_E_STATE, FZ_E_HIP, FZ_E_OVERFLOW = 0, -1, -2, -3, -4
+
+c_void_p = ctypes.c_void_p
+c_int = ctypes.c_int
+
+
+class FzParams(ctypes.Structure):
+    _fields_ = [("decay_syn", ctypes.c_float), ("decay_m", ctypes.c_float), ("u_th", ctypes.c_float),
+                ("u_reset", ctypes.c_float), ("kick", ctypes.c_float),
+                ("in_scale", ctypes.c_float), ("pw_qscale", ctypes.c_float),
+                ("n_ref", c_int), ("n_delay", c_int)]
+
+
+class FzState(ctypes.Structure):
+    _fields_ = [("B", c_int), ("u", c_void_p), ("g", c_void_p), ("ref", c_void_p), ("ring", c_void_p),
+                ("counts", c_void_p), ("clock", c_void_p)]
+
+
+class FzInputs(ctypes.Structure):
+    _fields_ = [("n_ext", c_int), ("ext_idx", c_void_p), ("p_ext", c_void_p), ("bias_ext", c_void_p),
+                ("pw", c_void_p), ("alive", c_void_p), ("seed", ctypes.c_uint)]
+
+
+class FzTune(ctypes.Structure):
+    _fields_ = [("upd_threads", c_int), ("push_threads", c_int), ("push_chunk", c_int),
+                ("push_grid", c_int), ("block_steps", c_int)]
+
+
+# name -> (restype, argtypes); every symbol declared in include/flyhip.h
+SIGNATURES = {
+    "fz_version": (c_int, []),
+    "fz_last_error": (ctypes.c_char_p, []),
+    "fz_create": (c_int, [c_int, ctypes.c_longlong, c_void_p, c_void_p, c_void_p, c_int, c_int,
+                          ctypes.POINTER(FzParams), ctypes.POINTER(c_void_p)]),
+    "fz_destroy": (c_int, [c_void_p]),
+    "fz_set_plastic": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    "fz_bind": (c_int, [c_void_p, ctypes.POINTER(FzState)]),
+    "fz_run": (c_int, [c_void_p, ctypes.POINTER(FzInputs), c_int, c_void_p, c_void_p,
+                       ctypes.POINTER(FzTune), c_void_p]),
+    "fz_run_timed": (c_int, [c_void_p, ctypes.POINTER(FzInputs), c_int, ctypes.POINTER(FzTune), c_void_p,
+                             ctypes.POINTER(ctypes.c_float), ctypes.POINTER(c_int),
+                             ctypes.POINTER(ctypes.c_float), ctypes.POINTER(c_int)]),
+    "fz_graph_create": (c_int, [c_void_p, ctypes.POINTER(FzInputs), c_int, ctypes.POINTER(FzTune),
+                                ctypes.POINTER(c_void_p)]),
+    "fz_graph_launch": (c_int, [c_void_p, c_void_p]),
+    "fz_graph_destroy": (c_int, [c_void_p]),
+    "fz_propagate": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    "fz_check": (c_int, [c_void_p, c_void_p]),
+}
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Load the library (once).  Raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                               "(flyhip has no CPU fallback)")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        if lib.fz_version() != FZ_ABI_VERSION:
+            raise RuntimeError(f"libflyhip ABI {lib.fz_version()} != {FZ_ABI_VERSION}")
+        _lib = lib
+    return _lib
+
+
+class FlyHipError(RuntimeError):
+    pass
+
+
+def check(rc: int, what: str = ""):
+    if rc != FZ_OK:
+        msg = load().fz_last_error().decode(errors="replace")
+        exc = ValueError if rc == FZ_E_ARG else FlyHipError
+        raise exc(f"{what}: {msg} (code {rc})")
+
+
+def ptr(t) -> int | None:
+    """Device (or host) address of a tensor, None for None / empty."""
+    if t is None or t.numel() == 0:
+        return None
+    return t.data_ptr()
+
+
+def stream_handle(device=None) -> int:
+    return torch.cuda.current_stream(device).cuda_stream

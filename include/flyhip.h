@@ -1,0 +1,128 @@
+/*
+ * flyhip -- MI355X-native whole-brain leaky integrate-and-fire step (C ABI).
+ *
+ * The reference reaches its GPU kernels through CuPy's RawModule with
+ * positional arguments (flyzero/gpu.py:124-128, 242-251) behind the duck-typed
+ * Python classes `Brain` (flyzero/brain.py:79-172) and `BatchBrain`
+ * (flyzero/gpu.py:131-254).  This header is the boundary that replaces those
+ * launches: plain pointers and sizes, no framework types.  All array pointers
+ * are DEVICE pointers owned by the caller; the engine keeps only its own
+ * scratch (spike work lists, counters, degree table, graphs).
+ *
+ * Every function returns 0 on success or a negative FZ_E* code; the message
+ * for the calling thread is in fz_last_error().  A handle is not re-entrant:
+ * use it from one host thread.  All work is enqueued on the given stream
+ * (a hipStream_t passed as void*; NULL = the default stream).
+ */
+#ifndef FLYHIP_H
+#define FLYHIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FZ_ABI_VERSION 1
+
+#define FZ_OK 0
+#define FZ_E_ARG (-1)      /* invalid argument (null pointer, size out of range) */
+#define FZ_E_STATE (-2)    /* call out of order (e.g. fz_run before fz_bind) */
+#define FZ_E_HIP (-3)      /* a HIP runtime call failed */
+#define FZ_E_OVERFLOW (-4) /* a device-side capacity check fired */
+
+typedef struct fz_engine fz_engine;
+typedef struct fz_graph fz_graph;
+
+/* Model constants, derived on the host exactly as flyzero/brain.py:92-96
+ * (n_delay, n_ref, decay_m = f32(dt/tau_m), decay_syn = f32(exp(-dt/tau_syn)),
+ * kick = f32(f_poisson*w_syn)).  The membrane potential is held relative to
+ * rest, u = v - v_rest: u_th = f32(v_th - v_rest), u_reset = f32(v_reset - v_rest).
+ * Synaptic input is integer: a ring value `acc` arrives as (float)acc * in_scale
+ * mV, in_scale = w_syn * 2^-frac_bits.  Plastic weights are given in mV and
+ * quantised per use as rintf(pw * pw_qscale) fixed-point units. */
+typedef struct fz_params {
+    float decay_syn, decay_m, u_th, u_reset, kick, in_scale, pw_qscale;
+    int n_ref, n_delay;
+} fz_params;
+
+/* Caller-owned per-fly state (flyzero/gpu.py:186-199 holds the same arrays). */
+typedef struct fz_state {
+    int B;              /* flies */
+    float* u;           /* [B][n] membrane potential relative to rest, mV */
+    float* g;           /* [B][n] synaptic conductance, mV */
+    signed char* ref;   /* [B][n] refractory steps left */
+    int* ring;          /* [B][n_delay][n] int32 delay ring, fixed-point synapse counts */
+    int* counts;        /* [B][n] spike counts of the last window */
+    long long* clock;   /* [4]: t (ring phase), step_no (RNG step), block parity, reserved */
+} fz_state;
+
+/* Per-window inputs (flyzero/gpu.py:174-179, 201-226; plastic weights gpu.py:162-163). */
+typedef struct fz_inputs {
+    int n_ext;                  /* ext slots */
+    const int* ext_idx;         /* [n] slot or -1 */
+    const float* p_ext;         /* [B][n_ext] Poisson probability per step (Hz*dt/1000) */
+    const float* bias_ext;      /* [B][n_ext] steady bias, mV */
+    const float* pw;            /* [B][n_pl] plastic weights, mV (NULL if n_pl == 0) */
+    const unsigned char* alive; /* [B] 0 = retired fly (skipped), or NULL = all alive */
+    unsigned seed;              /* RNG seed (flyzero/gpu.py:172) */
+} fz_inputs;
+
+/* Launch shape (chosen by the autotuner; zero fields take defaults). */
+typedef struct fz_tune {
+    int upd_threads;   /* LIF-update workgroup size, multiple of 64, <= 1024 */
+    int push_threads;  /* push workgroup size, multiple of 64, <= 1024 */
+    int push_chunk;    /* edges per push work item */
+    int push_grid;     /* push workgroups (grid-stride) */
+    int block_steps;   /* steps per temporal block, 1..min(n_delay, 32) */
+} fz_tune;
+
+int fz_version(void);
+const char* fz_last_error(void);
+
+/* Engine over a device CSR of the FIXED (non-plastic) synapses.  Replaces the
+ * device upload in flyzero/gpu.py:143-163.  indptr int32[n+1], col int32[nnz],
+ * w int16[nnz] (w_bits = 16: whole synapse counts, shifted by frac_bits on use)
+ * or int32[nnz] (w_bits = 32: fixed-point values).  Arrays must outlive the engine. */
+int fz_create(int n, long long nnz, const int* indptr, const int* col, const void* w, int w_bits,
+              int frac_bits, const fz_params* p, fz_engine** out);
+int fz_destroy(fz_engine* e);
+
+/* Plastic synapses as a CSR by pre neuron (flyzero/gpu.py:150-161): pl_indptr
+ * int32[n+1], pl_post int32[n_pl]; entry k uses weight pw[b][k]. n_pl may be 0. */
+int fz_set_plastic(fz_engine* e, int n_pl, const int* pl_indptr, const int* pl_post, void* stream);
+
+/* Bind caller state for B flies and size the scratch for it. */
+int fz_bind(fz_engine* e, const fz_state* st);
+
+/* Advance every live fly `steps` steps (one window; flyzero/gpu.py:228-254,
+ * flyzero/brain.py:133-172).  counts are overwritten with this window's spike
+ * counts.  hits: NULL (counter-hash RNG, flyzero/gpu.py:66-67) or precomputed
+ * Poisson hits uint32[steps][B][ceil(n_ext/32)], bit j = ext slot j kicked.
+ * raster: NULL or uint32[steps][B][ceil(n/32)] receiving every spike. */
+int fz_run(fz_engine* e, const fz_inputs* in, int steps, const unsigned* hits, unsigned* raster,
+           const fz_tune* tune, void* stream);
+
+/* As fz_run, timing every kernel with HIP events on `stream`; waits for the
+ * stream.  Out: summed milliseconds and launch counts of the update and push
+ * kernels. */
+int fz_run_timed(fz_engine* e, const fz_inputs* in, int steps, const fz_tune* tune, void* stream,
+                 float* upd_ms, int* upd_launches, float* push_ms, int* push_launches);
+
+/* One window of `steps` steps as a replayable HIP graph.  The graph reads the
+ * clock, inputs and weights from device memory at every replay, so callers
+ * update them in place between replays. */
+int fz_graph_create(fz_engine* e, const fz_inputs* in, int steps, const fz_tune* tune, fz_graph** out);
+int fz_graph_launch(fz_graph* gr, void* stream);
+int fz_graph_destroy(fz_graph* gr);
+
+/* acc[col] += weight (fixed-point units) for every fixed edge of the n_list
+ * neurons in `list` (int32, device): the spike scatter of flyzero/brain.py:123-131
+ * (`Brain._propagate`) as an integer sum.  acc is int32[n]. */
+int fz_propagate(fz_engine* e, int n_list, const int* list, int* acc, void* stream);
+
+/* Reads (and clears) the device-side error flag; waits for `stream`. */
+int fz_check(fz_engine* e, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,20 @@
+import os
+import sys
+
+import pytest
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+for p in (REPO, os.path.join(REPO, "can-the-fly-play-f-zero-_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs a HIP device (MI355X)")
+
+
+@pytest.fixture(autouse=True)
+def _name_the_case(request):
+    # flushed before every test so a GPU failure names its case
+    print(f"\n[case] {request.node.nodeid}", flush=True)
+    yield

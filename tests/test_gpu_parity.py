@@ -1,0 +1,228 @@
+"""GPU parity: the HIP engine against the CPU oracle, bit for bit.
+
+Restates the reference's GPU tests (flyzero tests/test_gpu.py:39-72,168-183) with the oracle in
+the role of the reference's CPU ``Brain``, and adds raster/state identity on the FlyWire-shaped
+workload.
+"""
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import torch
+
+from helpers import fly_inputs, poisson_hits, ref_weights
+
+pytestmark = pytest.mark.gpu
+
+from flyhip import BatchBrain, Brain, LIFParams  # noqa: E402
+from flyhip.connectome import flywire_like, smooth_field  # noqa: E402
+from oracle.oracle import OracleBrain  # noqa: E402
+
+FRAME_MS = 1000.0 / 60.0988
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _oracle_counts(w, dt=0.25, pw=None, plastic_pos=None, ms=50):
+    ob = OracleBrain(w, LIFParams(dt=dt), batch=1 if pw is None else len(pw), plastic_pos=plastic_pos)
+    if pw is not None:
+        ob.pw[...] = pw
+    ob.v[:, :40] = -40.0
+    return ob.run(ms)
+
+
+def test_gpu_matches_oracle_without_noise():
+    # tests/test_gpu.py:39-47
+    w = ref_weights()
+    ref = _oracle_counts(w)[0]
+    gb = BatchBrain(w, LIFParams(dt=0.25), batch=3)
+    gb.v[:, :40] = -40.0
+    out = _np(gb.run(50))
+    assert ref.sum() > 50
+    for k in range(3):
+        np.testing.assert_array_equal(out[k], ref)
+
+
+def test_each_fly_has_its_own_plastic_synapses():
+    # tests/test_gpu.py:50-62
+    w = ref_weights()
+    ws = sp.csr_matrix(w, dtype=np.float32)
+    ws.sort_indices()
+    pos = np.flatnonzero(ws.indices < 30)          # synapses onto neurons 0..29
+    gb = BatchBrain(w, LIFParams(dt=0.25), batch=2, plastic_pos=pos)
+    changed = gb.data.copy()
+    changed[pos] = np.abs(changed[pos]) * 3
+    gb.pw[1] = torch.from_numpy(changed[pos]).cuda()
+    gb.v[:, :40] = -40.0
+    out = _np(gb.run(50))
+    pw = np.stack([gb.w0, changed[pos]])
+    ref = _oracle_counts(w, pw=pw, plastic_pos=pos)
+    np.testing.assert_array_equal(out, ref)
+    assert not np.array_equal(out[0], out[1])
+
+
+def test_plastic_path_equals_fixed_path_for_whole_synapse_counts():
+    # with whole synapse counts the plastic weights quantise exactly, so a fly whose plastic
+    # weights are untouched spikes exactly like a brain without plastic synapses
+    w = ref_weights()
+    w.data = np.rint(w.data)
+    ws = sp.csr_matrix(w, dtype=np.float32)
+    ws.sort_indices()
+    pos = np.flatnonzero(ws.indices < 150)
+    gb = BatchBrain(w, LIFParams(dt=0.25), batch=1, plastic_pos=pos)
+    assert gb.frac_bits == 0 and gb.w_bits == 16
+    gb.v[:, :40] = -40.0
+    out = _np(gb.run(50))
+    np.testing.assert_array_equal(out, _oracle_counts(w))
+
+
+def test_poisson_input_rate():
+    # tests/test_gpu.py:65-72; 100 Hz with 2.2 ms dead time after each spike
+    w = sp.csr_matrix((10, 10), dtype=np.float32)
+    gb = BatchBrain(w, LIFParams(dt=0.25), batch=2, seed=5)
+    gb.set_input(np.arange(10), 100.0)
+    c = _np(gb.run(1000))
+    assert 70 < c.mean() < 105
+    assert not np.array_equal(c[0], c[1])
+    ob = OracleBrain(w, LIFParams(dt=0.25), batch=2, seed=5)
+    ob.set_input(np.arange(10), 100.0)
+    np.testing.assert_array_equal(c, ob.run(1000))
+
+
+def test_long_plastic_rows_cover_every_synapse():
+    # tests/test_gpu.py:168-183: every synapse plastic, rows of >300 synapses
+    rng = np.random.default_rng(4)
+    w = sp.random(600, 600, density=0.6, random_state=4, format="csr")
+    w.data = rng.normal(5.0, 3, w.nnz)
+    pos = np.arange(w.nnz)
+    gb = BatchBrain(w, LIFParams(dt=0.25), batch=2, plastic_pos=pos)
+    assert np.diff(gb.indptr).max() > 300
+    gb.pw[1] *= 0.5
+    gb.v[:, :40] = -40.0
+    out = _np(gb.run(30))
+    ob = OracleBrain(w, LIFParams(dt=0.25), batch=2, plastic_pos=pos)
+    ob.pw[1] *= 0.5
+    ob.v[:, :40] = -40.0
+    ref = ob.run(30)
+    assert ref[0].sum() > 50
+    np.testing.assert_array_equal(out, ref)
+    assert not np.array_equal(out[0], out[1])
+
+
+@pytest.fixture(scope="module")
+def small_fly():
+    return flywire_like(n=20_000, n_edges=20_000 * 109, seed=11)
+
+
+def _pair(fly, batch=1, seed=0, plastic_pos=None):
+    p = LIFParams()
+    gb = BatchBrain(fly.weights, p, batch=batch, seed=seed, plastic_pos=plastic_pos)
+    ob = OracleBrain(fly.weights, p, batch=batch, seed=seed, plastic_pos=plastic_pos)
+    idx, rate = fly_inputs(fly)
+    gb.set_input(idx, rate)
+    ob.set_input(idx, rate)
+    return gb, ob
+
+
+def _assert_state_equal(gb, ob):
+    np.testing.assert_array_equal(_np(gb.u), ob.u)
+    np.testing.assert_array_equal(_np(gb.g), ob.g)
+    np.testing.assert_array_equal(_np(gb.ref), ob.ref)
+    np.testing.assert_array_equal(_np(gb.ring), ob.ring)
+
+
+def test_raster_identical_with_precomputed_hits(small_fly):
+    gb, ob = _pair(small_fly, batch=2)
+    steps = 400
+    hits = poisson_hits(ob.p_ext[0], steps, 2, seed=3)
+    cg, rg = gb.run_steps(steps, hits=torch.from_numpy(hits.view(np.int32)).cuda(), raster=True)
+    co, ro = ob.run_steps(steps, hits=hits, raster=True)
+    rg = _np(rg).view(np.uint32)
+    assert co.sum() > 1000
+    first = np.flatnonzero((rg != ro).any(axis=(1, 2)))
+    assert len(first) == 0, f"first divergent step {first[:1]}"
+    np.testing.assert_array_equal(_np(cg), co)
+    _assert_state_equal(gb, ob)
+
+
+def test_hash_rng_frames_graph_and_eager_match_oracle(small_fly):
+    gb, ob = _pair(small_fly, batch=3, seed=9)
+    ge, _ = _pair(small_fly, batch=3, seed=9)
+    xy = np.r_[small_fly.input_xy, [[0.5, 0.5]]]
+    for frame in range(6):
+        rate = np.r_[smooth_field(small_fly.input_xy, frame), np.float32(60.0)]
+        idx = np.r_[small_fly.input_idx, small_fly.drive_idx]
+        for b in (gb, ob, ge):
+            b.set_input(idx, rate)
+        cg = _np(gb.run(FRAME_MS))                 # graph replay
+        ce = _np(ge.run(FRAME_MS, graph=False))    # eager launches
+        co = ob.run(FRAME_MS)
+        np.testing.assert_array_equal(cg, co, err_msg=f"frame {frame}")
+        np.testing.assert_array_equal(ce, co, err_msg=f"frame {frame}")
+    assert xy.shape[0] == len(small_fly.input_idx) + 1
+    _assert_state_equal(gb, ob)
+    assert gb.t == ob.t and gb.step_no == ob.step_no
+
+
+def test_retired_flies_are_skipped_and_others_unaffected(small_fly):
+    gb, ob = _pair(small_fly, batch=4, seed=2)
+    gb.run(FRAME_MS)
+    ob.run(FRAME_MS)
+    v_before = _np(gb.u[1]).copy()
+    gb.alive[1] = 0
+    ob.alive = np.array([1, 0, 1, 1], np.uint8)
+    cg = _np(gb.run(FRAME_MS))
+    co = ob.run(FRAME_MS)
+    assert cg[1].sum() == 0
+    np.testing.assert_array_equal(_np(gb.u[1]), v_before)
+    np.testing.assert_array_equal(cg[[0, 2, 3]], co[[0, 2, 3]])
+    # respawn: reset the fly and revive it
+    gb.reset([1])
+    gb.alive[1] = 1
+    ob.u[1] = 0.0
+    ob.g[1] = 0
+    ob.ref[1] = 0
+    ob.ring[1] = 0
+    ob.alive = None
+    np.testing.assert_array_equal(_np(gb.run(FRAME_MS)), ob.run(FRAME_MS))
+
+
+def test_brain_single_fly_api():
+    # tests/test_flyzero.py:16-24 through the drop-in Brain
+    w = np.zeros((2, 2))
+    w[0, 1] = 100
+    b = Brain(sp.csr_matrix(w))
+    b.set_input([0], 200.0)
+    counts = b.run(200)
+    assert isinstance(counts, np.ndarray) and counts.dtype == np.int32
+    assert counts[0] > 10 and counts[1] > 5
+    b = Brain(sp.csr_matrix(-w))
+    b.set_input([0], 200.0)
+    assert b.run(200)[1] == 0
+
+
+def test_brain_propagate_matches_dense_matrix():
+    # tests/test_flyzero.py:27-34
+    w = sp.random(50, 50, density=0.2, random_state=1, format="csr") * 10
+    b = Brain(w)
+    spiking = np.array([3, 7, 20, 49])
+    out = np.zeros(50, np.float32)
+    b._propagate(spiking, out)
+    expected = np.asarray(w[spiking].sum(0)).ravel() * b.p.w_syn
+    # integer sums of weights quantised to 2**-frac_bits synapses: half a unit per term
+    atol = len(spiking) * 0.5 * 2.0 ** -b._bb.frac_bits * b.p.w_syn
+    np.testing.assert_allclose(out, expected, rtol=1e-5, atol=atol)
+
+
+def test_brain_weight_edits_reach_the_engine():
+    w = np.zeros((3, 3))
+    w[0, 1] = 100
+    b = Brain(sp.csr_matrix(w))
+    b.set_input([0], 200.0)
+    assert b.run(100)[2] == 0
+    b.data[0] = 0.0                 # remove 0 -> 1 (mV layout, like RewardPlasticity edits)
+    b.reset()
+    b.set_input([0], 200.0)
+    assert b.run(100)[1] == 0
