@@ -91,7 +91,7 @@ def main():
         sys.exit(2)
 
     import torch
-    from flyhip import BatchBrain, LIFParams
+    from flyhip import BatchBrain, LIFParams, autotune
     from flyhip.connectome import flywire_like
 
     t = time.perf_counter()
@@ -105,6 +105,10 @@ def main():
     # ---- B = 1 (headline) -------------------------------------------------
     bb = BatchBrain(fly.weights, p, batch=1, seed=0)
     rates = torch.from_numpy(frame_rates(fly, total)).cuda()
+    j1 = bb.slots(np.r_[fly.input_idx, fly.drive_idx])
+    bb.p_ext[:, j1] = rates[0] * float(np.float32(p.dt / 1000.0))
+    shape1 = autotune.tune(bb, use_cache=False)
+    log(f"B=1 launch shape: {shape1}")
     wall = run_workload(bb, fly, rates, args.steps, args.warmup, torch)
     sim_s = args.steps * steps_per_frame * p.dt / 1000.0
     value = sim_s / wall
@@ -148,7 +152,7 @@ def main():
                 "avg_launch_us": round(avg_ms * 1000.0, 3), "bytes_per_launch": round(bytes_launch),
                 "update_avg_us": round(upd_avg_ms * 1000.0, 3), "push_avg_us": round(push_avg_ms * 1000.0, 3),
                 "update_launches": tim["update_launches"], "push_launches": tim["push_launches"],
-                "spikes_per_step": round(n_spikes / max(upd_steps, 1), 1)}
+                "spikes_per_step": round(n_spikes / max(upd_steps, 1), 1), "launch_shape": shape1}
     log(f"kernels: update {upd_avg_ms * 1000:.1f} us/launch, push {push_avg_ms * 1000:.1f} us/launch; "
         f"dominant {dom}: {achieved:.1f} GB/s")
 
@@ -162,9 +166,14 @@ def main():
         nb = args.batch_steps + 2
         gbb = BatchBrain(fly.weights, p, batch=B, seed=1)
         rb = torch.from_numpy(frame_rates(fly, nb, gain, drive)).cuda()
+        jb = gbb.slots(np.r_[fly.input_idx, fly.drive_idx])
+        gbb.p_ext[:, jb] = rb[0] * float(np.float32(p.dt / 1000.0))
+        shapeb = autotune.tune(gbb, use_cache=False, reps=1)
+        log(f"B={B} launch shape: {shapeb}")
         wb = run_workload(gbb, fly, rb, args.batch_steps, 2, torch)
         vb = B * args.batch_steps * steps_per_frame * p.dt / 1000.0 / wb
         batched = {"B": B, "frames": args.batch_steps, "value": round(vb, 3), "unit": "fly-sim-seconds/wall-second",
+                   "launch_shape": shapeb,
                    "ms_per_frame": round(wb / args.batch_steps * 1000.0, 3),
                    "vs_b1": round(vb / value, 2)}
         log(f"B={B}: {vb:.2f} fly-sim-s/s aggregate ({batched['ms_per_frame']:.2f} ms/frame)")

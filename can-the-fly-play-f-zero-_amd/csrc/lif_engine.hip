@@ -5,15 +5,18 @@
 //  * Temporal blocking.  A spike at step s lands exactly n_delay steps later
 //    (brain.py:54-56), so for K <= n_delay consecutive steps every neuron's
 //    input is already sitting in the ring.  `lif_update` advances each neuron
-//    through a whole block of K steps with its state in registers: v, g, ref
+//    through a whole block of K steps with its state in registers: u, g, ref
 //    are read and written once per block instead of once per step.
-//  * Spikes of the block are compacted with a wavefront ballot + prefix sum and
-//    one 64-bit atomic per wavefront that reserves both the list entries and
-//    the range of outgoing edges they own, so the edge space of the block is
-//    one contiguous, ordered range.
-//  * `lif_push` splits that edge range into equal chunks (perfect load balance
-//    for any row-length distribution), stages the row segments of a chunk in
-//    LDS and scatters signed integer synapse counts into the int32 ring with
+//  * Spikes of the block go to a per-fly work list.  Each wavefront compacts
+//    its spikes with a prefix sum over its lanes, the waves of a workgroup
+//    combine in LDS, and one 64-bit atomic per workgroup reserves both the
+//    list entries and the range of outgoing edges they own, so each fly's
+//    edges of the block form one contiguous, ordered range.  Every entry
+//    carries its row offsets, and the update kernel records which entry holds
+//    the first edge of every fixed-size edge chunk.
+//  * `lif_push` gives every workgroup whole edge chunks (perfect load balance
+//    for any row-length distribution), stages the chunk's row segments in LDS
+//    and scatters signed integer synapse counts into the int32 ring with
 //    no-return atomics.  Integer adds commute, so the result is bit-exact and
 //    independent of scheduling.
 //  * One window is a chain of (update, push) pairs plus a one-thread clock
@@ -55,11 +58,10 @@ int fail(int code, const char* fmt, ...) {
         if (e_ != hipSuccess) return fail(FZ_E_HIP, "%s failed: %s", #x, hipGetErrorString(e_)); \
     } while (0)
 
-constexpr int KMAX = 32;                          // steps per block held in registers
-constexpr int ENT_SHIFT = 36;                     // packed counter: entries << 36 | edges
-constexpr unsigned long long EDGE_MASK = (1ull << ENT_SHIFT) - 1;
-constexpr long long ENT_LIMIT = 1ll << (64 - ENT_SHIFT);
-constexpr int PUSH_MAX_THREADS = 1024;
+constexpr int KMAX = 32;          // steps per block held in registers
+constexpr int MAX_THREADS = 1024;
+constexpr int MAX_WAVES = MAX_THREADS / 64;
+constexpr int QMIN = 64;          // smallest push chunk (sizes the chunk-start table)
 
 struct Consts {
     float decay_syn, decay_m, u_th, u_reset, kick, in_scale, pw_qscale;
@@ -77,9 +79,22 @@ __device__ __forceinline__ uint32_t mix32(uint32_t x) {
     return x;
 }
 
+// Per-fly spike work list of one block (entries of fly b start at b * cap).
+struct WorkList {
+    unsigned long long* ctr;  // [2][B] packed (entries << 32 | edges), parity-selected
+    unsigned* off;            // first edge of the entry within the fly's edge range
+    int* fbeg;                // fixed CSR row start
+    int* fdeg;                // fixed row length
+    int* pbeg;                // plastic CSR row start
+    int* slot;                // ring slot the row lands in
+    unsigned* cs;             // [B][ncs] entry holding edge c * chunk
+    long long cap;            // entries per fly
+    long long ncs;            // chunk-start slots per fly
+};
+
 struct UpdArgs {
     const long long* clock;
-    int N, B, K, kb, blk, first;
+    int N, B, K, kb, blk, first, chunk;
     float* u;              // membrane potential relative to rest, mV
     float* g;
     signed char* ref;
@@ -95,21 +110,21 @@ struct UpdArgs {
     int rw;
     unsigned seed;
     const unsigned char* alive;
-    const int* deg;        // fixed + plastic out-degree per neuron
-    unsigned long long* ctr;  // [2], parity-selected
-    int* ent_nrn;
-    int* ent_meta;            // fly << 8 | ring slot
-    unsigned long long* ent_off;
-    long long cap;
+    const int* indptr;
+    const int* pl_indptr;  // null without plastic synapses
+    WorkList wl;
     int* err;
     Consts c;
 };
 
 // One thread per (fly, neuron); advances K steps of flyzero/brain.py:37-56.
-__global__ void __launch_bounds__(1024) lif_update(UpdArgs a) {
+__global__ void __launch_bounds__(MAX_THREADS) lif_update(UpdArgs a) {
+    __shared__ unsigned long long s_wave[MAX_WAVES];
+    __shared__ unsigned long long s_base;
     const int b = blockIdx.y;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
     const bool valid = i < a.N;
     const size_t bn = (size_t)b * a.N;
     if (a.alive && !a.alive[b]) {  // retired fly: uniform per workgroup
@@ -209,45 +224,65 @@ __global__ void __launch_bounds__(1024) lif_update(UpdArgs a) {
         }
     }
 
-    // Spike work list: wavefront prefix sum, one packed atomic per wavefront.
-    int d = 0, nsp = 0;
+    // ---- spike work list: lane prefix sums, wave totals combined in LDS, one
+    // packed atomic per workgroup on this fly's counter.
+    int fb = 0, fd = 0, pb = 0, d = 0, nsp = 0;
     if (valid && mask) {
-        d = a.deg[i];
+        fb = a.indptr[i];
+        fd = a.indptr[i + 1] - fb;
+        d = fd;
+        if (a.pl_indptr) {
+            pb = a.pl_indptr[i];
+            d += a.pl_indptr[i + 1] - pb;
+        }
         if (d > 0) nsp = __popc(mask);
     }
-    if (__ballot(nsp > 0)) {
-        int xn = nsp;
-        unsigned long long xe = (unsigned long long)nsp * (unsigned)d;
+    unsigned long long x = ((unsigned long long)nsp << 32) + (unsigned long long)nsp * (unsigned)d;
+    const unsigned long long mine = x;
 #pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const int yn = __shfl_up(xn, o);
-            const unsigned long long ye = __shfl_up(xe, o);
-            if (lane >= o) {
-                xn += yn;
-                xe += ye;
-            }
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned long long y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) s_wave[wave] = x;  // wave total
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long tot = 0;
+        const int nw = blockDim.x >> 6;
+        for (int w = 0; w < nw; w++) {
+            const unsigned long long t = s_wave[w];
+            s_wave[w] = tot;  // exclusive prefix over waves
+            tot += t;
         }
-        unsigned long long old = 0;
-        if (lane == 63) {
+        unsigned long long base = 0;
+        if (tot) {
             const int par = (int)((a.clock[2] + a.blk) & 1);
-            old = atomicAdd(&a.ctr[par], ((unsigned long long)xn << ENT_SHIFT) + xe);
+            base = atomicAdd(&a.wl.ctr[(size_t)par * a.B + b], tot);
         }
-        old = __shfl(old, 63);
-        if (nsp) {
-            long long pos = (long long)(old >> ENT_SHIFT) + xn - nsp;
-            unsigned long long e = (old & EDGE_MASK) + xe - (unsigned long long)nsp * (unsigned)d;
-            if (pos + nsp > a.cap) {
-                atomicOr(a.err, 1);
-            } else {
-                for (int k = 0; k < a.K; k++) {
-                    if ((mask >> k) & 1u) {
-                        const int slot = (slot0 + k) % nd;
-                        a.ent_nrn[pos] = i;
-                        a.ent_meta[pos] = (b << 8) | slot;
-                        a.ent_off[pos] = e;
-                        pos++;
-                        e += (unsigned)d;
-                    }
+        s_base = base;
+    }
+    __syncthreads();
+    if (nsp) {
+        const unsigned long long start = s_base + s_wave[wave] + x - mine;
+        long long pos = (long long)(start >> 32);
+        unsigned e = (unsigned)(start & 0xffffffffull);
+        if (pos + nsp > a.wl.cap) {
+            atomicOr(a.err, 1);
+        } else {
+            const size_t fo = (size_t)b * a.wl.cap;
+            unsigned* cs = a.wl.cs + (size_t)b * a.wl.ncs;
+            const unsigned q = (unsigned)a.chunk;
+            for (int k = 0; k < a.K; k++) {
+                if ((mask >> k) & 1u) {
+                    a.wl.off[fo + pos] = e;
+                    a.wl.fbeg[fo + pos] = fb;
+                    a.wl.fdeg[fo + pos] = fd;
+                    a.wl.pbeg[fo + pos] = pb;
+                    a.wl.slot[fo + pos] = (slot0 + k) % nd;
+                    // chunks whose first edge lies in [e, e + d)
+                    for (unsigned c = (e + q - 1) / q; c <= (e + (unsigned)d - 1) / q; c++) cs[c] = (unsigned)pos;
+                    pos++;
+                    e += (unsigned)d;
                 }
             }
         }
@@ -257,80 +292,67 @@ __global__ void __launch_bounds__(1024) lif_update(UpdArgs a) {
 template <typename WT>
 struct PushArgs {
     const long long* clock;
-    int N, n_delay, blk, chunk, frac_bits;
-    const int* indptr;
+    int N, B, n_delay, blk, chunk, frac_bits;
     const int* col;
     const WT* w;
-    const int* pl_indptr;
     const int* pl_post;
     const float* pw;
     int n_pl;
     float pw_qscale;
     int* ring;
-    unsigned long long* ctr;
-    const int* ent_nrn;
-    const int* ent_meta;
-    const unsigned long long* ent_off;
-    long long cap;
+    WorkList wl;
 };
 
 __device__ __forceinline__ int wval(int16_t x, int s) { return (int)x << s; }
 __device__ __forceinline__ int wval(int32_t x, int) { return x; }
 
-// Scatter the block's spikes (flyzero/gpu.py:98-116) over equal edge chunks.
+// Scatter fly blockIdx.y's spikes of the block (flyzero/gpu.py:98-116), one
+// edge chunk per workgroup iteration.
 template <typename WT>
-__global__ void __launch_bounds__(PUSH_MAX_THREADS) lif_push(PushArgs<WT> a) {
-    __shared__ unsigned long long L_off[PUSH_MAX_THREADS];
-    __shared__ int L_fbeg[PUSH_MAX_THREADS];
-    __shared__ int L_fdeg[PUSH_MAX_THREADS];
-    __shared__ int L_pbeg[PUSH_MAX_THREADS];
-    __shared__ int L_meta[PUSH_MAX_THREADS];
-    __shared__ unsigned long long S_end;
+__global__ void __launch_bounds__(MAX_THREADS) lif_push(PushArgs<WT> a) {
+    __shared__ unsigned L_off[MAX_THREADS];
+    __shared__ int L_fbeg[MAX_THREADS];
+    __shared__ int L_fdeg[MAX_THREADS];
+    __shared__ int L_pbeg[MAX_THREADS];
+    __shared__ int L_slot[MAX_THREADS];
+    __shared__ unsigned S_end;
     const int nt = blockDim.x, tid = threadIdx.x;
+    const int b = blockIdx.y;
     const int par = (int)((a.clock[2] + a.blk) & 1);
-    const unsigned long long c = a.ctr[par];
-    if (blockIdx.x == 0 && tid == 0) a.ctr[par ^ 1] = 0;  // the list the next block fills
-    const long long nent = (long long)(c >> ENT_SHIFT);
-    const unsigned long long T = c & EDGE_MASK;
-    if (nent > a.cap) return;  // the update kernel flagged the overflow and wrote no entries
-    const size_t ring_fly = (size_t)a.n_delay * a.N;
+    const unsigned long long c = a.wl.ctr[(size_t)par * a.B + b];
+    if (blockIdx.x == 0 && tid == 0) a.wl.ctr[(size_t)(par ^ 1) * a.B + b] = 0;  // the list the next block fills
+    const long long nent = (long long)(c >> 32);
+    const unsigned T = (unsigned)(c & 0xffffffffull);
+    if (nent > a.wl.cap) return;  // the update kernel flagged the overflow and wrote no entries
+    const size_t fo = (size_t)b * a.wl.cap;
+    const unsigned* off = a.wl.off + fo;
+    const unsigned* cs = a.wl.cs + (size_t)b * a.wl.ncs;
+    int* ring = a.ring + (size_t)b * a.n_delay * a.N;
+    const float* pw = a.pw ? a.pw + (size_t)b * a.n_pl : nullptr;
+    const unsigned q = (unsigned)a.chunk;
+    const unsigned nch = (T + q - 1) / q;
 
-    for (unsigned long long e0 = (unsigned long long)blockIdx.x * a.chunk; e0 < T;
-         e0 += (unsigned long long)gridDim.x * a.chunk) {
-        const unsigned long long e1 = e0 + (unsigned long long)a.chunk < T ? e0 + a.chunk : T;
-        // last entry whose edge range starts at or before e0 (offsets ascend)
-        long long lo = 0, hi = nent;
-        while (hi - lo > 1) {
-            const long long m = hi - lo - 1;
-            auto posf = [&](long long t) -> long long {
-                return lo + 1 + (m >= nt ? (t * m) / nt : (t < m ? t : m - 1));
-            };
-            const int f = a.ent_off[posf(tid)] <= e0;
-            const int cnt = __syncthreads_count(f);
-            const long long nlo = cnt > 0 ? posf(cnt - 1) : lo;
-            const long long nhi = cnt < nt ? posf(cnt) : hi;
-            lo = nlo;
-            hi = nhi;
-        }
-        long long s = lo;
-        while (true) {
-            const int mc = (int)((nent - s) < nt ? (nent - s) : nt);
+    for (unsigned ch = blockIdx.x; ch < nch; ch += gridDim.x) {
+        const unsigned e0 = ch * q;
+        const unsigned e1 = (T - e0) < q ? T : e0 + q;
+        const long long s_lo = cs[ch];
+        const long long s_hi = (ch + 1 < nch) ? (long long)cs[ch + 1] : nent - 1;  // last entry touching the chunk
+        for (long long s = s_lo; s <= s_hi; s += nt) {
+            const int mc = (int)((s_hi - s + 1) < nt ? (s_hi - s + 1) : nt);
             if (tid < mc) {
-                const long long q = s + tid;
-                const int nrn = a.ent_nrn[q];
-                const int fb = a.indptr[nrn];
-                L_off[tid] = a.ent_off[q];
-                L_fbeg[tid] = fb;
-                L_fdeg[tid] = a.indptr[nrn + 1] - fb;
-                L_pbeg[tid] = a.pl_indptr ? a.pl_indptr[nrn] : 0;
-                L_meta[tid] = a.ent_meta[q];
+                const long long qq = s + tid;
+                L_off[tid] = off[qq];
+                L_fbeg[tid] = a.wl.fbeg[fo + qq];
+                L_fdeg[tid] = a.wl.fdeg[fo + qq];
+                L_pbeg[tid] = a.wl.pbeg[fo + qq];
+                L_slot[tid] = a.wl.slot[fo + qq];
             }
-            if (tid == 0) S_end = (s + mc < nent) ? a.ent_off[s + mc] : T;
+            if (tid == 0) S_end = (s + mc < nent) ? off[s + mc] : T;
             __syncthreads();
-            const unsigned long long st_end = S_end;
-            const unsigned long long lo_e = e0 > L_off[0] ? e0 : L_off[0];
-            const unsigned long long hi_e = e1 < st_end ? e1 : st_end;
-            for (unsigned long long e = lo_e + tid; e < hi_e; e += nt) {
+            const unsigned st_end = S_end;
+            const unsigned lo_e = e0 > L_off[0] ? e0 : L_off[0];
+            const unsigned hi_e = e1 < st_end ? e1 : st_end;
+            for (unsigned e = lo_e + tid; e < hi_e; e += nt) {
                 int jl = 0, jh = mc;
                 while (jh - jl > 1) {
                     const int mid = (jl + jh) >> 1;
@@ -338,8 +360,6 @@ __global__ void __launch_bounds__(PUSH_MAX_THREADS) lif_push(PushArgs<WT> a) {
                     else jh = mid;
                 }
                 const int rr = (int)(e - L_off[jl]);
-                const int meta = L_meta[jl];
-                const int b = meta >> 8, slot = meta & 255;
                 int post, val;
                 if (rr < L_fdeg[jl]) {
                     const int k = L_fbeg[jl] + rr;
@@ -348,15 +368,13 @@ __global__ void __launch_bounds__(PUSH_MAX_THREADS) lif_push(PushArgs<WT> a) {
                 } else {
                     const int k = L_pbeg[jl] + rr - L_fdeg[jl];
                     post = a.pl_post[k];
-                    val = (int)rintf(a.pw[(size_t)b * a.n_pl + k] * a.pw_qscale);
+                    val = (int)rintf(pw[k] * a.pw_qscale);
                 }
                 if (val != 0)
-                    __hip_atomic_fetch_add(a.ring + (size_t)b * ring_fly + (size_t)slot * a.N + post, val,
-                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_fetch_add(ring + (size_t)L_slot[jl] * a.N + post, val, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
             }
             __syncthreads();
-            if (st_end >= e1) break;
-            s += mc;
         }
     }
 }
@@ -385,11 +403,6 @@ __global__ void lif_scatter_rows(int n_list, const int* list, const int* indptr,
     }
 }
 
-__global__ void lif_degree(int n, const int* indptr, const int* pl_indptr, int* deg) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) deg[i] = indptr[i + 1] - indptr[i] + (pl_indptr ? pl_indptr[i + 1] - pl_indptr[i] : 0);
-}
-
 }  // namespace
 
 struct fz_engine {
@@ -403,14 +416,10 @@ struct fz_engine {
     int n_pl = 0;
     const int* pl_indptr = nullptr;
     const int* pl_post = nullptr;
-    int* deg = nullptr;
     fz_state st{};
     bool bound = false;
-    unsigned long long* ctr = nullptr;
-    int* ent_nrn = nullptr;
-    int* ent_meta = nullptr;
-    unsigned long long* ent_off = nullptr;
-    long long cap = 0;
+    WorkList wl{};
+    int wl_B = 0;  // flies the counters are sized for
     int* err = nullptr;
     int kmax = 1;
     int n_cu = 256;
@@ -430,13 +439,15 @@ struct Shape {
 int resolve(const fz_engine* e, const fz_tune* t, Shape* s) {
     s->ut = (t && t->upd_threads) ? t->upd_threads : 256;
     s->pt = (t && t->push_threads) ? t->push_threads : 256;
-    s->chunk = (t && t->push_chunk) ? t->push_chunk : 2048;
+    s->chunk = (t && t->push_chunk) ? t->push_chunk : 256;
     s->grid = (t && t->push_grid) ? t->push_grid : e->n_cu * 8;
     s->K = (t && t->block_steps) ? t->block_steps : e->kmax;
-    if (s->ut < 64 || s->ut > 1024 || s->ut % 64) return fail(FZ_E_ARG, "upd_threads %d: need a multiple of 64 in [64, 1024]", s->ut);
-    if (s->pt < 64 || s->pt > PUSH_MAX_THREADS || s->pt % 64)
-        return fail(FZ_E_ARG, "push_threads %d: need a multiple of 64 in [64, %d]", s->pt, PUSH_MAX_THREADS);
-    if (s->chunk < 1) return fail(FZ_E_ARG, "push_chunk %d < 1", s->chunk);
+    if (s->ut < 64 || s->ut > MAX_THREADS || s->ut % 64)
+        return fail(FZ_E_ARG, "upd_threads %d: need a multiple of 64 in [64, %d]", s->ut, MAX_THREADS);
+    if (s->pt < 64 || s->pt > MAX_THREADS || s->pt % 64)
+        return fail(FZ_E_ARG, "push_threads %d: need a multiple of 64 in [64, %d]", s->pt, MAX_THREADS);
+    if (s->chunk < QMIN || s->chunk > (1 << 20))
+        return fail(FZ_E_ARG, "push_chunk %d outside [%d, 2^20]", s->chunk, QMIN);
     if (s->grid < 1) return fail(FZ_E_ARG, "push_grid %d < 1", s->grid);
     if (s->K < 1 || s->K > e->kmax) return fail(FZ_E_ARG, "block_steps %d outside [1, %d]", s->K, e->kmax);
     return FZ_OK;
@@ -453,7 +464,7 @@ int check_inputs(const fz_engine* e, const fz_inputs* in) {
     return FZ_OK;
 }
 
-UpdArgs make_upd(const fz_engine* e, const fz_inputs* in, int K, int kb, int blk, int first,
+UpdArgs make_upd(const fz_engine* e, const fz_inputs* in, const Shape& s, int K, int kb, int blk, int first,
                  const unsigned* hits, unsigned* raster) {
     UpdArgs a{};
     a.clock = e->st.clock;
@@ -463,6 +474,7 @@ UpdArgs make_upd(const fz_engine* e, const fz_inputs* in, int K, int kb, int blk
     a.kb = kb;
     a.blk = blk;
     a.first = first;
+    a.chunk = s.chunk;
     a.u = e->st.u;
     a.g = e->st.g;
     a.ref = e->st.ref;
@@ -478,85 +490,90 @@ UpdArgs make_upd(const fz_engine* e, const fz_inputs* in, int K, int kb, int blk
     a.rw = (e->n + 31) / 32;
     a.seed = in->seed;
     a.alive = in->alive;
-    a.deg = e->deg;
-    a.ctr = e->ctr;
-    a.ent_nrn = e->ent_nrn;
-    a.ent_meta = e->ent_meta;
-    a.ent_off = e->ent_off;
-    a.cap = e->cap;
+    a.indptr = e->indptr;
+    a.pl_indptr = e->n_pl > 0 ? e->pl_indptr : nullptr;
+    a.wl = e->wl;
     a.err = e->err;
     a.c = e->c;
     return a;
 }
 
 template <typename WT>
-PushArgs<WT> make_push(const fz_engine* e, const fz_inputs* in, int blk, int chunk) {
+PushArgs<WT> make_push(const fz_engine* e, const fz_inputs* in, const Shape& s, int blk) {
     PushArgs<WT> a{};
     a.clock = e->st.clock;
     a.N = e->n;
+    a.B = e->st.B;
     a.n_delay = e->c.n_delay;
     a.blk = blk;
-    a.chunk = chunk;
+    a.chunk = s.chunk;
     a.frac_bits = e->c.frac_bits;
-    a.indptr = e->indptr;
     a.col = e->col;
     a.w = static_cast<const WT*>(e->w);
-    a.pl_indptr = e->n_pl > 0 ? e->pl_indptr : nullptr;
-    a.pl_post = e->pl_post;
-    a.pw = in->pw;
+    a.pl_post = e->n_pl > 0 ? e->pl_post : nullptr;
+    a.pw = e->n_pl > 0 ? in->pw : nullptr;
     a.n_pl = e->n_pl;
     a.pw_qscale = e->c.pw_qscale;
     a.ring = e->st.ring;
-    a.ctr = e->ctr;
-    a.ent_nrn = e->ent_nrn;
-    a.ent_meta = e->ent_meta;
-    a.ent_off = e->ent_off;
-    a.cap = e->cap;
+    a.wl = e->wl;
     return a;
 }
 
-// Enqueue one window.  With `ev` non-null, events bracket every kernel.
+dim3 push_grid(const fz_engine* e, const Shape& s) {
+    int gx = (s.grid + e->st.B - 1) / e->st.B;
+    if (gx < 1) gx = 1;
+    return dim3(gx, e->st.B);
+}
+
+// One window as a sequence of kernel launches (the graph builder mirrors it).
+// With `ev_*` non-null, events bracket every kernel.
 int enqueue(fz_engine* e, const fz_inputs* in, int steps, const unsigned* hits, unsigned* raster,
             const Shape& s, hipStream_t stream, std::vector<hipEvent_t>* ev_upd, std::vector<hipEvent_t>* ev_push) {
     const dim3 ugrid((e->n + s.ut - 1) / s.ut, e->st.B);
+    const dim3 pgrid = push_grid(e, s);
     int kb = 0, blk = 0;
+    auto mark = [&](std::vector<hipEvent_t>* ev) -> int {
+        if (!ev) return FZ_OK;
+        hipEvent_t x;
+        HIPCHK(hipEventCreate(&x));
+        ev->push_back(x);
+        HIPCHK(hipEventRecord(x, stream));
+        return FZ_OK;
+    };
+    int rc;
     while (kb < steps) {
         const int K = (steps - kb) < s.K ? (steps - kb) : s.K;
-        UpdArgs ua = make_upd(e, in, K, kb, blk, kb == 0, hits, raster);
-        if (ev_upd) {
-            hipEvent_t a0, a1;
-            HIPCHK(hipEventCreate(&a0));
-            HIPCHK(hipEventCreate(&a1));
-            ev_upd->push_back(a0);
-            ev_upd->push_back(a1);
-            HIPCHK(hipEventRecord(a0, stream));
-        }
+        UpdArgs ua = make_upd(e, in, s, K, kb, blk, kb == 0, hits, raster);
+        if ((rc = mark(ev_upd))) return rc;
         hipLaunchKernelGGL(lif_update, ugrid, dim3(s.ut), 0, stream, ua);
         HIPCHK(hipGetLastError());
-        if (ev_upd) HIPCHK(hipEventRecord(ev_upd->back(), stream));
-        if (ev_push) {
-            hipEvent_t a0, a1;
-            HIPCHK(hipEventCreate(&a0));
-            HIPCHK(hipEventCreate(&a1));
-            ev_push->push_back(a0);
-            ev_push->push_back(a1);
-            HIPCHK(hipEventRecord(a0, stream));
-        }
+        if ((rc = mark(ev_upd))) return rc;
+        if ((rc = mark(ev_push))) return rc;
         if (e->w_bits == 16) {
-            hipLaunchKernelGGL(lif_push<int16_t>, dim3(s.grid), dim3(s.pt), 0, stream,
-                               make_push<int16_t>(e, in, blk, s.chunk));
+            hipLaunchKernelGGL(lif_push<int16_t>, pgrid, dim3(s.pt), 0, stream, make_push<int16_t>(e, in, s, blk));
         } else {
-            hipLaunchKernelGGL(lif_push<int32_t>, dim3(s.grid), dim3(s.pt), 0, stream,
-                               make_push<int32_t>(e, in, blk, s.chunk));
+            hipLaunchKernelGGL(lif_push<int32_t>, pgrid, dim3(s.pt), 0, stream, make_push<int32_t>(e, in, s, blk));
         }
         HIPCHK(hipGetLastError());
-        if (ev_push) HIPCHK(hipEventRecord(ev_push->back(), stream));
+        if ((rc = mark(ev_push))) return rc;
         kb += K;
         blk++;
     }
     hipLaunchKernelGGL(lif_clock, dim3(1), dim3(64), 0, stream, e->st.clock, steps, blk);
     HIPCHK(hipGetLastError());
     return FZ_OK;
+}
+
+void free_worklist(fz_engine* e) {
+    (void)hipFree(e->wl.ctr);
+    (void)hipFree(e->wl.off);
+    (void)hipFree(e->wl.fbeg);
+    (void)hipFree(e->wl.fdeg);
+    (void)hipFree(e->wl.pbeg);
+    (void)hipFree(e->wl.slot);
+    (void)hipFree(e->wl.cs);
+    e->wl = WorkList{};
+    e->wl_B = 0;
 }
 
 }  // namespace
@@ -571,10 +588,10 @@ int fz_create(int n, long long nnz, const int* indptr, const int* col, const voi
               const fz_params* p, fz_engine** out) {
     if (!out) return fail(FZ_E_ARG, "null output handle");
     *out = nullptr;
-    if (n < 0 || n >= (1 << 30)) return fail(FZ_E_ARG, "n = %d out of range", n);
+    if (n < 1 || n >= (1 << 30)) return fail(FZ_E_ARG, "n = %d outside [1, 2^30)", n);
     if (nnz < 0 || nnz >= (1ll << 31)) return fail(FZ_E_ARG, "nnz = %lld out of int32 range", nnz);
     if (!p) return fail(FZ_E_ARG, "null params");
-    if (n > 0 && !indptr) return fail(FZ_E_ARG, "null indptr");
+    if (!indptr) return fail(FZ_E_ARG, "null indptr");
     if (nnz > 0 && (!col || !w)) return fail(FZ_E_ARG, "null col or w");
     if (w_bits != 16 && w_bits != 32) return fail(FZ_E_ARG, "w_bits must be 16 or 32, got %d", w_bits);
     if (frac_bits < 0 || frac_bits > 24) return fail(FZ_E_ARG, "frac_bits %d outside [0, 24]", frac_bits);
@@ -594,23 +611,9 @@ int fz_create(int n, long long nnz, const int* indptr, const int* col, const voi
     hipDeviceProp_t prop;
     if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
         e->n_cu = prop.multiProcessorCount;
-    if (hipMalloc(&e->deg, sizeof(int) * (size_t)(n > 0 ? n : 1)) != hipSuccess ||
-        hipMalloc(&e->ctr, sizeof(unsigned long long) * 2) != hipSuccess ||
-        hipMalloc(&e->err, sizeof(int)) != hipSuccess) {
+    if (hipMalloc(&e->err, sizeof(int)) != hipSuccess || hipMemset(e->err, 0, sizeof(int)) != hipSuccess) {
         fz_destroy(e);
-        return fail(FZ_E_HIP, "hipMalloc of engine tables failed");
-    }
-    if (hipMemset(e->ctr, 0, sizeof(unsigned long long) * 2) != hipSuccess ||
-        hipMemset(e->err, 0, sizeof(int)) != hipSuccess) {
-        fz_destroy(e);
-        return fail(FZ_E_HIP, "hipMemset of engine tables failed");
-    }
-    if (n > 0) {
-        hipLaunchKernelGGL(lif_degree, dim3((n + 255) / 256), dim3(256), 0, nullptr, n, indptr, nullptr, e->deg);
-        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
-            fz_destroy(e);
-            return fail(FZ_E_HIP, "degree table kernel failed");
-        }
+        return fail(FZ_E_HIP, "allocating the engine's error flag failed");
     }
     *out = e;
     return FZ_OK;
@@ -618,31 +621,23 @@ int fz_create(int n, long long nnz, const int* indptr, const int* col, const voi
 
 int fz_destroy(fz_engine* e) {
     if (!e) return FZ_OK;
-    hipFree(e->deg);
-    hipFree(e->ctr);
-    hipFree(e->err);
-    hipFree(e->ent_nrn);
-    hipFree(e->ent_meta);
-    hipFree(e->ent_off);
+    free_worklist(e);
+    (void)hipFree(e->err);
     delete e;
     return FZ_OK;
 }
 
 int fz_set_plastic(fz_engine* e, int n_pl, const int* pl_indptr, const int* pl_post, void* stream) {
+    (void)stream;
     if (!e) return fail(FZ_E_ARG, "null engine");
     if (n_pl < 0) return fail(FZ_E_ARG, "n_pl %d < 0", n_pl);
     if (n_pl > 0 && (!pl_indptr || !pl_post)) return fail(FZ_E_ARG, "n_pl > 0 needs pl_indptr and pl_post");
     e->n_pl = n_pl;
     e->pl_indptr = n_pl > 0 ? pl_indptr : nullptr;
     e->pl_post = n_pl > 0 ? pl_post : nullptr;
-    if (e->n > 0) {
-        hipStream_t s = static_cast<hipStream_t>(stream);
-        hipLaunchKernelGGL(lif_degree, dim3((e->n + 255) / 256), dim3(256), 0, s, e->n, e->indptr, e->pl_indptr,
-                           e->deg);
-        HIPCHK(hipGetLastError());
-    }
-    if (e->bound) {  // re-check capacity limits with the new degree
+    if (e->bound) {  // re-check capacity limits with the new edge count
         fz_state st = e->st;
+        e->bound = false;
         return fz_bind(e, &st);
     }
     return FZ_OK;
@@ -651,28 +646,30 @@ int fz_set_plastic(fz_engine* e, int n_pl, const int* pl_indptr, const int* pl_p
 int fz_bind(fz_engine* e, const fz_state* st) {
     if (!e || !st) return fail(FZ_E_ARG, "null engine or state");
     if (st->B < 1 || st->B > 65535) return fail(FZ_E_ARG, "B = %d outside [1, 65535]", st->B);
-    if (st->B >= (1 << 23)) return fail(FZ_E_ARG, "B too large for the work-list encoding");
     if (!st->u || !st->g || !st->ref || !st->ring || !st->counts || !st->clock)
         return fail(FZ_E_ARG, "null state array");
     const int per = e->c.n_ref > 1 ? e->c.n_ref : 1;
     const long long maxspk = (e->kmax + per - 1) / per;  // spikes per neuron per block
-    const long long cap = (long long)st->B * e->n * maxspk;
-    if (cap >= ENT_LIMIT) return fail(FZ_E_ARG, "B*n too large for one work list (%lld entries)", cap);
-    const double edges = (double)st->B * (double)(e->nnz + e->n_pl) * (double)maxspk;
-    if (edges >= (double)EDGE_MASK) return fail(FZ_E_ARG, "B*nnz too large for one work list");
-    if (cap > e->cap) {
-        hipFree(e->ent_nrn);
-        hipFree(e->ent_meta);
-        hipFree(e->ent_off);
-        e->ent_nrn = nullptr;
-        e->ent_meta = nullptr;
-        e->ent_off = nullptr;
-        e->cap = 0;
-        const size_t c1 = (size_t)(cap > 0 ? cap : 1);
-        HIPCHK(hipMalloc(&e->ent_nrn, sizeof(int) * c1));
-        HIPCHK(hipMalloc(&e->ent_meta, sizeof(int) * c1));
-        HIPCHK(hipMalloc(&e->ent_off, sizeof(unsigned long long) * c1));
-        e->cap = cap;
+    const long long cap = (long long)e->n * maxspk;       // entries per fly
+    const long long edges = (e->nnz + e->n_pl) * maxspk;  // edges per fly and block, at most
+    if (cap >= (1ll << 31)) return fail(FZ_E_ARG, "n too large for the per-fly work list");
+    if (edges >= (1ll << 31)) return fail(FZ_E_ARG, "nnz too large for the per-fly work list");
+    const long long ncs = edges / QMIN + 2;
+    if (st->B != e->wl_B || cap != e->wl.cap || ncs != e->wl.ncs) {
+        free_worklist(e);
+        const size_t ne = (size_t)st->B * (size_t)cap;
+        WorkList& w = e->wl;
+        HIPCHK(hipMalloc(&w.ctr, sizeof(unsigned long long) * 2 * (size_t)st->B));
+        HIPCHK(hipMemset(w.ctr, 0, sizeof(unsigned long long) * 2 * (size_t)st->B));
+        HIPCHK(hipMalloc(&w.off, sizeof(unsigned) * ne));
+        HIPCHK(hipMalloc(&w.fbeg, sizeof(int) * ne));
+        HIPCHK(hipMalloc(&w.fdeg, sizeof(int) * ne));
+        HIPCHK(hipMalloc(&w.pbeg, sizeof(int) * ne));
+        HIPCHK(hipMalloc(&w.slot, sizeof(int) * ne));
+        HIPCHK(hipMalloc(&w.cs, sizeof(unsigned) * (size_t)st->B * (size_t)ncs));
+        w.cap = cap;
+        w.ncs = ncs;
+        e->wl_B = st->B;
     }
     e->st = *st;
     e->bound = true;
@@ -714,8 +711,8 @@ int fz_run_timed(fz_engine* e, const fz_inputs* in, int steps, const fz_tune* tu
             tp += ms;
         }
     }
-    for (auto ev : eu) hipEventDestroy(ev);
-    for (auto ev : ep) hipEventDestroy(ev);
+    for (auto ev : eu) (void)hipEventDestroy(ev);
+    for (auto ev : ep) (void)hipEventDestroy(ev);
     if (upd_ms) *upd_ms = tu;
     if (push_ms) *push_ms = tp;
     if (upd_launches) *upd_launches = (int)(eu.size() / 2);
@@ -733,16 +730,15 @@ int fz_graph_create(fz_engine* e, const fz_inputs* in, int steps, const fz_tune*
     if ((rc = resolve(e, tune, &s))) return rc;
     fz_graph* gr = new fz_graph();
     auto bail = [&](const char* what, hipError_t err) {
-        if (gr->graph) hipGraphDestroy(gr->graph);
+        if (gr->graph) (void)hipGraphDestroy(gr->graph);
         delete gr;
         return fail(FZ_E_HIP, "%s failed: %s", what, hipGetErrorString(err));
     };
     hipError_t err = hipGraphCreate(&gr->graph, 0);
     if (err != hipSuccess) return bail("hipGraphCreate", err);
     hipGraphNode_t prev = nullptr;
-    auto add = [&](void* fn, dim3 grid, dim3 block, void* arg) -> hipError_t {
+    auto add = [&](void* fn, dim3 grid, dim3 block, void** args) -> hipError_t {
         hipKernelNodeParams kp{};
-        void* args[] = {arg};
         kp.func = fn;
         kp.gridDim = grid;
         kp.blockDim = block;
@@ -755,18 +751,22 @@ int fz_graph_create(fz_engine* e, const fz_inputs* in, int steps, const fz_tune*
         return r;
     };
     const dim3 ugrid((e->n + s.ut - 1) / s.ut, e->st.B);
+    const dim3 pgrid = push_grid(e, s);
     int kb = 0, blk = 0;
     while (kb < steps) {
         const int K = (steps - kb) < s.K ? (steps - kb) : s.K;
-        UpdArgs ua = make_upd(e, in, K, kb, blk, kb == 0, nullptr, nullptr);
-        if ((err = add(reinterpret_cast<void*>(&lif_update), ugrid, dim3(s.ut), &ua)) != hipSuccess)
+        UpdArgs ua = make_upd(e, in, s, K, kb, blk, kb == 0, nullptr, nullptr);
+        void* uargs[] = {&ua};
+        if ((err = add(reinterpret_cast<void*>(&lif_update), ugrid, dim3(s.ut), uargs)) != hipSuccess)
             return bail("hipGraphAddKernelNode(update)", err);
         if (e->w_bits == 16) {
-            PushArgs<int16_t> pa = make_push<int16_t>(e, in, blk, s.chunk);
-            err = add(reinterpret_cast<void*>(&lif_push<int16_t>), dim3(s.grid), dim3(s.pt), &pa);
+            PushArgs<int16_t> pa = make_push<int16_t>(e, in, s, blk);
+            void* pargs[] = {&pa};
+            err = add(reinterpret_cast<void*>(&lif_push<int16_t>), pgrid, dim3(s.pt), pargs);
         } else {
-            PushArgs<int32_t> pa = make_push<int32_t>(e, in, blk, s.chunk);
-            err = add(reinterpret_cast<void*>(&lif_push<int32_t>), dim3(s.grid), dim3(s.pt), &pa);
+            PushArgs<int32_t> pa = make_push<int32_t>(e, in, s, blk);
+            void* pargs[] = {&pa};
+            err = add(reinterpret_cast<void*>(&lif_push<int32_t>), pgrid, dim3(s.pt), pargs);
         }
         if (err != hipSuccess) return bail("hipGraphAddKernelNode(push)", err);
         kb += K;
@@ -774,17 +774,9 @@ int fz_graph_create(fz_engine* e, const fz_inputs* in, int steps, const fz_tune*
     }
     long long* clock = e->st.clock;
     int st = steps, nb = blk;
-    {
-        hipKernelNodeParams kp{};
-        void* args[] = {&clock, &st, &nb};
-        kp.func = reinterpret_cast<void*>(&lif_clock);
-        kp.gridDim = dim3(1);
-        kp.blockDim = dim3(64);
-        kp.kernelParams = args;
-        hipGraphNode_t node;
-        err = hipGraphAddKernelNode(&node, gr->graph, prev ? &prev : nullptr, prev ? 1 : 0, &kp);
-        if (err != hipSuccess) return bail("hipGraphAddKernelNode(clock)", err);
-    }
+    void* cargs[] = {&clock, &st, &nb};
+    if ((err = add(reinterpret_cast<void*>(&lif_clock), dim3(1), dim3(64), cargs)) != hipSuccess)
+        return bail("hipGraphAddKernelNode(clock)", err);
     err = hipGraphInstantiate(&gr->exec, gr->graph, nullptr, nullptr, 0);
     if (err != hipSuccess) return bail("hipGraphInstantiate", err);
     *out = gr;
@@ -799,8 +791,8 @@ int fz_graph_launch(fz_graph* gr, void* stream) {
 
 int fz_graph_destroy(fz_graph* gr) {
     if (!gr) return FZ_OK;
-    if (gr->exec) hipGraphExecDestroy(gr->exec);
-    if (gr->graph) hipGraphDestroy(gr->graph);
+    if (gr->exec) (void)hipGraphExecDestroy(gr->exec);
+    if (gr->graph) (void)hipGraphDestroy(gr->graph);
     delete gr;
     return FZ_OK;
 }

@@ -1,0 +1,109 @@
+"""Launch-shape autotuner for the LIF engine.
+
+Times candidate ``fz_tune`` shapes (update workgroup size, push workgroup size, push edge
+chunk, push grid) with HIP events on real windows of the caller's own workload, then restores
+the brain's state, so tuning never advances the simulation.  Search is coordinate descent:
+the update kernel's shape first, then the push kernel's, each over a small grid of shapes that
+respect the kernels' limits (multiples of 64 threads, chunk >= 64 edges).  Results are cached
+per (device, neurons, connections, flies, plastic synapses) in memory and in a JSON file.
+"""
+
+from __future__ import annotations
+
+import json
+import os
+
+import torch
+
+from . import _lib
+
+CACHE = os.path.join(os.path.expanduser("~"), ".cache", "flyhip", "autotune.json")
+_mem: dict = {}
+
+UPD_THREADS = (256, 512, 1024)
+PUSH_THREADS = (64, 128, 256, 512)
+PUSH_CHUNK = (64, 128, 256, 512, 1024)
+PUSH_GRID_PER_CU = (2, 4, 8, 16)
+
+
+def _key(bb) -> str:
+    name = torch.cuda.get_device_name(bb.device)
+    return f"{name}|n={bb.n}|nnz={int(bb.d_col.numel())}|B={bb.B}|npl={len(bb.plastic_pos)}"
+
+
+def _snapshot(bb):
+    return [t.clone() for t in (bb.u, bb.g, bb.ref, bb.ring, bb.counts, bb.clock)], (bb._t, bb._step_no)
+
+
+def _restore(bb, snap):
+    tensors, (t, step_no) = snap
+    for dst, src in zip((bb.u, bb.g, bb.ref, bb.ring, bb.counts), tensors):
+        dst.copy_(src)
+    # clock[2] (block parity of the engine's work-list counters) keeps running: only the
+    # simulated time and the RNG step go back
+    bb.clock[0:2].copy_(tensors[5][0:2])
+    bb._t, bb._step_no = t, step_no
+
+
+def _time(bb, shape: dict, steps: int, reps: int, snap) -> float:
+    bb.set_tune(**shape)
+    best = float("inf")
+    for _ in range(reps):
+        _restore(bb, snap)
+        r = bb.run_timed(steps)
+        best = min(best, r["update_ms"] + r["push_ms"])
+    return best
+
+
+def tune(bb, duration_ms: float = 1000.0 / 60.0988, reps: int = 2, use_cache: bool = True,
+         verbose: bool = False) -> dict:
+    """Pick and apply the fastest launch shape for ``bb``'s current workload.  Returns it."""
+    key = _key(bb)
+    if use_cache:
+        hit = _mem.get(key)
+        if hit is None and os.path.exists(CACHE):
+            try:
+                hit = json.load(open(CACHE)).get(key)
+            except (OSError, ValueError):
+                hit = None
+        if hit is not None:
+            bb.set_tune(**hit)
+            return hit
+    steps = bb.steps_of(duration_ms)
+    n_cu = torch.cuda.get_device_properties(bb.device).multi_processor_count
+    snap = _snapshot(bb)
+    # warm-up window (first launches pay code-object loading)
+    _time(bb, {}, steps, 1, snap)
+    cur = {"upd_threads": 256, "push_threads": 256, "push_chunk": 256, "push_grid": n_cu * 8}
+    best_t = _time(bb, cur, steps, reps, snap)
+    space = [("upd_threads", UPD_THREADS), ("push_chunk", PUSH_CHUNK), ("push_threads", PUSH_THREADS),
+             ("push_grid", tuple(n_cu * k for k in PUSH_GRID_PER_CU))]
+    for name, values in space:
+        for v in values:
+            if v == cur[name]:
+                continue
+            trial = dict(cur, **{name: v})
+            t = _time(bb, trial, steps, reps, snap)
+            if verbose:
+                print(f"autotune {trial}: {t * 1000:.1f} us", flush=True)
+            if t < best_t:
+                best_t, cur = t, trial
+    _restore(bb, snap)
+    bb.set_tune(**cur)
+    cur = dict(cur)
+    _mem[key] = cur
+    if use_cache:
+        try:
+            os.makedirs(os.path.dirname(CACHE), exist_ok=True)
+            data = json.load(open(CACHE)) if os.path.exists(CACHE) else {}
+            data[key] = cur
+            json.dump(data, open(CACHE, "w"), indent=1)
+        except (OSError, ValueError):
+            pass
+    if verbose:
+        print(f"autotune picked {cur}: {best_t * 1000:.1f} us per window", flush=True)
+    return cur
+
+
+def current(bb) -> dict:
+    return {f: getattr(bb.tune, f) for f, _ in _lib.FzTune._fields_}

@@ -226,3 +226,25 @@ def test_brain_weight_edits_reach_the_engine():
     b.reset()
     b.set_input([0], 200.0)
     assert b.run(100)[1] == 0
+
+
+def test_autotuned_shapes_keep_bit_exact_results(small_fly):
+    from flyhip import autotune
+
+    gb, ob = _pair(small_fly, batch=2, seed=4)
+    shape = autotune.tune(gb, use_cache=False, reps=1)
+    assert shape["upd_threads"] % 64 == 0 and shape["push_chunk"] >= 64
+    for _ in range(3):
+        np.testing.assert_array_equal(_np(gb.run(FRAME_MS)), ob.run(FRAME_MS))
+    _assert_state_equal(gb, ob)
+
+
+@pytest.mark.parametrize("shape", [dict(upd_threads=1024, push_threads=64, push_chunk=64, push_grid=7),
+                                   dict(upd_threads=64, push_threads=1024, push_chunk=4096, push_grid=3000,
+                                        block_steps=5)])
+def test_extreme_launch_shapes_are_exact(small_fly, shape):
+    gb, ob = _pair(small_fly, batch=3, seed=6)
+    gb.set_tune(**shape)
+    for _ in range(2):
+        np.testing.assert_array_equal(_np(gb.run(FRAME_MS)), ob.run(FRAME_MS))
+    _assert_state_equal(gb, ob)
