@@ -36,13 +36,14 @@
 #include <string>
 #include <vector>
 
+#include "errors.h"
 #include "flyhip.h"
 
 namespace {
-
 thread_local std::string g_err;
+}  // namespace
 
-int fail(int code, const char* fmt, ...) {
+int fzi_fail(int code, const char* fmt, ...) {
     char buf[512];
     va_list ap;
     va_start(ap, fmt);
@@ -51,6 +52,10 @@ int fail(int code, const char* fmt, ...) {
     g_err = buf;
     return code;
 }
+
+namespace {
+
+#define fail fzi_fail
 
 #define HIPCHK(x)                                                                              \
     do {                                                                                       \

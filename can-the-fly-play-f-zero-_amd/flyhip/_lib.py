@@ -63,6 +63,9 @@ SIGNATURES = {
     "fz_graph_destroy": (c_int, [c_void_p]),
     "fz_propagate": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     "fz_check": (c_int, [c_void_p, c_void_p]),
+    "fz_reward_step": (c_int, [c_void_p, c_void_p]),
+    "fz_dn_rates": (c_int, [c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, ctypes.c_float, ctypes.c_float,
+                            c_void_p, c_void_p]),
 }
 
 _lib = None

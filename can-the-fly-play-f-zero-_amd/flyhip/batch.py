@@ -1,0 +1,171 @@
+"""Batched learning rule and readout on a ``BatchBrain``: drop-ins for flyzero/batch.py.
+
+* ``BatchReward``: the exploratory-Hebbian reward rule (flyzero/learning.py:89-131, batched as
+  flyzero/batch.py:454-495 with the per-fly averaging of batch.py:141-172), one fused HIP kernel
+  per frame (``fz_reward_step``) that reads the window's spike counts on the device.  The rule
+  runs in float64 with weights in synapse units; the simulation uses the integer synapse count
+  ``rint(w)`` (``pw = count * w_syn``), so the counts the brain sees change in whole synapses.
+* ``DNReadout``: smoothed descending-neuron rates for the 8 groups of flyzero/instruct.py:43-48
+  (the rate half of ``BatchMotor.update``, flyzero/batch.py:107-111), on the device.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+
+
+@dataclass
+class LearnParams:
+    """The rule's fields of flyzero/learning.py:36-53 (same names and defaults), plus
+    ``w_abs_max``: an absolute cap in synapses (inf: only the reference's cap applies)."""
+    eta: float = 2e-4             # mV per unit of (reward x eligibility)
+    tau_elig_ms: float = 500.0
+    tau_post_ms: float = 500.0
+    tau_reward_ms: float = 1000.0
+    cap: float = 3.0              # |w| <= cap * max(|w0|, one synapse)
+    noise_hz: float = 20.0
+    noise_tau_ms: float = 300.0
+    targets: tuple = ("DNa02", "DNa01", "DNg02*", "DNp09", "MDN")
+    w_abs_max: float = float("inf")
+
+
+class FzRewardArgs(ctypes.Structure):
+    _fields_ = [("B", ctypes.c_int), ("n", ctypes.c_int), ("n_pl", ctypes.c_int), ("n_post", ctypes.c_int),
+                ("n_flies", ctypes.c_int), ("any_learn", ctypes.c_int)] + \
+               [(f, ctypes.c_void_p) for f in ("counts", "pre", "post", "post_k", "upost", "sign", "wmax", "elig",
+                                               "w", "count", "pw", "post_mean", "r_mean", "reward", "learn",
+                                               "fly_ptr", "fly_slots")] + \
+               [("ae", ctypes.c_double), ("ap", ctypes.c_double), ("ar", ctypes.c_double), ("eta", ctypes.c_double),
+                ("w_syn", ctypes.c_float), ("adv_out", ctypes.c_void_p)]
+
+
+def rule_constants(p: LearnParams, window_ms: float, w_syn: float) -> dict:
+    """flyzero/batch.py:484-486; eta converted from mV to synapses."""
+    return {"ae": float(np.exp(-window_ms / p.tau_elig_ms)), "ap": float(1 - np.exp(-window_ms / p.tau_post_ms)),
+            "ar": float(1 - np.exp(-window_ms / p.tau_reward_ms)),
+            "eta": float(p.eta) / float(np.float32(w_syn))}
+
+
+def fly_csr(fly_of_slot: np.ndarray):
+    fos = np.asarray(fly_of_slot, np.int64)
+    n_flies = int(fos.max()) + 1
+    order = np.argsort(fos, kind="stable")            # ascending slots within each fly
+    ptr = np.concatenate([[0], np.cumsum(np.bincount(fos, minlength=n_flies))]).astype(np.int32)
+    return n_flies, ptr, order.astype(np.int32)
+
+
+class BatchReward:
+    """Exploratory-Hebbian reward learning on the brain's plastic synapses
+    (flyzero/batch.py:454-495; the rule of flyzero/learning.py:116-131)."""
+
+    def __init__(self, brain, conn, params: LearnParams | None = None, fly_of_slot=None, seed: int = 0):
+        self.p = p = params or LearnParams()
+        self.brain = brain
+        self.B = brain.B
+        dev = brain.device
+        self.rng = np.random.default_rng(seed)
+        self.targets = np.unique(np.concatenate([conn.find(t) for t in p.targets]))
+        self.upre, pre_inv = np.unique(brain.pl_pre, return_inverse=True)
+        self.upost, post_inv = np.unique(brain.pl_post, return_inverse=True)
+        self.pre_k, self.post_k_np = pre_inv, post_inv
+        w0 = np.asarray(brain.w0_syn, np.float64)
+        self.sign_np = np.sign(w0).astype(np.int8)
+        self.wmax_np = np.minimum(p.cap * np.maximum(np.abs(w0), 1.0), p.w_abs_max)
+        fos = np.arange(self.B) if fly_of_slot is None else np.asarray(fly_of_slot)
+        self.fly_of_slot = fos
+        self.n_flies, fptr, fslots = fly_csr(fos)
+        t = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a).astype(dt)).to(dev)  # noqa: E731
+        self.d_pre = t(brain.pl_pre, np.int32)
+        self.d_post = t(brain.pl_post, np.int32)
+        self.d_post_k = t(post_inv, np.int32)
+        self.d_upost = t(self.upost, np.int32)
+        self.d_sign = t(self.sign_np, np.int8)
+        self.d_wmax = t(self.wmax_np, np.float64)
+        self.d_fly_ptr = t(fptr, np.int32)
+        self.d_fly_slots = t(fslots, np.int32)
+        n_pl = len(brain.plastic_pos)
+        self.elig = torch.zeros((self.B, n_pl), dtype=torch.float32, device=dev)
+        self.w = t(np.tile(w0, (self.B, 1)), np.float64)
+        self.count = torch.round(self.w).to(torch.int32)
+        self.post_mean = torch.zeros((self.B, len(self.upost)), dtype=torch.float64, device=dev)
+        self.r_mean = torch.zeros(self.B, dtype=torch.float64, device=dev)
+        self.noise = np.zeros((self.B, len(self.targets)))
+        self._reward = torch.zeros(self.B, dtype=torch.float64, device=dev)
+        self._learn = torch.zeros(self.B, dtype=torch.uint8, device=dev)
+        self._adv = torch.zeros(self.B, dtype=torch.float64, device=dev)
+
+    def reset(self, slots):
+        s = np.atleast_1d(slots)
+        self.elig[torch.as_tensor(s, device=self.brain.device)] = 0
+        self.noise[s] = 0
+
+    def exploration(self, window_ms: float) -> np.ndarray:
+        """Extra Poisson rates (slots, len(self.targets)) for the target neurons (batch.py:476-480)."""
+        a = window_ms / self.p.noise_tau_ms
+        self.noise += -a * self.noise + np.sqrt(2 * a) * self.rng.standard_normal(self.noise.shape)
+        return np.maximum(self.p.noise_hz * self.noise, 0.0)
+
+    def step(self, counts: torch.Tensor, reward, learn, window_ms: float) -> np.ndarray:
+        """One frame: traces, means and (if any slot learns) weights.  ``counts``: the window's
+        (B, n) int32 device counts; ``reward``, ``learn``: per slot.  Returns the advantage."""
+        brain = self.brain
+        if counts.dtype != torch.int32 or not counts.is_contiguous() or tuple(counts.shape) != (self.B, brain.n):
+            raise ValueError("counts must be the window's contiguous int32 (B, n) device counts")
+        learn = np.broadcast_to(np.asarray(learn, bool), (self.B,))
+        self._reward.copy_(torch.from_numpy(np.broadcast_to(np.asarray(reward, np.float64), (self.B,)).copy()))
+        self._learn.copy_(torch.from_numpy(learn.astype(np.uint8)))
+        c = rule_constants(self.p, window_ms, brain.p.w_syn)
+        if not brain.pw.is_contiguous() or brain.pw.dtype != torch.float32:
+            raise ValueError("brain.pw must be contiguous float32")
+        P = _lib.ptr
+        a = FzRewardArgs(self.B, brain.n, len(brain.plastic_pos), len(self.upost), self.n_flies, int(learn.any()),
+                         counts.data_ptr(), P(self.d_pre), P(self.d_post), P(self.d_post_k), P(self.d_upost),
+                         P(self.d_sign), P(self.d_wmax), P(self.elig), P(self.w), P(self.count), P(brain.pw),
+                         P(self.post_mean), P(self.r_mean), P(self._reward), P(self._learn), P(self.d_fly_ptr),
+                         P(self.d_fly_slots), c["ae"], c["ap"], c["ar"], c["eta"], np.float32(brain.p.w_syn),
+                         P(self._adv))
+        _lib.check(_lib.load().fz_reward_step(ctypes.byref(a), _lib.stream_handle(brain.device)), "fz_reward_step")
+        return self._adv.cpu().numpy()
+
+    def drift(self) -> np.ndarray:
+        """Mean relative change from the initial weights, per slot (flyzero/batch.py:221-223)."""
+        w0 = torch.from_numpy(np.asarray(self.brain.w0_syn, np.float64)).to(self.w.device)
+        return (torch.abs(self.w - w0) / self.d_wmax).mean(1).cpu().numpy()
+
+
+class DNReadout:
+    """Smoothed rates (Hz) of the descending-neuron groups (flyzero/batch.py:107-111), on the GPU."""
+
+    def __init__(self, conn, batch: int, groups: dict | None = None, tau_ms: float = 80.0, device=None):
+        from .connectome import GROUPS
+
+        self.groups = dict(GROUPS if groups is None else groups)
+        self.names = list(self.groups)
+        members = [np.asarray(conn.find(t, s), np.int64) for t, s in self.groups.values()]
+        ptr = np.concatenate([[0], np.cumsum([len(m) for m in members])]).astype(np.int32)
+        dev = torch.device(device if device is not None else "cuda")
+        self.d_ptr = torch.from_numpy(ptr).to(dev)
+        self.d_idx = torch.from_numpy(np.concatenate(members).astype(np.int32) if len(members) else
+                                      np.zeros(0, np.int32)).to(dev)
+        self.B, self.tau_ms, self.n = batch, tau_ms, conn.n
+        self.rates = torch.zeros((batch, len(self.names)), dtype=torch.float32, device=dev)
+
+    def reset(self, slots=None):
+        if slots is None:
+            self.rates.zero_()
+        else:
+            self.rates[torch.as_tensor(np.atleast_1d(slots), device=self.rates.device)] = 0
+
+    def update(self, counts: torch.Tensor, window_ms: float) -> torch.Tensor:
+        alpha = np.float32(1.0 - np.exp(-window_ms / self.tau_ms))
+        hz = np.float32(1000.0 / window_ms)
+        _lib.check(_lib.load().fz_dn_rates(self.B, self.n, counts.data_ptr(), len(self.names), _lib.ptr(self.d_ptr),
+                                           _lib.ptr(self.d_idx), hz, alpha, self.rates.data_ptr(),
+                                           _lib.stream_handle(self.rates.device)), "fz_dn_rates")
+        return self.rates

@@ -66,6 +66,7 @@ class BatchBrain:
         self.pl_pre = rows[pos]
         self.pl_post = self.indices[pos]
         self.w0 = self.data[pos].copy()
+        self.w0_syn = count[pos].copy()          # the same weights in synapses
 
         c = derived(p)
         self.n_delay, self.n_ref = c["n_delay"], c["n_ref"]

@@ -119,6 +119,50 @@ int fz_graph_destroy(fz_graph* gr);
  * (`Brain._propagate`) as an integer sum.  acc is int32[n]. */
 int fz_propagate(fz_engine* e, int n_list, const int* list, int* acc, void* stream);
 
+/* ---- plasticity and readout (stand-alone: they act on the counts of a window) ---- */
+
+/* One frame of reward-modulated plasticity on the plastic synapses (the exploratory-Hebbian
+ * rule of flyzero/learning.py:116-131, batched as flyzero/batch.py:482-495 with the per-fly
+ * averaging of batch.py:158-172).  Float64 throughout, weights in synapse units; the simulation
+ * uses count = rint(w) and pw = count * w_syn.  Every array is a device pointer. */
+typedef struct fz_reward_args {
+    int B;                      /* slots (simulated flies) */
+    int n;                      /* neurons */
+    int n_pl;                   /* plastic synapses */
+    int n_post;                 /* distinct post neurons of plastic synapses */
+    int n_flies;                /* learners; a learner averages its slots' changes */
+    int any_learn;              /* 0: traces and means only (flyzero/batch.py:493) */
+    const int* counts;          /* [B][n] spike counts of the window */
+    const int* pre;             /* [n_pl] pre neuron */
+    const int* post;            /* [n_pl] post neuron */
+    const int* post_k;          /* [n_pl] index of post in upost */
+    const int* upost;           /* [n_post] */
+    const signed char* sign;    /* [n_pl] Dale sign, +1 / -1 */
+    const double* wmax;         /* [n_pl] cap, synapses */
+    float* elig;                /* [B][n_pl] eligibility trace */
+    double* w;                  /* [B][n_pl] weight, synapses */
+    int* count;                 /* [B][n_pl] rint(w) */
+    float* pw;                  /* [B][n_pl] count * w_syn, mV (the engine's plastic weights) */
+    double* post_mean;          /* [B][n_post] running mean of post counts */
+    double* r_mean;             /* [B] running mean reward */
+    const double* reward;       /* [B] this frame's reward */
+    const unsigned char* learn; /* [B] slot learns this frame */
+    const int* fly_ptr;         /* [n_flies + 1] slots of each learner ... */
+    const int* fly_slots;       /* ... ascending slot indices */
+    double ae, ap, ar;          /* exp(-w/tau_e), 1-exp(-w/tau_post), 1-exp(-w/tau_R) */
+    double eta;                 /* synapses per unit of (reward x eligibility) */
+    float w_syn;                /* mV per synapse */
+    double* adv_out;            /* [B] advantage R - <R> before the update */
+} fz_reward_args;
+
+int fz_reward_step(const fz_reward_args* a, void* stream);
+
+/* rates[b][g] += alpha * (mean count of group g * hz_scale - rates[b][g]) for the
+ * descending-neuron groups (flyzero/batch.py:107-111; groups flyzero/instruct.py:43-48).
+ * Group g's neurons are gidx[gptr[g] .. gptr[g+1]). */
+int fz_dn_rates(int B, int n, const int* counts, int n_groups, const int* gptr, const int* gidx, float hz_scale,
+                float alpha, float* rates, void* stream);
+
 /* Reads (and clears) the device-side error flag; waits for `stream`. */
 int fz_check(fz_engine* e, void* stream);
 
