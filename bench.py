@@ -1,17 +1,18 @@
 """Benchmark: fly-sim-seconds per wall-second of the whole-brain LIF step on one MI355X.
 
 One bench "step" is one game frame: 166 LIF steps (dt = 0.1 ms, 16.6 ms of brain time) for every
-fly, with the frame's Poisson rates resampled from a smooth retinotopic field (the N=1 workload
-is the single-fly 10,000-step run on the seeded FlyWire-shaped connectome: N = 138,639 neurons,
-E = 15.1 M connections).  The frame is one HIP graph replay; the rates of every frame are staged
-in HBM before the timed region and copied device-to-device into the input slots each frame.
+fly, with the frame's Poisson rates resampled from a smooth retinotopic field.  The N=1 workload
+is the single-fly run on the seeded FlyWire-shaped connectome (N = 138,639 neurons, E = 15.1 M
+connections).  A frame is one HIP graph replay; the rates of every frame are staged in HBM before
+the timed region and copied device-to-device into the input slots each frame.
 
-Also reported:
-* ``roofline``: the dominant kernel's algorithmic bytes per launch / its mean launch time (HIP
-  events on the engine's stream), against the MI355X HBM peak;
+Each engine configuration (push engine; tile engine with several tile sizes) is autotuned and
+timed; ``value`` is the best at B = 1.  Also reported:
+* ``roofline``: the dominant kernel of the best B = 1 configuration: algorithmic bytes per
+  launch / its mean launch time (HIP events on the engine's stream), against the HBM peak;
 * ``cpu_baseline``: the CPU oracle (a C restatement of the reference's time loop, one thread)
   on a bounded sample of the same workload on this host;
-* ``batched``: the same metric for B = 256 flies (per-fly drive and visual gain) on one GPU.
+* ``batched``: the same metric for B = 256 flies (per-fly drive and visual gain).
 """
 
 from __future__ import annotations
@@ -50,29 +51,63 @@ def frame_rates(fly, frames: int, gain=None, drive=None) -> np.ndarray:
     return np.minimum(out, 1e4).astype(np.float32)
 
 
-def run_workload(bb, fly, rates_dev, frames, warmup, torch):
-    import flyhip  # noqa: F401
+def measure_line(fly, p, B, engine, tile, frames, warmup, torch, gain=None, drive=None, seed=0,
+                 kernel_frames=0):
+    """Build, autotune and time one engine configuration on ``frames`` game frames."""
+    from flyhip import BatchBrain, autotune
 
+    bb = BatchBrain(fly.weights, p, batch=B, seed=seed, engine=engine, tile=tile)
     idx = np.r_[fly.input_idx, fly.drive_idx]
     j = bb.slots(idx)
-    dt_scale = np.float32(bb.p.dt / 1000.0)
-    probs = rates_dev * float(dt_scale)                   # [F, B, n_in+1] per-step probability
-    total = warmup + frames
-    assert probs.shape[0] >= total
-
-    def one(f):
-        bb.p_ext[:, j] = probs[f]
-        return bb.run(FRAME_MS, copy=False)
-
+    probs = torch.from_numpy(frame_rates(fly, warmup + frames, gain, drive)).cuda() * float(np.float32(p.dt / 1000.0))
+    bb.p_ext[:, j] = probs[0]
+    shape = autotune.tune(bb, use_cache=False, reps=2 if B <= 16 else 1)
     for f in range(warmup):
-        one(f)
+        bb.p_ext[:, j] = probs[f]
+        bb.run(FRAME_MS, copy=False)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for f in range(warmup, total):
-        one(f)
+    for f in range(warmup, warmup + frames):
+        bb.p_ext[:, j] = probs[f]
+        bb.run(FRAME_MS, copy=False)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    return wall
+    steps = bb.steps_of(FRAME_MS)
+    res = {"engine": engine if engine == "push" else f"tile{tile}", "B": B, "frames": frames,
+           "value": B * frames * steps * p.dt / 1000.0 / wall, "ms_per_frame": wall / frames * 1000.0,
+           "launch_shape": shape}
+    if kernel_frames:
+        deg = np.diff(bb.indptr).astype(np.float64)
+        tim = {"update_ms": 0.0, "update_launches": 0, "push_ms": 0.0, "push_launches": 0}
+        spikes = push_bytes = 0.0
+        for f in range(kernel_frames):
+            bb.p_ext[:, j] = probs[f % (warmup + frames)]
+            r = bb.run_timed(steps)
+            c = bb.counts.to(torch.float64).sum(0).cpu().numpy()
+            push_bytes += float((c * (8.0 + 10.0 * deg)).sum())
+            spikes += float(c.sum())
+            for k in tim:
+                tim[k] += r[k]
+        n_in = len(idx)
+        upd_bytes = kernel_frames * steps * B * (26.0 * bb.n + 4.0 * n_in)
+        if engine == "tile":   # one kernel does the whole step
+            dom, nbytes, ms, launches = "lif_tile", upd_bytes + push_bytes, tim["update_ms"], tim["update_launches"]
+        elif tim["update_ms"] >= tim["push_ms"]:
+            dom, nbytes, ms, launches = "lif_update", upd_bytes, tim["update_ms"], tim["update_launches"]
+        else:
+            dom, nbytes, ms, launches = "lif_push", push_bytes, tim["push_ms"], tim["push_launches"]
+        avg_ms = ms / max(launches, 1)
+        bpl = nbytes / max(launches, 1)
+        achieved = bpl / (avg_ms / 1000.0) / 1e9
+        res["roofline"] = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                           "avg_launch_us": round(avg_ms * 1000.0, 3), "bytes_per_launch": round(bpl),
+                           "launches": launches,
+                           "kernel_ms_per_frame": round((tim["update_ms"] + tim["push_ms"]) / kernel_frames, 4),
+                           "spikes_per_step_per_fly": round(spikes / (kernel_frames * steps * B), 1)}
+    del bb, probs
+    torch.cuda.empty_cache()
+    return res
 
 
 def main():
@@ -82,6 +117,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--batch", type=int, default=256, help="flies in the batched line (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=10)
+    ap.add_argument("--engines1", default="push,tile256,tile512,tile1024")
+    ap.add_argument("--enginesB", default="tile1024,tile2048,push")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample length")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -91,100 +128,57 @@ def main():
         sys.exit(2)
 
     import torch
-    from flyhip import BatchBrain, LIFParams, autotune
+    from flyhip import LIFParams
     from flyhip.connectome import flywire_like
 
     t = time.perf_counter()
     fly = flywire_like()
     log(f"connectome: n={fly.n} nnz={fly.weights.nnz} ({time.perf_counter() - t:.1f}s)")
     p = LIFParams()
-    n, n_in = fly.n, len(fly.input_idx) + 1
     steps_per_frame = int(round(FRAME_MS / p.dt))
-    total = args.warmup + args.steps
 
-    # ---- B = 1 (headline) -------------------------------------------------
-    bb = BatchBrain(fly.weights, p, batch=1, seed=0)
-    rates = torch.from_numpy(frame_rates(fly, total)).cuda()
-    j1 = bb.slots(np.r_[fly.input_idx, fly.drive_idx])
-    bb.p_ext[:, j1] = rates[0] * float(np.float32(p.dt / 1000.0))
-    shape1 = autotune.tune(bb, use_cache=False)
-    log(f"B=1 launch shape: {shape1}")
-    wall = run_workload(bb, fly, rates, args.steps, args.warmup, torch)
-    sim_s = args.steps * steps_per_frame * p.dt / 1000.0
-    value = sim_s / wall
-    ms_per_frame = wall / args.steps * 1000.0
-    log(f"B=1: {args.steps} frames in {wall:.3f}s -> {value:.2f} fly-sim-s/s ({ms_per_frame:.3f} ms/frame)")
+    def parse(spec):
+        out = []
+        for e in spec.split(","):
+            e = e.strip()
+            if e:
+                out.append(("push", 0) if e == "push" else ("tile", int(e[4:])))
+        return out
 
-    # per-kernel timing (HIP events on the engine stream) over a few eager frames
-    deg = np.diff(fly.weights.indptr).astype(np.float64)
-    tim = {"update_ms": 0.0, "update_launches": 0, "push_ms": 0.0, "push_launches": 0}
-    push_bytes = 0.0
-    n_spikes = 0.0
-    kframes = 6
-    idx = np.r_[fly.input_idx, fly.drive_idx]
-    j = bb.slots(idx)
-    for f in range(kframes):
-        bb.p_ext[:, j] = rates[f % total] * float(np.float32(p.dt / 1000.0))
-        r = bb.run_timed(steps_per_frame)
-        c = bb.counts[0].cpu().numpy().astype(np.float64)
-        push_bytes += float((c * (8.0 + 10.0 * deg)).sum())
-        n_spikes += float(c.sum())
-        for k in tim:
-            tim[k] += r[k]
-    upd_steps = kframes * steps_per_frame
-    upd_bytes_total = upd_steps * (26.0 * n + 4.0 * n_in)
-    upd_avg_ms = tim["update_ms"] / max(tim["update_launches"], 1)
-    push_avg_ms = tim["push_ms"] / max(tim["push_launches"], 1)
-    if tim["update_ms"] >= tim["push_ms"]:
-        dom, bytes_launch, avg_ms = "lif_update", upd_bytes_total / tim["update_launches"], upd_avg_ms
-    else:
-        dom, bytes_launch, avg_ms = "lif_push", push_bytes / tim["push_launches"], push_avg_ms
-    achieved = bytes_launch / (avg_ms / 1000.0) / 1e9
-    traffic = None
-    tpath = os.path.join(REPO, "profiles", "traffic.json")
-    if os.path.exists(tpath):
-        try:
-            traffic = json.load(open(tpath)).get(dom, {}).get("bytes_per_launch")
-        except Exception:
-            traffic = None
-    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
-                "avg_launch_us": round(avg_ms * 1000.0, 3), "bytes_per_launch": round(bytes_launch),
-                "update_avg_us": round(upd_avg_ms * 1000.0, 3), "push_avg_us": round(push_avg_ms * 1000.0, 3),
-                "update_launches": tim["update_launches"], "push_launches": tim["push_launches"],
-                "spikes_per_step": round(n_spikes / max(upd_steps, 1), 1), "launch_shape": shape1}
-    log(f"kernels: update {upd_avg_ms * 1000:.1f} us/launch, push {push_avg_ms * 1000:.1f} us/launch; "
-        f"dominant {dom}: {achieved:.1f} GB/s")
+    # ---- B = 1 (headline): every engine configuration, best one reported ----------------
+    lines1 = []
+    for engine, tile in parse(args.engines1):
+        r = measure_line(fly, p, 1, engine, tile, args.steps, args.warmup, torch, kernel_frames=6)
+        log(f"B=1 {r['engine']}: {r['value']:.2f} fly-sim-s/s ({r['ms_per_frame']:.3f} ms/frame) "
+            f"{r['launch_shape']}; {r['roofline']['kernel']} {r['roofline']['avg_launch_us']} us/launch")
+        lines1.append(r)
+    best = max(lines1, key=lambda r: r["value"])
+    value, ms_per_frame = best["value"], best["ms_per_frame"]
 
-    # ---- B = 256 (batched line) --------------------------------------------
+    # ---- B = 256 (batched line) -------------------------------------------------------------
     batched = None
     if args.batch > 0:
         B = args.batch
         rng = np.random.default_rng(5)
         drive = rng.uniform(20.0, 120.0, B).astype(np.float32)
         gain = rng.uniform(0.5, 2.0, B).astype(np.float32)
-        nb = args.batch_steps + 2
-        gbb = BatchBrain(fly.weights, p, batch=B, seed=1)
-        rb = torch.from_numpy(frame_rates(fly, nb, gain, drive)).cuda()
-        jb = gbb.slots(np.r_[fly.input_idx, fly.drive_idx])
-        gbb.p_ext[:, jb] = rb[0] * float(np.float32(p.dt / 1000.0))
-        shapeb = autotune.tune(gbb, use_cache=False, reps=1)
-        log(f"B={B} launch shape: {shapeb}")
-        wb = run_workload(gbb, fly, rb, args.batch_steps, 2, torch)
-        vb = B * args.batch_steps * steps_per_frame * p.dt / 1000.0 / wb
-        batched = {"B": B, "frames": args.batch_steps, "value": round(vb, 3), "unit": "fly-sim-seconds/wall-second",
-                   "launch_shape": shapeb,
-                   "ms_per_frame": round(wb / args.batch_steps * 1000.0, 3),
-                   "vs_b1": round(vb / value, 2)}
-        log(f"B={B}: {vb:.2f} fly-sim-s/s aggregate ({batched['ms_per_frame']:.2f} ms/frame)")
-        del gbb, rb
-        torch.cuda.empty_cache()
+        linesB = []
+        for engine, tile in parse(args.enginesB):
+            r = measure_line(fly, p, B, engine, tile, args.batch_steps, 2, torch, gain, drive, seed=1,
+                             kernel_frames=2)
+            log(f"B={B} {r['engine']}: {r['value']:.2f} fly-sim-s/s ({r['ms_per_frame']:.2f} ms/frame) "
+                f"{r['launch_shape']}; {r['roofline']['kernel']} {r['roofline']['avg_launch_us']} us/launch")
+            linesB.append(r)
+        bestB = max(linesB, key=lambda r: r["value"])
+        batched = dict(bestB, unit="fly-sim-seconds/wall-second", vs_b1=round(bestB["value"] / value, 2),
+                       candidates=[{k: r[k] for k in ("engine", "value", "ms_per_frame")} for r in linesB])
 
-    # ---- CPU baseline (oracle, one thread) ----------------------------------
+    # ---- CPU baseline (oracle, one thread) --------------------------------------------------
     cpu = None
     if not args.no_cpu:
         from oracle.oracle import OracleBrain
 
+        idx = np.r_[fly.input_idx, fly.drive_idx]
         ob = OracleBrain(fly.weights, p, batch=1, seed=0)
         rr = frame_rates(fly, 64)
         ob.set_input(idx, rr[0, 0])
@@ -210,9 +204,11 @@ def main():
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
             "data": "synthetic (seeded FlyWire-shaped connectome, smooth-field Poisson input)",
             "config": {"workload": "plain LIF, 1 fly, dt=0.1 ms, 166-step game frames (hipGraph), "
-                                   "rates resampled per frame", "neurons": n, "connections": int(fly.weights.nnz),
-                       "batch": 1, "steps_per_frame": steps_per_frame, "parallelism": "single GPU"},
-            "roofline": roofline, "cpu_baseline": cpu, "batched": batched}
+                                   "rates resampled per frame", "neurons": fly.n,
+                       "connections": int(fly.weights.nnz), "batch": 1, "steps_per_frame": steps_per_frame,
+                       "engine": best["engine"], "parallelism": "single GPU"},
+            "roofline": best["roofline"], "cpu_baseline": cpu, "batched": batched,
+            "b1_candidates": [{k: r[k] for k in ("engine", "value", "ms_per_frame")} for r in lines1]}
     print(json.dumps(line), flush=True)
 
 

@@ -67,6 +67,8 @@ constexpr int KMAX = 32;          // steps per block held in registers
 constexpr int MAX_THREADS = 1024;
 constexpr int MAX_WAVES = MAX_THREADS / 64;
 constexpr int QMIN = 64;          // smallest push chunk (sizes the chunk-start table)
+constexpr size_t TILE_STATIC_LDS = 1024;  // room left for lif_tile's static shared arrays
+constexpr int UNROLL = 4;         // tile engine: edge loads in flight per thread
 
 struct Consts {
     float decay_syn, decay_m, u_th, u_reset, kick, in_scale, pw_qscale;
@@ -384,6 +386,317 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_push(PushArgs<WT> a) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Tile engine: arrivals pulled into LDS per neuron tile (no global atomics for
+// fixed synapses, no dense ring traffic).
+//
+// A workgroup owns TS neurons of one fly.  For a block of K <= n_delay steps
+// the inputs arriving at steps [t0, t0+K) come from the spikes of steps
+// [t0-n_delay, t0+K-n_delay), all already on the fly's per-step spike lists.
+// For each listed spike the workgroup reads the segment of the spiking row
+// that targets its tile (a per-(row, tile) start table; columns are sorted
+// within rows), and adds the signed counts into an LDS accumulator
+// acc[K][TS] with LDS atomics, after a block-wide prefix sum that spreads the
+// segments' edges evenly over the threads.  Plastic synapses are still pushed
+// at spike time (their weights may change between windows) into a ring of
+// 2*n_delay slots that is read only for the tile's plastic targets; pushes go
+// to slots n_delay ahead, so reads and writes in one launch never meet.
+// Then each thread advances its NPT neurons K steps from LDS and appends its
+// spikes to the per-step lists (one LDS count per step, one global atomic per
+// step and workgroup).  Lists have 3*n_delay slots: the launch for [t0, t0+K)
+// reads slots t0-n_delay.., writes t0.., and clears the counters of the next
+// n_delay steps, three disjoint ranges.
+
+struct TileArgs {
+    const long long* clock;
+    int N, B, K, kb, first, TS, n_tiles, nd, M, ring_slots;
+    float* u;
+    float* g;
+    signed char* ref;
+    int* ring;             // [B][ring_slots][N]: plastic arrivals only
+    int* counts;
+    const int* ext_idx;
+    int n_ext;
+    const float* p_ext;
+    const float* bias_ext;
+    const unsigned* hits;
+    int hw;
+    unsigned* raster;
+    int rw;
+    unsigned seed;
+    const unsigned char* alive;
+    const int* col;
+    const void* w;
+    int frac_bits;
+    const int* tb;         // [N][n_tiles + 1] first fixed edge of row i with post >= tile * TS
+    const int* pl_indptr;  // null without plastic synapses
+    const int* pl_post;
+    const float* pw;
+    int n_pl;
+    float pw_qscale;
+    const int* ptg_ptr;    // [n_tiles + 1] plastic target neurons of each tile ...
+    const int* ptg_idx;    // ... (null without plastic synapses)
+    int* spk_idx;          // [B][M][N]
+    int* spk_cnt;          // [B][M]
+    Consts c;
+};
+
+// Inclusive prefix sum over the workgroup; wsum needs MAX_WAVES + 1 ints.
+__device__ __forceinline__ int block_incl_scan(int v, int* wsum, int* total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[wave] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int w = 0; w < nw; w++) {
+            const int t = wsum[w];
+            wsum[w] = acc;
+            acc += t;
+        }
+        wsum[nw] = acc;
+    }
+    __syncthreads();
+    *total = wsum[nw];
+    return x + wsum[wave];
+}
+
+// IPT: spike-list entries staged per thread (more loads in flight, more LDS).
+template <typename WT, int NPT, int IPT>
+__global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
+    extern __shared__ int lds[];
+    __shared__ int s_off[KMAX + 1];
+    __shared__ int s_lcnt[KMAX];
+    __shared__ int s_wsum[MAX_WAVES + 1];
+    const int TPB = blockDim.x, tid = threadIdx.x, lane = tid & 63;
+    const int TS = TPB * NPT;
+    int* acc = lds;                      // [K][TS]
+    int* st_seg = acc + a.K * TS;        // [TPB * IPT]
+    int* st_end = st_seg + TPB * IPT;    // [TPB * IPT] inclusive prefix of segment lengths
+    int* st_k = st_end + TPB * IPT;      // [TPB * IPT]
+    const int b = blockIdx.y, T = blockIdx.x;
+    const int i0 = T * TS;
+    const size_t bn = (size_t)b * a.N;
+    const long long t0 = a.clock[0] + a.kb;
+    const uint32_t step0 = (uint32_t)a.clock[1] + (uint32_t)a.kb;
+    const int nd = a.nd, M = a.M;
+    const int lt0 = (int)(((t0 % M) + M) % M);             // list slot of step t0
+    const int rs0 = (int)(((t0 % a.ring_slots) + a.ring_slots) % a.ring_slots);
+    int* cnt_b = a.spk_cnt + (size_t)b * M;
+    int* idx_b = a.spk_idx + (size_t)b * M * a.N;
+    if (T == 0 && tid < nd) cnt_b[(lt0 + a.K + tid) % M] = 0;   // lists of the next n_delay steps
+    if (a.alive && !a.alive[b]) {                               // retired fly
+        if (a.first)
+            for (int p = 0; p < NPT; p++) {
+                const int i = i0 + p * TPB + tid;
+                if (i < a.N) a.counts[bn + i] = 0;
+            }
+        return;
+    }
+    for (int q = tid; q < a.K * TS; q += TPB) acc[q] = 0;
+    if (tid < a.K) {
+        const long long src = t0 + tid - nd;
+        const int c = src >= 0 ? cnt_b[(lt0 + tid - nd + M) % M] : 0;
+        s_lcnt[tid] = c < a.N ? c : a.N;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int o = 0;
+        for (int k = 0; k < a.K; k++) {
+            s_off[k] = o;
+            o += s_lcnt[k];
+        }
+        s_off[a.K] = o;
+    }
+    __syncthreads();
+    const int total = s_off[a.K];
+    const WT* w = static_cast<const WT*>(a.w);
+    // Entries in chunks of TPB * IPT; each thread owns IPT consecutive entries so that all of
+    // its list and table loads are in flight together.
+    for (int base = 0; base < total; base += TPB * IPT) {
+        int seg[IPT], len[IPT], kk[IPT];
+        int ids[IPT];
+#pragma unroll
+        for (int r = 0; r < IPT; r++) {
+            const int e = base + tid * IPT + r;
+            kk[r] = 0;
+            ids[r] = -1;
+            if (e < total) {
+                while (s_off[kk[r] + 1] <= e) kk[r]++;
+                ids[r] = idx_b[(size_t)((lt0 + kk[r] - nd + M) % M) * a.N + (e - s_off[kk[r]])];
+            }
+        }
+        int sum = 0;
+#pragma unroll
+        for (int r = 0; r < IPT; r++) {
+            seg[r] = 0;
+            len[r] = 0;
+            if ((unsigned)ids[r] < (unsigned)a.N) {
+                const int* tr = a.tb + (size_t)ids[r] * (a.n_tiles + 1) + T;
+                seg[r] = tr[0];
+                len[r] = tr[1] - seg[r];
+            }
+            sum += len[r];
+        }
+        int tot;
+        int run = block_incl_scan(sum, s_wsum, &tot) - sum;
+#pragma unroll
+        for (int r = 0; r < IPT; r++) {
+            run += len[r];
+            st_seg[tid * IPT + r] = seg[r];
+            st_end[tid * IPT + r] = run;
+            st_k[tid * IPT + r] = kk[r];
+        }
+        __syncthreads();
+        const int ns = TPB * IPT;
+        for (int q0 = tid; q0 < tot; q0 += TPB * UNROLL) {
+            int ek[UNROLL], slot[UNROLL];
+#pragma unroll
+            for (int u = 0; u < UNROLL; u++) {
+                const int q = q0 + u * TPB;
+                ek[u] = -1;
+                slot[u] = 0;
+                if (q < tot) {
+                    int lo = 0, hi = ns - 1;  // first j with st_end[j] > q
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if (st_end[mid] > q) hi = mid;
+                        else lo = mid + 1;
+                    }
+                    ek[u] = st_seg[lo] + (q - (lo ? st_end[lo - 1] : 0));
+                    slot[u] = st_k[lo] * TS;
+                }
+            }
+            int post[UNROLL], val[UNROLL];
+#pragma unroll
+            for (int u = 0; u < UNROLL; u++) {
+                post[u] = 0;
+                val[u] = 0;
+                if (ek[u] >= 0) {
+                    post[u] = a.col[ek[u]];
+                    val[u] = wval(w[ek[u]], a.frac_bits);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < UNROLL; u++)
+                if (val[u] != 0) atomicAdd(&acc[slot[u] + (post[u] - i0)], val[u]);
+        }
+        __syncthreads();
+    }
+    if (a.ptg_ptr) {  // plastic arrivals (pushed at spike time), tile's targets only
+        const int p0 = a.ptg_ptr[T], np_ = a.ptg_ptr[T + 1] - p0;
+        for (int q = tid; q < np_ * a.K; q += TPB) {
+            const int k = q / np_;
+            const int i = a.ptg_idx[p0 + (q - k * np_)];
+            int* r = a.ring + ((size_t)b * a.ring_slots + (rs0 + k) % a.ring_slots) * a.N + i;
+            const int v = *r;
+            if (v != 0) {
+                atomicAdd(&acc[k * TS + (i - i0)], v);
+                *r = 0;
+            }
+        }
+        __syncthreads();
+    }
+
+    // K steps of flyzero/brain.py:37-53 from LDS, NPT neurons per thread
+    const uint32_t hseed = mix32(a.seed + (uint32_t)b);
+    uint32_t masks[NPT];
+#pragma unroll
+    for (int p = 0; p < NPT; p++) {
+        const int li = p * TPB + tid, i = i0 + li;
+        uint32_t mask = 0;
+        if (i < a.N) {
+            float u = a.u[bn + i], g = a.g[bn + i], bias = 0.f, pp = 0.f;
+            int r = a.ref[bn + i];
+            const float u_in = u, g_in = g;
+            const int r_in = r;
+            const int j = a.ext_idx ? a.ext_idx[i] : -1;
+            if (j >= 0) {
+                const size_t e = (size_t)b * a.n_ext + j;
+                if (a.bias_ext) bias = a.bias_ext[e];
+                if (a.p_ext) pp = a.p_ext[e];
+            }
+            const uint32_t tid32 = (uint32_t)(bn + (size_t)i);
+            for (int k = 0; k < a.K; k++) {
+                const float inj = (float)acc[k * TS + li] * a.c.in_scale;
+                g = g * a.c.decay_syn + inj;
+                if (r > 0) {
+                    r -= 1;
+                } else {
+                    u += (g + bias - u) * a.c.decay_m;
+                }
+                bool hit = false;
+                if (j >= 0) {
+                    if (a.hits) {
+                        hit = (a.hits[((size_t)(a.kb + k) * a.B + b) * a.hw + (j >> 5)] >> (j & 31)) & 1u;
+                    } else if (pp > 0.f) {
+                        const uint32_t h = mix32(tid32 ^ mix32((step0 + (uint32_t)k) * 0x9e3779b9U ^ hseed));
+                        hit = (float)(h >> 8) * (1.0f / 16777216.0f) < pp;
+                    }
+                }
+                if (hit) u += a.c.kick;
+                if (u >= a.c.u_th && r <= 0) {
+                    u = a.c.u_reset;
+                    r = a.c.n_ref;
+                    mask |= 1u << k;
+                }
+            }
+            if (u != u_in) a.u[bn + i] = u;
+            if (g != g_in) a.g[bn + i] = g;
+            if (r != r_in) a.ref[bn + i] = (signed char)r;
+            const int ns = __popc(mask);
+            if (a.first) a.counts[bn + i] = ns;
+            else if (ns) a.counts[bn + i] += ns;
+        }
+        masks[p] = mask;
+    }
+
+    if (a.raster) {
+#pragma unroll
+        for (int p = 0; p < NPT; p++) {
+            const int wbase = (i0 + p * TPB + (tid & ~63)) >> 5;
+            for (int k = 0; k < a.K; k++) {
+                const unsigned long long bal = __ballot((masks[p] >> k) & 1u);
+                unsigned* row = a.raster + ((size_t)(a.kb + k) * a.B + b) * a.rw;
+                if (lane == 0 && wbase < a.rw) row[wbase] = (unsigned)bal;
+                if (lane == 32 && wbase + 1 < a.rw) row[wbase + 1] = (unsigned)(bal >> 32);
+            }
+        }
+    }
+
+    // append spikes to the per-step lists; push plastic rows n_delay steps ahead
+    if (tid < a.K) s_lcnt[tid] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < NPT; p++)
+        for (uint32_t m = masks[p]; m; m &= m - 1) atomicAdd(&s_lcnt[__ffs(m) - 1], 1);
+    __syncthreads();
+    if (tid < a.K && s_lcnt[tid] > 0) s_lcnt[tid] = atomicAdd(&cnt_b[(lt0 + tid) % M], s_lcnt[tid]);
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < NPT; p++) {
+        const int i = i0 + p * TPB + tid;
+        for (uint32_t m = masks[p]; m; m &= m - 1) {
+            const int k = __ffs(m) - 1;
+            const int pos = atomicAdd(&s_lcnt[k], 1);
+            if (pos < a.N) idx_b[(size_t)((lt0 + k) % M) * a.N + pos] = i;
+            if (a.pl_indptr) {
+                int* r = a.ring + ((size_t)b * a.ring_slots + (rs0 + k + nd) % a.ring_slots) * a.N;
+                const float* pwb = a.pw + (size_t)b * a.n_pl;
+                for (int q = a.pl_indptr[i]; q < a.pl_indptr[i + 1]; q++) {
+                    const int val = (int)rintf(pwb[q] * a.pw_qscale);
+                    if (val != 0) atomicAdd(r + a.pl_post[q], val);
+                }
+            }
+        }
+    }
+}
+
 __global__ void lif_clock(long long* clock, int steps, int nblocks) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         clock[0] += steps;
@@ -428,6 +741,13 @@ struct fz_engine {
     int* err = nullptr;
     int kmax = 1;
     int n_cu = 256;
+    size_t lds_max = 65536;
+    // tile engine (mode 2)
+    int mode = 1;
+    int ts = 0, n_tiles = 0;
+    const int* tb = nullptr;
+    const int* ptg_ptr = nullptr;
+    const int* ptg_idx = nullptr;
 };
 
 struct fz_graph {
@@ -438,7 +758,7 @@ struct fz_graph {
 namespace {
 
 struct Shape {
-    int ut, pt, chunk, grid, K;
+    int ut, pt, chunk, grid, K, ipt;
 };
 
 int resolve(const fz_engine* e, const fz_tune* t, Shape* s) {
@@ -447,6 +767,19 @@ int resolve(const fz_engine* e, const fz_tune* t, Shape* s) {
     s->chunk = (t && t->push_chunk) ? t->push_chunk : 256;
     s->grid = (t && t->push_grid) ? t->push_grid : e->n_cu * 8;
     s->K = (t && t->block_steps) ? t->block_steps : e->kmax;
+    s->ipt = (t && t->tile_stage) ? t->tile_stage : 1;
+    if (s->ipt != 1 && s->ipt != 4) return fail(FZ_E_ARG, "tile_stage %d: must be 1 or 4", s->ipt);
+    if (e->mode == 2) {
+        if (!(t && t->upd_threads)) s->ut = e->ts < 256 ? e->ts : 256;
+        const int npt = s->ut > 0 ? e->ts / s->ut : 0;
+        if (s->ut <= 0 || e->ts % s->ut || (npt != 1 && npt != 2 && npt != 4 && npt != 8))
+            return fail(FZ_E_ARG, "tile engine: upd_threads %d must divide the tile (%d neurons) into 1, 2, 4 or 8",
+                        s->ut, e->ts);
+        const size_t lds = (size_t)s->K * e->ts * 4 + (size_t)3 * s->ipt * s->ut * 4;
+        if (lds > e->lds_max - TILE_STATIC_LDS)
+            return fail(FZ_E_ARG, "tile engine: %zu B of LDS for %d steps x %d neurons exceeds %zu", lds, s->K,
+                        e->ts, e->lds_max);
+    }
     if (s->ut < 64 || s->ut > MAX_THREADS || s->ut % 64)
         return fail(FZ_E_ARG, "upd_threads %d: need a multiple of 64 in [64, %d]", s->ut, MAX_THREADS);
     if (s->pt < 64 || s->pt > MAX_THREADS || s->pt % 64)
@@ -530,6 +863,72 @@ dim3 push_grid(const fz_engine* e, const Shape& s) {
     return dim3(gx, e->st.B);
 }
 
+TileArgs make_tile(const fz_engine* e, const fz_inputs* in, int K, int kb, int first, const unsigned* hits,
+                   unsigned* raster) {
+    TileArgs a{};
+    a.clock = e->st.clock;
+    a.N = e->n;
+    a.B = e->st.B;
+    a.K = K;
+    a.kb = kb;
+    a.first = first;
+    a.TS = e->ts;
+    a.n_tiles = e->n_tiles;
+    a.nd = e->c.n_delay;
+    a.M = 3 * e->c.n_delay;
+    a.ring_slots = e->st.ring_slots;
+    a.u = e->st.u;
+    a.g = e->st.g;
+    a.ref = e->st.ref;
+    a.ring = e->st.ring;
+    a.counts = e->st.counts;
+    a.ext_idx = in->n_ext > 0 ? in->ext_idx : nullptr;
+    a.n_ext = in->n_ext;
+    a.p_ext = in->n_ext > 0 ? in->p_ext : nullptr;
+    a.bias_ext = in->n_ext > 0 ? in->bias_ext : nullptr;
+    a.hits = hits;
+    a.hw = (in->n_ext + 31) / 32;
+    a.raster = raster;
+    a.rw = (e->n + 31) / 32;
+    a.seed = in->seed;
+    a.alive = in->alive;
+    a.col = e->col;
+    a.w = e->w;
+    a.frac_bits = e->c.frac_bits;
+    a.tb = e->tb;
+    a.pl_indptr = e->n_pl > 0 ? e->pl_indptr : nullptr;
+    a.pl_post = e->n_pl > 0 ? e->pl_post : nullptr;
+    a.pw = e->n_pl > 0 ? in->pw : nullptr;
+    a.n_pl = e->n_pl;
+    a.pw_qscale = e->c.pw_qscale;
+    a.ptg_ptr = e->n_pl > 0 ? e->ptg_ptr : nullptr;
+    a.ptg_idx = e->n_pl > 0 ? e->ptg_idx : nullptr;
+    a.spk_idx = e->st.spk_idx;
+    a.spk_cnt = e->st.spk_cnt;
+    a.c = e->c;
+    return a;
+}
+
+template <typename WT, int IPT>
+void* tile_kernel_npt(int npt) {
+    switch (npt) {
+        case 1: return reinterpret_cast<void*>(&lif_tile<WT, 1, IPT>);
+        case 2: return reinterpret_cast<void*>(&lif_tile<WT, 2, IPT>);
+        case 4: return reinterpret_cast<void*>(&lif_tile<WT, 4, IPT>);
+        default: return reinterpret_cast<void*>(&lif_tile<WT, 8, IPT>);
+    }
+}
+
+// The lif_tile instantiation for a weight type, neurons per thread and staging depth.
+void* tile_kernel(int w_bits, int npt, int ipt) {
+    if (w_bits == 16) return ipt == 1 ? tile_kernel_npt<int16_t, 1>(npt) : tile_kernel_npt<int16_t, 4>(npt);
+    return ipt == 1 ? tile_kernel_npt<int32_t, 1>(npt) : tile_kernel_npt<int32_t, 4>(npt);
+}
+
+size_t tile_lds(const fz_engine* e, const Shape& s, int K) {
+    return (size_t)K * e->ts * 4 + (size_t)3 * s.ipt * s.ut * 4;
+}
+
 // One window as a sequence of kernel launches (the graph builder mirrors it).
 // With `ev_*` non-null, events bracket every kernel.
 int enqueue(fz_engine* e, const fz_inputs* in, int steps, const unsigned* hits, unsigned* raster,
@@ -546,6 +945,23 @@ int enqueue(fz_engine* e, const fz_inputs* in, int steps, const unsigned* hits, 
         return FZ_OK;
     };
     int rc;
+    if (e->mode == 2) {
+        const dim3 tgrid(e->n_tiles, e->st.B);
+        void* fn = tile_kernel(e->w_bits, e->ts / s.ut, s.ipt);
+        while (kb < steps) {
+            const int K = (steps - kb) < s.K ? (steps - kb) : s.K;
+            TileArgs ta = make_tile(e, in, K, kb, kb == 0, hits, raster);
+            void* args[] = {&ta};
+            if ((rc = mark(ev_upd))) return rc;
+            HIPCHK(hipLaunchKernel(fn, tgrid, dim3(s.ut), args, tile_lds(e, s, K), stream));
+            if ((rc = mark(ev_upd))) return rc;
+            kb += K;
+            blk++;
+        }
+        hipLaunchKernelGGL(lif_clock, dim3(1), dim3(64), 0, stream, e->st.clock, steps, blk);
+        HIPCHK(hipGetLastError());
+        return FZ_OK;
+    }
     while (kb < steps) {
         const int K = (steps - kb) < s.K ? (steps - kb) : s.K;
         UpdArgs ua = make_upd(e, in, s, K, kb, blk, kb == 0, hits, raster);
@@ -614,8 +1030,10 @@ int fz_create(int n, long long nnz, const int* indptr, const int* col, const voi
     e->kmax = p->n_delay < KMAX ? p->n_delay : KMAX;
     int dev = 0;
     hipDeviceProp_t prop;
-    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess) {
         e->n_cu = prop.multiProcessorCount;
+        e->lds_max = prop.sharedMemPerBlock;
+    }
     if (hipMalloc(&e->err, sizeof(int)) != hipSuccess || hipMemset(e->err, 0, sizeof(int)) != hipSuccess) {
         fz_destroy(e);
         return fail(FZ_E_HIP, "allocating the engine's error flag failed");
@@ -653,6 +1071,12 @@ int fz_bind(fz_engine* e, const fz_state* st) {
     if (st->B < 1 || st->B > 65535) return fail(FZ_E_ARG, "B = %d outside [1, 65535]", st->B);
     if (!st->u || !st->g || !st->ref || !st->ring || !st->counts || !st->clock)
         return fail(FZ_E_ARG, "null state array");
+    const int want_slots = e->mode == 2 ? 2 * e->c.n_delay : e->c.n_delay;
+    if (st->ring_slots != want_slots)
+        return fail(FZ_E_ARG, "ring has %d slots, the %s engine needs %d", st->ring_slots,
+                    e->mode == 2 ? "tile" : "push", want_slots);
+    if (e->mode == 2 && (!st->spk_idx || !st->spk_cnt))
+        return fail(FZ_E_ARG, "the tile engine needs the spike lists spk_idx and spk_cnt");
     const int per = e->c.n_ref > 1 ? e->c.n_ref : 1;
     const long long maxspk = (e->kmax + per - 1) / per;  // spikes per neuron per block
     const long long cap = (long long)e->n * maxspk;       // entries per fly
@@ -678,6 +1102,37 @@ int fz_bind(fz_engine* e, const fz_state* st) {
     }
     e->st = *st;
     e->bound = true;
+    return FZ_OK;
+}
+
+int fz_set_tiles(fz_engine* e, int tile, int n_tiles, const int* tb, const int* ptg_ptr, const int* ptg_idx) {
+    if (!e) return fail(FZ_E_ARG, "null engine");
+    if (tile < 64 || tile > 8192 || tile % 64) return fail(FZ_E_ARG, "tile %d: need a multiple of 64 in [64, 8192]", tile);
+    if (n_tiles != (e->n + tile - 1) / tile) return fail(FZ_E_ARG, "n_tiles %d != ceil(n / tile)", n_tiles);
+    if (!tb) return fail(FZ_E_ARG, "null tile table");
+    if (e->n_pl > 0 && (!ptg_ptr || !ptg_idx)) return fail(FZ_E_ARG, "plastic synapses need the per-tile target lists");
+    const size_t lds = (size_t)e->kmax * tile * 4 + (size_t)3 * 64 * 4;
+    if (lds > e->lds_max - TILE_STATIC_LDS)
+        return fail(FZ_E_ARG, "tile %d needs %zu B of LDS for %d steps; the device allows %zu", tile, lds, e->kmax,
+                    e->lds_max - TILE_STATIC_LDS);
+    for (int npt : {1, 2, 4, 8})
+        for (int ipt : {1, 4}) {
+            hipError_t r = hipFuncSetAttribute(tile_kernel(e->w_bits, npt, ipt),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               (int)(e->lds_max - TILE_STATIC_LDS));
+            if (r != hipSuccess) {
+                (void)hipGetLastError();  // do not leave a sticky error for the caller's next HIP call
+                return fail(FZ_E_HIP, "hipFuncSetAttribute(max dynamic LDS %zu) failed: %s",
+                            e->lds_max - TILE_STATIC_LDS, hipGetErrorString(r));
+            }
+        }
+    e->mode = 2;
+    e->ts = tile;
+    e->n_tiles = n_tiles;
+    e->tb = tb;
+    e->ptg_ptr = ptg_ptr;
+    e->ptg_idx = ptg_idx;
+    e->bound = false;  // the state must be re-bound (ring slots, spike lists)
     return FZ_OK;
 }
 
@@ -742,12 +1197,12 @@ int fz_graph_create(fz_engine* e, const fz_inputs* in, int steps, const fz_tune*
     hipError_t err = hipGraphCreate(&gr->graph, 0);
     if (err != hipSuccess) return bail("hipGraphCreate", err);
     hipGraphNode_t prev = nullptr;
-    auto add = [&](void* fn, dim3 grid, dim3 block, void** args) -> hipError_t {
+    auto add = [&](void* fn, dim3 grid, dim3 block, void** args, size_t lds = 0) -> hipError_t {
         hipKernelNodeParams kp{};
         kp.func = fn;
         kp.gridDim = grid;
         kp.blockDim = block;
-        kp.sharedMemBytes = 0;
+        kp.sharedMemBytes = (unsigned)lds;
         kp.kernelParams = args;
         kp.extra = nullptr;
         hipGraphNode_t node;
@@ -758,6 +1213,19 @@ int fz_graph_create(fz_engine* e, const fz_inputs* in, int steps, const fz_tune*
     const dim3 ugrid((e->n + s.ut - 1) / s.ut, e->st.B);
     const dim3 pgrid = push_grid(e, s);
     int kb = 0, blk = 0;
+    if (e->mode == 2) {
+        const dim3 tgrid(e->n_tiles, e->st.B);
+        void* fn = tile_kernel(e->w_bits, e->ts / s.ut, s.ipt);
+        while (kb < steps) {
+            const int K = (steps - kb) < s.K ? (steps - kb) : s.K;
+            TileArgs ta = make_tile(e, in, K, kb, kb == 0, nullptr, nullptr);
+            void* targs[] = {&ta};
+            if ((err = add(fn, tgrid, dim3(s.ut), targs, tile_lds(e, s, K))) != hipSuccess)
+                return bail("hipGraphAddKernelNode(tile)", err);
+            kb += K;
+            blk++;
+        }
+    }
     while (kb < steps) {
         const int K = (steps - kb) < s.K ? (steps - kb) : s.K;
         UpdArgs ua = make_upd(e, in, s, K, kb, blk, kb == 0, nullptr, nullptr);

@@ -30,7 +30,8 @@ class FzParams(ctypes.Structure):
 
 class FzState(ctypes.Structure):
     _fields_ = [("B", c_int), ("u", c_void_p), ("g", c_void_p), ("ref", c_void_p), ("ring", c_void_p),
-                ("counts", c_void_p), ("clock", c_void_p)]
+                ("counts", c_void_p), ("clock", c_void_p), ("ring_slots", c_int), ("spk_idx", c_void_p),
+                ("spk_cnt", c_void_p)]
 
 
 class FzInputs(ctypes.Structure):
@@ -40,7 +41,7 @@ class FzInputs(ctypes.Structure):
 
 class FzTune(ctypes.Structure):
     _fields_ = [("upd_threads", c_int), ("push_threads", c_int), ("push_chunk", c_int),
-                ("push_grid", c_int), ("block_steps", c_int)]
+                ("push_grid", c_int), ("block_steps", c_int), ("tile_stage", c_int)]
 
 
 # name -> (restype, argtypes); every symbol declared in include/flyhip.h
@@ -52,6 +53,7 @@ SIGNATURES = {
     "fz_destroy": (c_int, [c_void_p]),
     "fz_set_plastic": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     "fz_bind": (c_int, [c_void_p, ctypes.POINTER(FzState)]),
+    "fz_set_tiles": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
     "fz_run": (c_int, [c_void_p, ctypes.POINTER(FzInputs), c_int, c_void_p, c_void_p,
                        ctypes.POINTER(FzTune), c_void_p]),
     "fz_run_timed": (c_int, [c_void_p, ctypes.POINTER(FzInputs), c_int, ctypes.POINTER(FzTune), c_void_p,

@@ -28,29 +28,39 @@ PUSH_GRID_PER_CU = (2, 4, 8, 16)
 
 def _key(bb) -> str:
     name = torch.cuda.get_device_name(bb.device)
-    return f"{name}|n={bb.n}|nnz={int(bb.d_col.numel())}|B={bb.B}|npl={len(bb.plastic_pos)}"
+    eng = bb.engine + (str(bb.tile) if bb.engine == "tile" else "")
+    return f"{name}|{eng}|n={bb.n}|nnz={int(bb.d_col.numel())}|B={bb.B}|npl={len(bb.plastic_pos)}"
+
+
+def _state(bb):
+    extra = (bb.spk_idx, bb.spk_cnt) if bb.spk_idx is not None else ()
+    return (bb.u, bb.g, bb.ref, bb.ring, bb.counts) + extra
 
 
 def _snapshot(bb):
-    return [t.clone() for t in (bb.u, bb.g, bb.ref, bb.ring, bb.counts, bb.clock)], (bb._t, bb._step_no)
+    return [t.clone() for t in _state(bb)], bb.clock.clone(), (bb._t, bb._step_no)
 
 
 def _restore(bb, snap):
-    tensors, (t, step_no) = snap
-    for dst, src in zip((bb.u, bb.g, bb.ref, bb.ring, bb.counts), tensors):
+    tensors, clock, (t, step_no) = snap
+    for dst, src in zip(_state(bb), tensors):
         dst.copy_(src)
     # clock[2] (block parity of the engine's work-list counters) keeps running: only the
     # simulated time and the RNG step go back
-    bb.clock[0:2].copy_(tensors[5][0:2])
+    bb.clock[0:2].copy_(clock[0:2])
     bb._t, bb._step_no = t, step_no
 
 
 def _time(bb, shape: dict, steps: int, reps: int, snap) -> float:
+    """Best of ``reps`` windows; inf for a shape the engine rejects (e.g. LDS over the limit)."""
     bb.set_tune(**shape)
     best = float("inf")
     for _ in range(reps):
         _restore(bb, snap)
-        r = bb.run_timed(steps)
+        try:
+            r = bb.run_timed(steps)
+        except ValueError:
+            return float("inf")
         best = min(best, r["update_ms"] + r["push_ms"])
     return best
 
@@ -74,13 +84,30 @@ def tune(bb, duration_ms: float = 1000.0 / 60.0988, reps: int = 2, use_cache: bo
     snap = _snapshot(bb)
     # warm-up window (first launches pay code-object loading)
     _time(bb, {}, steps, 1, snap)
-    cur = {"upd_threads": 256, "push_threads": 256, "push_chunk": 256, "push_grid": n_cu * 8}
-    best_t = _time(bb, cur, steps, reps, snap)
-    space = [("upd_threads", UPD_THREADS), ("push_chunk", PUSH_CHUNK), ("push_threads", PUSH_THREADS),
-             ("push_grid", tuple(n_cu * k for k in PUSH_GRID_PER_CU))]
+    if bb.engine == "tile":
+        # full grid (small): threads per workgroup (1, 2, 4 or 8 neurons per thread) x block length
+        # x spike-list staging depth; shapes over the LDS budget are skipped
+        best_t, cur = float("inf"), None
+        for ut in (bb.tile // k for k in (8, 4, 2, 1)):
+            if not 64 <= ut <= 1024:
+                continue
+            for ks in sorted({bb.n_delay, max(1, bb.n_delay // 2)}):
+                for stage in (1, 4):
+                    trial = {"upd_threads": ut, "block_steps": ks, "tile_stage": stage}
+                    t = _time(bb, trial, steps, reps, snap)
+                    if verbose:
+                        print(f"autotune {trial}: {t * 1000:.1f} us", flush=True)
+                    if t < best_t:
+                        best_t, cur = t, trial
+        space = []
+    else:
+        cur = {"upd_threads": 256, "push_threads": 256, "push_chunk": 256, "push_grid": n_cu * 8}
+        space = [("upd_threads", UPD_THREADS), ("push_chunk", PUSH_CHUNK), ("push_threads", PUSH_THREADS),
+                 ("push_grid", tuple(n_cu * k for k in PUSH_GRID_PER_CU))]
+        best_t = _time(bb, cur, steps, reps, snap)
     for name, values in space:
         for v in values:
-            if v == cur[name]:
+            if v == cur.get(name):
                 continue
             trial = dict(cur, **{name: v})
             t = _time(bb, trial, steps, reps, snap)

@@ -30,6 +30,24 @@ from .params import LIFParams, derived, frac_bits_for, quantize, scales
 from .state import MembraneView
 
 
+def tile_tables(indptr: np.ndarray, col: np.ndarray, n: int, tile: int, pl_post: np.ndarray):
+    """Tile engine tables.  tb[i, t] = first edge of row i (fixed CSR, columns sorted within rows)
+    whose post neuron is >= t * tile, for t = 0 .. n_tiles; and the plastic post neurons of each
+    tile as a CSR (ptg_ptr, ptg_idx)."""
+    nt = (n + tile - 1) // tile
+    rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(indptr))
+    key = rows * (nt + 1) + np.asarray(col, np.int64) // tile
+    tb = np.empty((n, nt + 1), np.int32)
+    step = max(1, (1 << 22) // (nt + 1))
+    for r0 in range(0, n, step):
+        r1 = min(n, r0 + step)
+        q = (np.arange(r0, r1, dtype=np.int64)[:, None] * (nt + 1) + np.arange(nt + 1)[None, :]).ravel()
+        tb[r0:r1] = np.searchsorted(key, q, side="left").reshape(r1 - r0, nt + 1)
+    upost = np.unique(np.asarray(pl_post, np.int64))
+    ptg_ptr = np.concatenate([[0], np.cumsum(np.bincount(upost // tile, minlength=nt))]).astype(np.int32)
+    return tb, ptg_ptr, upost.astype(np.int32)
+
+
 class BatchBrain:
     """``batch`` flies with shared weights and private plastic synapses.
 
@@ -39,7 +57,13 @@ class BatchBrain:
 
     def __init__(self, weights: sp.csr_matrix, params: LIFParams | None = None, batch: int = 1,
                  seed: int = 0, plastic_pos: np.ndarray | None = None, frac_bits: int | None = None,
-                 device=None):
+                 device=None, engine: str = "push", tile: int = 1024):
+        """``engine``: ``"push"`` (spikes scattered into an int32 delay ring with global atomics) or
+        ``"tile"`` (arrivals pulled into LDS per ``tile`` neurons from per-step spike lists; fixed
+        weights are then immutable, which the reference's ``BatchBrain`` also assumes)."""
+        if engine not in ("push", "tile"):
+            raise ValueError(f"engine must be 'push' or 'tile', got {engine!r}")
+        self.engine = engine
         if not torch.cuda.is_available():
             raise RuntimeError("flyhip.BatchBrain needs a HIP device (there is no CPU fallback)")
         self._lib = lib = _lib.load()
@@ -110,7 +134,12 @@ class BatchBrain:
         self.u = torch.zeros((batch, n), dtype=torch.float32, device=dev)
         self.g = torch.zeros((batch, n), dtype=torch.float32, device=dev)
         self.ref = torch.zeros((batch, n), dtype=torch.int8, device=dev)
-        self.ring = torch.zeros((batch, self.n_delay, n), dtype=torch.int32, device=dev)
+        self.ring_slots = 2 * self.n_delay if engine == "tile" else self.n_delay
+        self.ring = torch.zeros((batch, self.ring_slots, n), dtype=torch.int32, device=dev)
+        self.spk_idx = self.spk_cnt = None
+        if engine == "tile":   # in-flight spikes: per fly, per step (3 n_delay slots), neuron ids
+            self.spk_idx = torch.zeros((batch, 3 * self.n_delay, n), dtype=torch.int32, device=dev)
+            self.spk_cnt = torch.zeros((batch, 3 * self.n_delay), dtype=torch.int32, device=dev)
         self.counts = torch.zeros((batch, n), dtype=torch.int32, device=dev)
         self.clock = torch.zeros(4, dtype=torch.int64, device=dev)
 
@@ -126,10 +155,21 @@ class BatchBrain:
         self._h = h
         _lib.check(lib.fz_set_plastic(h, len(pos), _lib.ptr(self.d_pl_indptr) if len(pos) else None,
                                       _lib.ptr(self.d_pl_post), _lib.stream_handle(dev)), "fz_set_plastic")
+        if engine == "tile":
+            self.tile = int(tile)
+            tb, ptg_ptr, ptg_idx = tile_tables(fixed_indptr, self.indices[keep], n, self.tile, self.pl_post)
+            self.d_tb = torch.from_numpy(tb).to(dev)
+            self.d_ptg_ptr = torch.from_numpy(ptg_ptr).to(dev)
+            self.d_ptg_idx = torch.from_numpy(ptg_idx).to(dev)
+            torch.cuda.synchronize(dev)
+            _lib.check(lib.fz_set_tiles(h, self.tile, tb.shape[1] - 1, _lib.ptr(self.d_tb),
+                                        _lib.ptr(self.d_ptg_ptr) if len(pos) else None,
+                                        _lib.ptr(self.d_ptg_idx) if len(pos) else None), "fz_set_tiles")
         self._st = _lib.FzState(batch, self.u.data_ptr(), self.g.data_ptr(), self.ref.data_ptr(),
-                                self.ring.data_ptr(), self.counts.data_ptr(), self.clock.data_ptr())
+                                self.ring.data_ptr(), self.counts.data_ptr(), self.clock.data_ptr(),
+                                self.ring_slots, _lib.ptr(self.spk_idx), _lib.ptr(self.spk_cnt))
         _lib.check(lib.fz_bind(h, ctypes.byref(self._st)), "fz_bind")
-        self.tune = _lib.FzTune(0, 0, 0, 0, 0)
+        self.tune = _lib.FzTune(0, 0, 0, 0, 0, 0)
         self._graphs: dict = {}
 
     # ------------------------------------------------------------------ clock
@@ -158,7 +198,9 @@ class BatchBrain:
 
     @property
     def pending(self) -> torch.Tensor:
-        """Delay ring in mV, (n_delay, B, n) like the reference; a read-only copy."""
+        """Delay ring in mV, (n_delay, B, n) like the reference; a read-only copy (push engine)."""
+        if self.engine != "push":
+            raise AttributeError("the tile engine keeps in-flight spikes as lists (spk_idx/spk_cnt), not a ring")
         return (self.ring.to(torch.float32) * float(self.in_scale)).transpose(0, 1).contiguous()
 
     # ------------------------------------------------------------------ state
@@ -169,6 +211,8 @@ class BatchBrain:
             self.g.zero_()
             self.ref.zero_()
             self.ring.zero_()
+            if self.spk_cnt is not None:
+                self.spk_cnt.zero_()
             self.t = 0
             return
         f = torch.as_tensor(np.atleast_1d(flies), dtype=torch.long, device=self.device)
@@ -176,6 +220,8 @@ class BatchBrain:
         self.g[f] = 0.0
         self.ref[f] = 0
         self.ring[f] = 0
+        if self.spk_cnt is not None:
+            self.spk_cnt[f] = 0
 
     def slots(self, idx) -> torch.Tensor:
         """Ext slots of neurons ``idx`` (adding any that don't have one yet)."""
@@ -284,8 +330,8 @@ class BatchBrain:
         """Raise if a device-side capacity check fired (waits for the stream)."""
         _lib.check(self._lib.fz_check(self._h, _lib.stream_handle(self.device)), "fz_check")
 
-    def set_tune(self, upd_threads=0, push_threads=0, push_chunk=0, push_grid=0, block_steps=0):
-        self.tune = _lib.FzTune(upd_threads, push_threads, push_chunk, push_grid, block_steps)
+    def set_tune(self, upd_threads=0, push_threads=0, push_chunk=0, push_grid=0, block_steps=0, tile_stage=0):
+        self.tune = _lib.FzTune(upd_threads, push_threads, push_chunk, push_grid, block_steps, tile_stage)
         self._drop_graphs()
 
     def _graph(self, steps: int, inp: _lib.FzInputs):

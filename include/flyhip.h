@@ -53,6 +53,9 @@ typedef struct fz_state {
     int* ring;          /* [B][n_delay][n] int32 delay ring, fixed-point synapse counts */
     int* counts;        /* [B][n] spike counts of the last window */
     long long* clock;   /* [4]: t (ring phase), step_no (RNG step), block parity, reserved */
+    int ring_slots;     /* n_delay (push engine) or 2 * n_delay (tile engine) */
+    int* spk_idx;       /* tile engine: [B][3 n_delay][n] spiking neurons per step (else NULL) */
+    int* spk_cnt;       /* tile engine: [B][3 n_delay] list lengths, zero at rest (else NULL) */
 } fz_state;
 
 /* Per-window inputs (flyzero/gpu.py:174-179, 201-226; plastic weights gpu.py:162-163). */
@@ -73,6 +76,7 @@ typedef struct fz_tune {
     int push_chunk;    /* edges per push work item */
     int push_grid;     /* push workgroups (grid-stride) */
     int block_steps;   /* steps per temporal block, 1..min(n_delay, 32) */
+    int tile_stage;    /* tile engine: spike-list entries staged per thread, 1 or 4 */
 } fz_tune;
 
 int fz_version(void);
@@ -92,6 +96,14 @@ int fz_set_plastic(fz_engine* e, int n_pl, const int* pl_indptr, const int* pl_p
 
 /* Bind caller state for B flies and size the scratch for it. */
 int fz_bind(fz_engine* e, const fz_state* st);
+
+/* Switch the engine to arrivals pulled per neuron tile (call before fz_bind).  tile: neurons
+ * per workgroup (multiple of 64); tb int32[n][n_tiles+1]: first fixed edge of row i whose
+ * post neuron is >= t*tile (columns sorted within rows); ptg_ptr int32[n_tiles+1] / ptg_idx:
+ * the plastic synapses' post neurons of each tile (NULL without plastic synapses).  The ring
+ * then has 2*n_delay slots and carries plastic arrivals only; fixed arrivals come from the
+ * per-step spike lists in fz_state.  Fixed weights must not change after this call. */
+int fz_set_tiles(fz_engine* e, int tile, int n_tiles, const int* tb, const int* ptg_ptr, const int* ptg_idx);
 
 /* Advance every live fly `steps` steps (one window; flyzero/gpu.py:228-254,
  * flyzero/brain.py:133-172).  counts are overwritten with this window's spike

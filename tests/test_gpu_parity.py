@@ -33,11 +33,15 @@ def _oracle_counts(w, dt=0.25, pw=None, plastic_pos=None, ms=50):
     return ob.run(ms)
 
 
-def test_gpu_matches_oracle_without_noise():
+ENGINES = ["push", "tile"]
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+def test_gpu_matches_oracle_without_noise(engine):
     # tests/test_gpu.py:39-47
     w = ref_weights()
     ref = _oracle_counts(w)[0]
-    gb = BatchBrain(w, LIFParams(dt=0.25), batch=3)
+    gb = BatchBrain(w, LIFParams(dt=0.25), batch=3, engine=engine)
     gb.v[:, :40] = -40.0
     out = _np(gb.run(50))
     assert ref.sum() > 50
@@ -45,13 +49,14 @@ def test_gpu_matches_oracle_without_noise():
         np.testing.assert_array_equal(out[k], ref)
 
 
-def test_each_fly_has_its_own_plastic_synapses():
+@pytest.mark.parametrize("engine", ENGINES)
+def test_each_fly_has_its_own_plastic_synapses(engine):
     # tests/test_gpu.py:50-62
     w = ref_weights()
     ws = sp.csr_matrix(w, dtype=np.float32)
     ws.sort_indices()
     pos = np.flatnonzero(ws.indices < 30)          # synapses onto neurons 0..29
-    gb = BatchBrain(w, LIFParams(dt=0.25), batch=2, plastic_pos=pos)
+    gb = BatchBrain(w, LIFParams(dt=0.25), batch=2, plastic_pos=pos, engine=engine)
     changed = gb.data.copy()
     changed[pos] = np.abs(changed[pos]) * 3
     gb.pw[1] = torch.from_numpy(changed[pos]).cuda()
@@ -63,7 +68,8 @@ def test_each_fly_has_its_own_plastic_synapses():
     assert not np.array_equal(out[0], out[1])
 
 
-def test_plastic_path_equals_fixed_path_for_whole_synapse_counts():
+@pytest.mark.parametrize("engine", ENGINES)
+def test_plastic_path_equals_fixed_path_for_whole_synapse_counts(engine):
     # with whole synapse counts the plastic weights quantise exactly, so a fly whose plastic
     # weights are untouched spikes exactly like a brain without plastic synapses
     w = ref_weights()
@@ -71,17 +77,18 @@ def test_plastic_path_equals_fixed_path_for_whole_synapse_counts():
     ws = sp.csr_matrix(w, dtype=np.float32)
     ws.sort_indices()
     pos = np.flatnonzero(ws.indices < 150)
-    gb = BatchBrain(w, LIFParams(dt=0.25), batch=1, plastic_pos=pos)
+    gb = BatchBrain(w, LIFParams(dt=0.25), batch=1, plastic_pos=pos, engine=engine)
     assert gb.frac_bits == 0 and gb.w_bits == 16
     gb.v[:, :40] = -40.0
     out = _np(gb.run(50))
     np.testing.assert_array_equal(out, _oracle_counts(w))
 
 
-def test_poisson_input_rate():
+@pytest.mark.parametrize("engine", ENGINES)
+def test_poisson_input_rate(engine):
     # tests/test_gpu.py:65-72; 100 Hz with 2.2 ms dead time after each spike
     w = sp.csr_matrix((10, 10), dtype=np.float32)
-    gb = BatchBrain(w, LIFParams(dt=0.25), batch=2, seed=5)
+    gb = BatchBrain(w, LIFParams(dt=0.25), batch=2, seed=5, engine=engine)
     gb.set_input(np.arange(10), 100.0)
     c = _np(gb.run(1000))
     assert 70 < c.mean() < 105
@@ -91,13 +98,14 @@ def test_poisson_input_rate():
     np.testing.assert_array_equal(c, ob.run(1000))
 
 
-def test_long_plastic_rows_cover_every_synapse():
+@pytest.mark.parametrize("engine", ENGINES)
+def test_long_plastic_rows_cover_every_synapse(engine):
     # tests/test_gpu.py:168-183: every synapse plastic, rows of >300 synapses
     rng = np.random.default_rng(4)
     w = sp.random(600, 600, density=0.6, random_state=4, format="csr")
     w.data = rng.normal(5.0, 3, w.nnz)
     pos = np.arange(w.nnz)
-    gb = BatchBrain(w, LIFParams(dt=0.25), batch=2, plastic_pos=pos)
+    gb = BatchBrain(w, LIFParams(dt=0.25), batch=2, plastic_pos=pos, engine=engine)
     assert np.diff(gb.indptr).max() > 300
     gb.pw[1] *= 0.5
     gb.v[:, :40] = -40.0
@@ -116,9 +124,9 @@ def small_fly():
     return flywire_like(n=20_000, n_edges=20_000 * 109, seed=11)
 
 
-def _pair(fly, batch=1, seed=0, plastic_pos=None):
+def _pair(fly, batch=1, seed=0, plastic_pos=None, engine="push", tile=1024):
     p = LIFParams()
-    gb = BatchBrain(fly.weights, p, batch=batch, seed=seed, plastic_pos=plastic_pos)
+    gb = BatchBrain(fly.weights, p, batch=batch, seed=seed, plastic_pos=plastic_pos, engine=engine, tile=tile)
     ob = OracleBrain(fly.weights, p, batch=batch, seed=seed, plastic_pos=plastic_pos)
     idx, rate = fly_inputs(fly)
     gb.set_input(idx, rate)
@@ -130,11 +138,13 @@ def _assert_state_equal(gb, ob):
     np.testing.assert_array_equal(_np(gb.u), ob.u)
     np.testing.assert_array_equal(_np(gb.g), ob.g)
     np.testing.assert_array_equal(_np(gb.ref), ob.ref)
-    np.testing.assert_array_equal(_np(gb.ring), ob.ring)
+    if gb.engine == "push":      # the tile engine keeps in-flight spikes as lists, not a ring
+        np.testing.assert_array_equal(_np(gb.ring), ob.ring)
 
 
-def test_raster_identical_with_precomputed_hits(small_fly):
-    gb, ob = _pair(small_fly, batch=2)
+@pytest.mark.parametrize("engine", ENGINES)
+def test_raster_identical_with_precomputed_hits(small_fly, engine):
+    gb, ob = _pair(small_fly, batch=2, engine=engine)
     steps = 400
     hits = poisson_hits(ob.p_ext[0], steps, 2, seed=3)
     cg, rg = gb.run_steps(steps, hits=torch.from_numpy(hits.view(np.int32)).cuda(), raster=True)
@@ -147,9 +157,10 @@ def test_raster_identical_with_precomputed_hits(small_fly):
     _assert_state_equal(gb, ob)
 
 
-def test_hash_rng_frames_graph_and_eager_match_oracle(small_fly):
-    gb, ob = _pair(small_fly, batch=3, seed=9)
-    ge, _ = _pair(small_fly, batch=3, seed=9)
+@pytest.mark.parametrize("engine", ENGINES)
+def test_hash_rng_frames_graph_and_eager_match_oracle(small_fly, engine):
+    gb, ob = _pair(small_fly, batch=3, seed=9, engine=engine)
+    ge, _ = _pair(small_fly, batch=3, seed=9, engine=engine)
     xy = np.r_[small_fly.input_xy, [[0.5, 0.5]]]
     for frame in range(6):
         rate = np.r_[smooth_field(small_fly.input_xy, frame), np.float32(60.0)]
@@ -166,8 +177,9 @@ def test_hash_rng_frames_graph_and_eager_match_oracle(small_fly):
     assert gb.t == ob.t and gb.step_no == ob.step_no
 
 
-def test_retired_flies_are_skipped_and_others_unaffected(small_fly):
-    gb, ob = _pair(small_fly, batch=4, seed=2)
+@pytest.mark.parametrize("engine", ENGINES)
+def test_retired_flies_are_skipped_and_others_unaffected(small_fly, engine):
+    gb, ob = _pair(small_fly, batch=4, seed=2, engine=engine)
     gb.run(FRAME_MS)
     ob.run(FRAME_MS)
     v_before = _np(gb.u[1]).copy()
@@ -228,12 +240,13 @@ def test_brain_weight_edits_reach_the_engine():
     assert b.run(100)[1] == 0
 
 
-def test_autotuned_shapes_keep_bit_exact_results(small_fly):
+@pytest.mark.parametrize("engine", ENGINES)
+def test_autotuned_shapes_keep_bit_exact_results(small_fly, engine):
     from flyhip import autotune
 
-    gb, ob = _pair(small_fly, batch=2, seed=4)
+    gb, ob = _pair(small_fly, batch=2, seed=4, engine=engine)
     shape = autotune.tune(gb, use_cache=False, reps=1)
-    assert shape["upd_threads"] % 64 == 0 and shape["push_chunk"] >= 64
+    assert shape["upd_threads"] % 64 == 0 and shape.get("push_chunk", 64) >= 64
     for _ in range(3):
         np.testing.assert_array_equal(_np(gb.run(FRAME_MS)), ob.run(FRAME_MS))
     _assert_state_equal(gb, ob)
@@ -248,3 +261,28 @@ def test_extreme_launch_shapes_are_exact(small_fly, shape):
     for _ in range(2):
         np.testing.assert_array_equal(_np(gb.run(FRAME_MS)), ob.run(FRAME_MS))
     _assert_state_equal(gb, ob)
+
+
+@pytest.mark.parametrize("tile,shape", [(512, dict(upd_threads=64, block_steps=5)),
+                                        (2048, dict(upd_threads=256)), (256, dict(upd_threads=256)),
+                                        (1024, dict(upd_threads=1024, block_steps=9, tile_stage=4)),
+                                        (1024, dict(upd_threads=128, tile_stage=4))])
+def test_tile_engine_shapes_are_exact(small_fly, tile, shape):
+    # shapes that the LDS budget admits, incl. 8 neurons per thread and 5-step blocks
+    fly = small_fly
+    idx = np.r_[fly.input_idx, fly.drive_idx]
+    gb, ob = _pair(fly, batch=2, seed=8, engine="tile", tile=tile)
+    gb.set_tune(**shape)
+    for f in range(3):
+        rate = np.r_[smooth_field(fly.input_xy, f), np.float32(60.0)]
+        gb.set_input(idx, rate)
+        ob.set_input(idx, rate)
+        np.testing.assert_array_equal(_np(gb.run(FRAME_MS)), ob.run(FRAME_MS))
+    _assert_state_equal(gb, ob)
+
+
+def test_tile_engine_rejects_shapes_over_the_lds_budget(small_fly):
+    gb, _ = _pair(small_fly, batch=1, engine="tile", tile=2048)
+    gb.set_tune(upd_threads=1024)                     # 18 x 2048 x 4 B + staging > 160 KB
+    with pytest.raises(ValueError, match="LDS"):
+        gb.run(FRAME_MS, graph=False)
