@@ -441,6 +441,22 @@ struct TileArgs {
     Consts c;
 };
 
+// Workgroups are handed to the 8 XCDs round-robin by linear id, and each XCD has its
+// own L2.  Map linear id p to the (tile, fly) pair so that XCD p % 8 receives one
+// contiguous run of the fly-major order: the tiles of a fly then share an L2, so a
+// spike's segment-table row and its row of edges are fetched once per XCD instead of
+// by every XCD.  The map is a bijection onto [0, n_tiles * B) for any dispatch order.
+constexpr int NXCD = 8;
+__device__ __forceinline__ void xcd_order(int n_tiles, int B, int* T, int* b) {
+    const int total = n_tiles * B;
+    const int p = blockIdx.y * gridDim.x + blockIdx.x;
+    const int x = p % NXCD, i = p / NXCD;
+    const int q = total / NXCD, r = total % NXCD;
+    const int L = x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+    *b = L / n_tiles;
+    *T = L - *b * n_tiles;
+}
+
 // Inclusive prefix sum over the workgroup; wsum needs MAX_WAVES + 1 ints.
 __device__ __forceinline__ int block_incl_scan(int v, int* wsum, int* total) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
@@ -479,7 +495,8 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
     int* st_seg = acc + a.K * TS;        // [TPB * IPT]
     int* st_end = st_seg + TPB * IPT;    // [TPB * IPT] inclusive prefix of segment lengths
     int* st_k = st_end + TPB * IPT;      // [TPB * IPT]
-    const int b = blockIdx.y, T = blockIdx.x;
+    int b, T;
+    xcd_order(a.n_tiles, a.B, &T, &b);
     const int i0 = T * TS;
     const size_t bn = (size_t)b * a.N;
     const long long t0 = a.clock[0] + a.kb;

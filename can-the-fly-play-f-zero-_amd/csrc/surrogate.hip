@@ -1,0 +1,319 @@
+// Learned rate surrogate: BF16 MFMA GEMMs (FP32 accumulate) and the element-wise training and
+// inference kernels of a small network that maps pooled visual drive to descending-neuron rates.
+//
+// Network (per frame; batch of independent sequences, one GRU state each):
+//   h1 = gelu(X W1^T + b1)            2048 -> 1024
+//   h2 = gelu(h1 W2^T + b2)           1024 -> 1024
+//   s' = GRU(h2, s)                   1024 -> 256 (PyTorch gate order r, z, n)
+//   y  = [h2, s'] Wo^T + bo           1280 -> 8
+//
+// sg_gemm: C[M][N] = sum_k A[m][k] B[k][n], A and B bf16 (each optionally stored transposed),
+// FP32 accumulation on v_mfma_f32_16x16x32_bf16.  Workgroup tile 64 x 64 x 32 staged in LDS
+// (k-contiguous rows for both operands), four wavefronts of 32 x 32, bounds-checked
+// epilogues: FP32 store, FP32 accumulate (gradient sums), BF16 store, bias + GELU (FP32
+// pre-activation and BF16 activation).  K must be a multiple of 32.
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "errors.h"
+#include "flyhip.h"
+
+namespace {
+
+typedef __attribute__((__vector_size__(8 * sizeof(short)))) short bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+__device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even (NaN kept quiet)
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x7fffffu)) return (uint16_t)((u >> 16) | 0x40);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ float gelu(float x) {  // tanh approximation
+    const float c = 0.7978845608028654f;
+    return 0.5f * x * (1.0f + tanhf(c * (x + 0.044715f * x * x * x)));
+}
+__device__ __forceinline__ float gelu_grad(float x) {
+    const float c = 0.7978845608028654f;
+    const float t = tanhf(c * (x + 0.044715f * x * x * x));
+    return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * c * (1.0f + 3.0f * 0.044715f * x * x);
+}
+__device__ __forceinline__ float sigm(float x) { return 1.0f / (1.0f + expf(-x)); }
+
+constexpr int BM = 64, BN = 64, BK = 32, LDK = BK + 8;  // LDS row stride padded against bank conflicts
+
+struct GemmArgs {
+    int M, N, K;
+    const uint16_t* A;  // [M][K] (lda) or, transposed, [K][M]
+    const uint16_t* B;  // [K][N] (ldb) or, transposed, [N][K]
+    int lda, ldb, ldc;
+    int mode;           // FZ_SG_*
+    float* C32;         // modes STORE32, ACC32; pre-activation for GELU
+    uint16_t* C16;      // modes STORE16, GELU
+    const float* bias;  // [N] or null
+};
+
+template <bool TA, bool TB>
+__global__ void __launch_bounds__(256) sg_gemm(GemmArgs g) {
+    __shared__ uint16_t As[BM][LDK];
+    __shared__ uint16_t Bs[BN][LDK];  // B staged as [n][k]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    for (int k0 = 0; k0 < g.K; k0 += BK) {
+        // A tile: 64 x 32 bf16 = 256 threads x 8
+        if (!TA) {
+            const int r = tid >> 2, c = (tid & 3) * 8, m = m0 + r;
+#pragma unroll
+            for (int q = 0; q < 8; q++) As[r][c + q] = m < g.M ? g.A[(size_t)m * g.lda + k0 + c + q] : 0;
+        } else {
+            const int kr = tid >> 3, c = (tid & 7) * 8;
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const int m = m0 + c + q;
+                As[c + q][kr] = m < g.M ? g.A[(size_t)(k0 + kr) * g.lda + m] : 0;
+            }
+        }
+        if (!TB) {
+            const int kr = tid >> 3, c = (tid & 7) * 8;
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const int n = n0 + c + q;
+                Bs[c + q][kr] = n < g.N ? g.B[(size_t)(k0 + kr) * g.ldb + n] : 0;
+            }
+        } else {
+            const int r = tid >> 2, c = (tid & 3) * 8, n = n0 + r;
+#pragma unroll
+            for (int q = 0; q < 8; q++) Bs[r][c + q] = n < g.N ? g.B[(size_t)n * g.ldb + k0 + c + q] : 0;
+        }
+        __syncthreads();
+        // fragments: lane holds row (lane % 16), k = 8 * (lane / 16) + 0..7
+        const int fr = lane & 15, fk = (lane >> 4) * 8;
+        bf16x8 af[2], bfm[2];
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int q = 0; q < 8; q++) af[a][q] = (short)As[wm + a * 16 + fr][fk + q];
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int q = 0; q < 8; q++) bfm[b][q] = (short)Bs[wn + b * 16 + fr][fk + q];
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int b = 0; b < 2; b++)
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bfm[b], acc[a][b], 0, 0, 0);
+        __syncthreads();
+    }
+    // output layout: lane holds C[4 * (lane / 16) + i][lane % 16]
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int m = m0 + wm + a * 16 + (lane >> 4) * 4 + i;
+                const int n = n0 + wn + b * 16 + (lane & 15);
+                if (m >= g.M || n >= g.N) continue;
+                float v = acc[a][b][i];
+                if (g.bias) v += g.bias[n];
+                const size_t o = (size_t)m * g.ldc + n;
+                switch (g.mode) {
+                    case FZ_SG_STORE32: g.C32[o] = v; break;
+                    case FZ_SG_ACC32: g.C32[o] += v; break;
+                    case FZ_SG_STORE16: g.C16[o] = f2bf(v); break;
+                    default:  // FZ_SG_GELU
+                        g.C32[o] = v;
+                        g.C16[o] = f2bf(gelu(v));
+                }
+            }
+}
+
+// dpre = dact * gelu'(pre); written as FP32 and as BF16 (the next GEMM operand)
+__global__ void sg_gelu_bwd_k(long long n, const float* dact, const float* pre, float* dpre32, uint16_t* dpre16) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float d = dact[i] * gelu_grad(pre[i]);
+    if (dpre32) dpre32[i] = d;
+    if (dpre16) dpre16[i] = f2bf(d);
+}
+
+// column sums: db[n] (+)= sum_m d[m][n]
+__global__ void sg_colsum_k(int M, int N, const float* d, float* db, int accumulate) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    float s = 0.f;
+    for (int m = 0; m < M; m++) s += d[(size_t)m * N + n];
+    db[n] = accumulate ? db[n] + s : s;
+}
+
+// GRU step (PyTorch gate order r, z, n); gx = x W_ih^T + b_ih, gh = h W_hh^T + b_hh, [B][3H]
+__global__ void sg_gru_fwd_k(int B, int H, const float* gx, const float* gh, const float* h, float* hn,
+                             uint16_t* hn16, float* gates) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= B * H) return;
+    const int b = t / H, j = t - b * H;
+    const float* x = gx + (size_t)b * 3 * H;
+    const float* y = gh + (size_t)b * 3 * H;
+    const float r = sigm(x[j] + y[j]);
+    const float z = sigm(x[H + j] + y[H + j]);
+    const float nn = tanhf(x[2 * H + j] + r * y[2 * H + j]);
+    const float hv = (1.0f - z) * nn + z * h[t];
+    hn[t] = hv;
+    if (hn16) hn16[t] = f2bf(hv);
+    if (gates) {  // saved for the backward pass
+        float* gs = gates + (size_t)b * 4 * H;
+        gs[j] = r;
+        gs[H + j] = z;
+        gs[2 * H + j] = nn;
+        gs[3 * H + j] = y[2 * H + j];
+    }
+}
+
+// GRU step backward: from dh' (dhn) to dgx, dgh (FP32 and BF16); the direct term dhn*z is
+// added into dh (which already holds the previous step's gradient from the output layer)
+__global__ void sg_gru_bwd_k(int B, int H, const float* dhn, const float* h, const float* gates, float* dgx,
+                             float* dgh, uint16_t* dgx16, uint16_t* dgh16, float* dh) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= B * H) return;
+    const int b = t / H, j = t - b * H;
+    const float* gs = gates + (size_t)b * 4 * H;
+    const float r = gs[j], z = gs[H + j], nn = gs[2 * H + j], hh = gs[3 * H + j];
+    const float d = dhn[t];
+    const float dn = d * (1.0f - z) * (1.0f - nn * nn);
+    const float dz = d * (h[t] - nn) * z * (1.0f - z);
+    const float dr = dn * hh * r * (1.0f - r);
+    const size_t o = (size_t)b * 3 * H;
+    const float vx[3] = {dr, dz, dn};
+    const float vh[3] = {dr, dz, dn * r};
+    for (int q = 0; q < 3; q++) {
+        dgx[o + q * H + j] = vx[q];
+        dgh[o + q * H + j] = vh[q];
+        if (dgx16) dgx16[o + q * H + j] = f2bf(vx[q]);
+        if (dgh16) dgh16[o + q * H + j] = f2bf(vh[q]);
+    }
+    dh[t] += d * z;
+}
+
+// mean squared error over valid outputs; dy = 2 (y - t) / count; loss accumulated by atomics
+__global__ void sg_mse_k(long long n, const float* y, const float* t, float* dy, float scale, float* loss) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    float l = 0.f;
+    if (i < n) {
+        const float e = y[i] - t[i];
+        dy[i] = 2.0f * e * scale;
+        l = e * e * scale;
+    }
+    for (int o = 32; o > 0; o >>= 1) l += __shfl_down(l, o);
+    if ((threadIdx.x & 63) == 0 && l != 0.f) atomicAdd(loss, l);
+}
+
+// Adam with bias correction; FP32 master weights, BF16 copy for the GEMMs
+__global__ void sg_adam_k(long long n, float* w, const float* grad, float* m, float* v, uint16_t* w16, float lr,
+                          float b1, float b2, float eps, float c1, float c2, float wd) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gr = grad[i] + wd * w[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gr;
+    const float vi = b2 * v[i] + (1.0f - b2) * gr * gr;
+    m[i] = mi;
+    v[i] = vi;
+    const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    w[i] = wi;
+    if (w16) w16[i] = f2bf(wi);
+}
+
+__global__ void sg_cast_k(long long n, const float* x, uint16_t* y) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = f2bf(x[i]);
+}
+
+inline unsigned blocks(long long n, int t) { return (unsigned)((n + t - 1) / t); }
+
+}  // namespace
+
+extern "C" {
+
+int fz_sg_gemm(int M, int N, int K, const void* A, int lda, int trans_a, const void* B, int ldb, int trans_b,
+               int mode, float* C32, void* C16, int ldc, const float* bias, void* stream) {
+    if (M < 1 || N < 1 || K < 1 || K % BK) return fzi_fail(FZ_E_ARG, "gemm %dx%dx%d: K must be a positive multiple of %d", M, N, K, BK);
+    if (!A || !B) return fzi_fail(FZ_E_ARG, "gemm: null operand");
+    if ((mode == FZ_SG_STORE32 || mode == FZ_SG_ACC32 || mode == FZ_SG_GELU) && !C32)
+        return fzi_fail(FZ_E_ARG, "gemm: mode %d needs C32", mode);
+    if ((mode == FZ_SG_STORE16 || mode == FZ_SG_GELU) && !C16) return fzi_fail(FZ_E_ARG, "gemm: mode %d needs C16", mode);
+    if (mode < FZ_SG_STORE32 || mode > FZ_SG_GELU) return fzi_fail(FZ_E_ARG, "gemm: unknown mode %d", mode);
+    GemmArgs g{M, N, K, static_cast<const uint16_t*>(A), static_cast<const uint16_t*>(B), lda, ldb, ldc, mode, C32,
+               static_cast<uint16_t*>(C16), bias};
+    const dim3 grid((N + BN - 1) / BN, (M + BM - 1) / BM);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (!trans_a && !trans_b) hipLaunchKernelGGL((sg_gemm<false, false>), grid, dim3(256), 0, s, g);
+    else if (!trans_a && trans_b) hipLaunchKernelGGL((sg_gemm<false, true>), grid, dim3(256), 0, s, g);
+    else if (trans_a && !trans_b) hipLaunchKernelGGL((sg_gemm<true, false>), grid, dim3(256), 0, s, g);
+    else hipLaunchKernelGGL((sg_gemm<true, true>), grid, dim3(256), 0, s, g);
+    if (hipGetLastError() != hipSuccess) return fzi_fail(FZ_E_HIP, "sg_gemm launch failed");
+    return FZ_OK;
+}
+
+int fz_sg_gelu_bwd(long long n, const float* dact, const float* pre, float* dpre32, void* dpre16, void* stream) {
+    if (n < 1 || !dact || !pre || (!dpre32 && !dpre16)) return fzi_fail(FZ_E_ARG, "gelu_bwd: bad arguments");
+    hipLaunchKernelGGL(sg_gelu_bwd_k, dim3(blocks(n, 256)), dim3(256), 0, static_cast<hipStream_t>(stream), n, dact,
+                       pre, dpre32, static_cast<uint16_t*>(dpre16));
+    return hipGetLastError() == hipSuccess ? FZ_OK : fzi_fail(FZ_E_HIP, "gelu_bwd launch failed");
+}
+
+int fz_sg_colsum(int M, int N, const float* d, float* db, int accumulate, void* stream) {
+    if (M < 1 || N < 1 || !d || !db) return fzi_fail(FZ_E_ARG, "colsum: bad arguments");
+    hipLaunchKernelGGL(sg_colsum_k, dim3(blocks(N, 256)), dim3(256), 0, static_cast<hipStream_t>(stream), M, N, d, db,
+                       accumulate);
+    return hipGetLastError() == hipSuccess ? FZ_OK : fzi_fail(FZ_E_HIP, "colsum launch failed");
+}
+
+int fz_sg_gru_fwd(int B, int H, const float* gx, const float* gh, const float* h, float* hn, void* hn16,
+                  float* gates, void* stream) {
+    if (B < 1 || H < 1 || !gx || !gh || !h || !hn) return fzi_fail(FZ_E_ARG, "gru_fwd: bad arguments");
+    hipLaunchKernelGGL(sg_gru_fwd_k, dim3(blocks((long long)B * H, 256)), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       B, H, gx, gh, h, hn, static_cast<uint16_t*>(hn16), gates);
+    return hipGetLastError() == hipSuccess ? FZ_OK : fzi_fail(FZ_E_HIP, "gru_fwd launch failed");
+}
+
+int fz_sg_gru_bwd(int B, int H, const float* dhn, const float* h, const float* gates, float* dgx, float* dgh,
+                  void* dgx16, void* dgh16, float* dh, void* stream) {
+    if (B < 1 || H < 1 || !dhn || !h || !gates || !dgx || !dgh || !dh) return fzi_fail(FZ_E_ARG, "gru_bwd: bad arguments");
+    hipLaunchKernelGGL(sg_gru_bwd_k, dim3(blocks((long long)B * H, 256)), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       B, H, dhn, h, gates, dgx, dgh, static_cast<uint16_t*>(dgx16), static_cast<uint16_t*>(dgh16), dh);
+    return hipGetLastError() == hipSuccess ? FZ_OK : fzi_fail(FZ_E_HIP, "gru_bwd launch failed");
+}
+
+int fz_sg_mse(long long n, const float* y, const float* t, float* dy, float scale, float* loss, void* stream) {
+    if (n < 1 || !y || !t || !dy || !loss) return fzi_fail(FZ_E_ARG, "mse: bad arguments");
+    hipLaunchKernelGGL(sg_mse_k, dim3(blocks(n, 256)), dim3(256), 0, static_cast<hipStream_t>(stream), n, y, t, dy,
+                       scale, loss);
+    return hipGetLastError() == hipSuccess ? FZ_OK : fzi_fail(FZ_E_HIP, "mse launch failed");
+}
+
+int fz_sg_adam(long long n, float* w, const float* grad, float* m, float* v, void* w16, float lr, float b1, float b2,
+               float eps, float c1, float c2, float wd, void* stream) {
+    if (n < 1 || !w || !grad || !m || !v) return fzi_fail(FZ_E_ARG, "adam: bad arguments");
+    hipLaunchKernelGGL(sg_adam_k, dim3(blocks(n, 256)), dim3(256), 0, static_cast<hipStream_t>(stream), n, w, grad, m,
+                       v, static_cast<uint16_t*>(w16), lr, b1, b2, eps, c1, c2, wd);
+    return hipGetLastError() == hipSuccess ? FZ_OK : fzi_fail(FZ_E_HIP, "adam launch failed");
+}
+
+int fz_sg_cast(long long n, const float* x, void* y, void* stream) {
+    if (n < 1 || !x || !y) return fzi_fail(FZ_E_ARG, "cast: bad arguments");
+    hipLaunchKernelGGL(sg_cast_k, dim3(blocks(n, 256)), dim3(256), 0, static_cast<hipStream_t>(stream), n, x,
+                       static_cast<uint16_t*>(y));
+    return hipGetLastError() == hipSuccess ? FZ_OK : fzi_fail(FZ_E_HIP, "cast launch failed");
+}
+
+}  // extern "C"

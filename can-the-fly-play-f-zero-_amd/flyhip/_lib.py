@@ -66,6 +66,18 @@ SIGNATURES = {
     "fz_propagate": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     "fz_check": (c_int, [c_void_p, c_void_p]),
     "fz_reward_step": (c_int, [c_void_p, c_void_p]),
+    "fz_sg_gemm": (c_int, [c_int, c_int, c_int, c_void_p, c_int, c_int, c_void_p, c_int, c_int, c_int, c_void_p,
+                           c_void_p, c_int, c_void_p, c_void_p]),
+    "fz_sg_gelu_bwd": (c_int, [ctypes.c_longlong, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "fz_sg_colsum": (c_int, [c_int, c_int, c_void_p, c_void_p, c_int, c_void_p]),
+    "fz_sg_gru_fwd": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "fz_sg_gru_bwd": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                              c_void_p, c_void_p]),
+    "fz_sg_mse": (c_int, [ctypes.c_longlong, c_void_p, c_void_p, c_void_p, ctypes.c_float, c_void_p, c_void_p]),
+    "fz_sg_adam": (c_int, [ctypes.c_longlong, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_float,
+                           ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                           ctypes.c_float, c_void_p]),
+    "fz_sg_cast": (c_int, [ctypes.c_longlong, c_void_p, c_void_p, c_void_p]),
     "fz_dn_rates": (c_int, [c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, ctypes.c_float, ctypes.c_float,
                             c_void_p, c_void_p]),
 }

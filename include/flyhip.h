@@ -175,6 +175,40 @@ int fz_reward_step(const fz_reward_args* a, void* stream);
 int fz_dn_rates(int B, int n, const int* counts, int n_groups, const int* gptr, const int* gidx, float hz_scale,
                 float alpha, float* rates, void* stream);
 
+/* ---- learned rate surrogate: BF16 MFMA GEMM and training / inference element-wise kernels ---- */
+
+#define FZ_SG_STORE32 0 /* C32 = A B (+ bias) */
+#define FZ_SG_ACC32 1   /* C32 += A B (+ bias) */
+#define FZ_SG_STORE16 2 /* C16 = bf16(A B (+ bias)) */
+#define FZ_SG_GELU 3    /* C32 = A B + bias (pre-activation), C16 = bf16(gelu(C32)) */
+
+/* C[M][N] = sum_k A[m][k] B[k][n]; A, B bf16 (raw uint16 bit patterns), FP32 accumulation on
+ * MFMA.  trans_a: A is stored [K][M] (lda = row stride); trans_b: B is stored [N][K].
+ * K must be a multiple of 32; M and N are free.  No counterpart in the reference (its
+ * nearest ancestor is the PyTorch MLP of flyzero/implant.py:136-154). */
+int fz_sg_gemm(int M, int N, int K, const void* A, int lda, int trans_a, const void* B, int ldb, int trans_b,
+               int mode, float* C32, void* C16, int ldc, const float* bias, void* stream);
+/* dpre = dact * gelu'(pre) (tanh-approximated GELU), as FP32 and/or BF16. */
+int fz_sg_gelu_bwd(long long n, const float* dact, const float* pre, float* dpre32, void* dpre16, void* stream);
+/* db[n] (+)= sum_m d[m][n] over a row-major [M][N] array. */
+int fz_sg_colsum(int M, int N, const float* d, float* db, int accumulate, void* stream);
+/* One GRU step (gate order r, z, n): gx = x W_ih^T + b_ih, gh = h W_hh^T + b_hh, [B][3H];
+ * hn = new state (FP32, optionally BF16); gates [B][4H] = (r, z, n, gh_n) for the backward. */
+int fz_sg_gru_fwd(int B, int H, const float* gx, const float* gh, const float* h, float* hn, void* hn16,
+                  float* gates, void* stream);
+/* Backward of one GRU step: dgx, dgh [B][3H] (FP32, optionally BF16) and dh += dhn * z (the
+ * direct term is added into dh; the caller adds dgh W_hh). */
+int fz_sg_gru_bwd(int B, int H, const float* dhn, const float* h, const float* gates, float* dgx, float* dgh,
+                  void* dgx16, void* dgh16, float* dh, void* stream);
+/* loss += scale * sum (y - t)^2 (device float); dy = 2 scale (y - t). */
+int fz_sg_mse(long long n, const float* y, const float* t, float* dy, float scale, float* loss, void* stream);
+/* Adam with bias correction factors c1 = 1 - b1^t, c2 = 1 - b2^t and decoupled-from-nothing
+ * L2 weight decay wd; w16 receives the BF16 copy. */
+int fz_sg_adam(long long n, float* w, const float* grad, float* m, float* v, void* w16, float lr, float b1, float b2,
+               float eps, float c1, float c2, float wd, void* stream);
+/* y = bf16(x). */
+int fz_sg_cast(long long n, const float* x, void* y, void* stream);
+
 /* Reads (and clears) the device-side error flag; waits for `stream`. */
 int fz_check(fz_engine* e, void* stream);
 

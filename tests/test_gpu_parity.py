@@ -283,6 +283,6 @@ def test_tile_engine_shapes_are_exact(small_fly, tile, shape):
 
 def test_tile_engine_rejects_shapes_over_the_lds_budget(small_fly):
     gb, _ = _pair(small_fly, batch=1, engine="tile", tile=2048)
-    gb.set_tune(upd_threads=1024)                     # 18 x 2048 x 4 B + staging > 160 KB
+    gb.set_tune(upd_threads=1024, tile_stage=4)       # 18 x 2048 x 4 B + 3 x 4 x 1024 x 4 B > 160 KB
     with pytest.raises(ValueError, match="LDS"):
         gb.run(FRAME_MS, graph=False)

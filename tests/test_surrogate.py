@@ -1,0 +1,177 @@
+"""Learned rate surrogate: the FP64 oracle is pinned by finite differences; the GPU GEMM by
+numpy; the GPU forward/backward (BF16 MFMA, FP32 accumulation) by the oracle with the same
+BF16 rounding points."""
+
+import numpy as np
+import pytest
+
+from oracle.surrogate_oracle import bf16, forward_backward
+
+
+def _params(n_in, hid, H, n_out, pad, rng, scale=0.3):
+    P = {"W1": rng.normal(0, scale, (hid, n_in)), "b1": rng.normal(0, 0.1, hid),
+         "W2": rng.normal(0, scale, (hid, hid)), "b2": rng.normal(0, 0.1, hid),
+         "Wih": rng.normal(0, scale, (3 * H, hid)), "bih": rng.normal(0, 0.1, 3 * H),
+         "Whh": rng.normal(0, scale, (3 * H, H)), "bhh": rng.normal(0, 0.1, 3 * H),
+         "Wo": np.zeros((pad, hid + H)), "bo": np.zeros(pad)}
+    P["Wo"][:n_out] = rng.normal(0, scale, (n_out, hid + H))
+    P["bo"][:n_out] = rng.normal(0, 0.1, n_out)
+    return P
+
+
+def test_oracle_backward_matches_finite_differences():
+    rng = np.random.default_rng(0)
+    n_in, hid, H, n_out, T, B = 6, 5, 4, 3, 3, 2
+    P = _params(n_in, hid, H, n_out, n_out, rng)
+    X = rng.normal(0, 1, (T, B, n_in))
+    h0 = rng.normal(0, 0.5, (B, H))
+    Y = rng.normal(0, 1, (T, B, n_out))
+    loss, g, _ = forward_backward(P, X, h0, Y, n_out)
+    eps = 1e-6
+    for name in P:
+        flat = P[name].ravel()
+        for idx in rng.choice(flat.size, min(6, flat.size), replace=False):
+            old = flat[idx]
+            flat[idx] = old + eps
+            lp = forward_backward(P, X, h0, Y, n_out)[0]
+            flat[idx] = old - eps
+            lm = forward_backward(P, X, h0, Y, n_out)[0]
+            flat[idx] = old
+            num = (lp - lm) / (2 * eps)
+            assert abs(num - g[name].ravel()[idx]) < 1e-6 + 1e-5 * abs(num), (name, idx, num, g[name].ravel()[idx])
+
+
+def test_bf16_rounding_matches_torch():
+    import torch
+
+    x = np.random.default_rng(1).normal(0, 3, 10000).astype(np.float32)
+    x[:4] = [1.00390625, 1.01171875, -2.0, 0.0]          # ties round to even
+    want = torch.from_numpy(x).to(torch.bfloat16).to(torch.float32).numpy()
+    assert np.array_equal(bf16(x).astype(np.float32), want)
+
+
+# ------------------------------------------------------------------------------------- GPU
+
+
+def _bf(t):
+    import torch
+
+    return torch.from_numpy(np.asarray(t, np.float32)).cuda().to(torch.bfloat16)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+def test_mfma_gemm_matches_numpy(ta, tb):
+    import torch
+
+    from flyhip import _lib
+
+    lib = _lib.load()
+    rng = np.random.default_rng(2)
+    M, N, K = 70, 45, 96                                  # ragged M, N; K a multiple of 32
+    A = bf16(rng.normal(0, 1, (M, K)))
+    Bm = bf16(rng.normal(0, 1, (K, N)))
+    bias = rng.normal(0, 1, N).astype(np.float32)
+    As = _bf(A.T.copy() if ta else A)
+    Bs = _bf(Bm.T.copy() if tb else Bm)
+    lda = M if ta else K
+    ldb = K if tb else N
+    want = A @ Bm + bias
+    c32 = torch.zeros((M, N), dtype=torch.float32, device="cuda")
+    c16 = torch.zeros((M, N), dtype=torch.bfloat16, device="cuda")
+    bt = torch.from_numpy(bias).cuda()
+    s = _lib.stream_handle()
+    _lib.check(lib.fz_sg_gemm(M, N, K, As.data_ptr(), lda, ta, Bs.data_ptr(), ldb, tb, 0, c32.data_ptr(), None, N,
+                              bt.data_ptr(), s), "gemm")
+    np.testing.assert_allclose(c32.cpu().numpy(), want, rtol=1e-5, atol=1e-4)
+    _lib.check(lib.fz_sg_gemm(M, N, K, As.data_ptr(), lda, ta, Bs.data_ptr(), ldb, tb, 1, c32.data_ptr(), None, N,
+                              None, s), "gemm acc")
+    np.testing.assert_allclose(c32.cpu().numpy(), want + (want - bias), rtol=1e-5, atol=2e-4)
+    _lib.check(lib.fz_sg_gemm(M, N, K, As.data_ptr(), lda, ta, Bs.data_ptr(), ldb, tb, 3, c32.data_ptr(),
+                              c16.data_ptr(), N, bt.data_ptr(), s), "gemm gelu")
+    from oracle.surrogate_oracle import gelu
+
+    np.testing.assert_allclose(c16.float().cpu().numpy(), bf16(gelu(c32.cpu().numpy())), rtol=1e-2, atol=1e-2)
+
+
+@pytest.mark.gpu
+def test_forward_backward_matches_oracle_with_bf16_points():
+    import torch
+
+    from flyhip.surrogate import N_OUT_PAD, RateSurrogate
+
+    n_in, hid, H, n_out, T, B = 64, 64, 32, 8, 4, 8
+    m = RateSurrogate(n_in, hid, H, n_out, seed=3)
+    P = m.params()
+    rng = np.random.default_rng(4)
+    X = bf16(rng.normal(0, 1, (T, B, n_in)))
+    h0 = rng.normal(0, 0.3, (B, H)).astype(np.float32)
+    Y = rng.normal(0, 1, (T, B, n_out)).astype(np.float32)
+    y, _ = m.forward(_bf(X), torch.from_numpy(h0).cuda(), train=True)
+    loss = float(m.loss_backward(_bf(X), torch.from_numpy(Y).cuda()).item())
+    Yp = np.zeros((T, B, N_OUT_PAD))
+    Yp[..., :n_out] = Y
+    want_loss, g, want_y = forward_backward(P, X, h0.astype(np.float64), Yp, n_out, bf16_ops=True)
+    np.testing.assert_allclose(y.cpu().numpy(), want_y[..., :n_out], rtol=1e-3, atol=1e-3)
+    assert abs(loss - want_loss) < 1e-4 * abs(want_loss) + 1e-6
+    G = m.grads()
+    for name in P:
+        a, b = G[name], g[name]
+        rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-12)
+        assert rel < 2e-3, (name, rel)
+
+
+@pytest.mark.gpu
+def test_training_reduces_loss_on_a_recurrent_target():
+    import torch
+
+    from flyhip.surrogate import RateSurrogate
+
+    n_in, T, B = 64, 8, 32
+    m = RateSurrogate(n_in, 128, 64, 8, seed=5)
+    rng = np.random.default_rng(6)
+    Wt = rng.normal(0, 0.3, (n_in, 8))
+    losses = []
+    for it in range(150):
+        X = rng.normal(0, 1, (T, B, n_in))
+        lin = X @ Wt
+        Y = np.tanh(lin + 0.5 * np.concatenate([np.zeros((1, B, 8)), lin[:-1]], 0))   # depends on the past frame
+        m.forward(_bf(X), torch.zeros((B, 64), device="cuda"), train=True)
+        losses.append(float(m.loss_backward(_bf(X), torch.from_numpy(Y.astype(np.float32)).cuda()).item()))
+        m.adam(lr=2e-3)
+    assert np.mean(losses[-10:]) < 0.2 * np.mean(losses[:10]), (losses[:3], losses[-3:])
+
+
+def test_grid_pool_is_the_cell_mean():
+    import torch
+
+    from flyhip.surrogate_pipeline import GridPool
+
+    rng = np.random.default_rng(7)
+    xy = rng.random((500, 2))
+    rates = rng.random((3, 500)).astype(np.float32)
+    got = GridPool(xy, nx=8, ny=4, device="cpu")(torch.from_numpy(rates)).numpy()
+    cell = np.minimum((xy[:, 1] * 4).astype(int), 3) * 8 + np.minimum((xy[:, 0] * 8).astype(int), 7)
+    for c in range(32):
+        m = cell == c
+        want = rates[:, m].mean(1) if m.any() else np.zeros(3)
+        np.testing.assert_allclose(got[:, c], want, rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.gpu
+def test_surrogate_pipeline_end_to_end_small():
+    from flyhip.connectome import flywire_like
+    from flyhip.surrogate import Ensemble
+    from flyhip.surrogate_pipeline import generate, judge, train_ensemble
+
+    fly = flywire_like(n=4000, n_edges=4000 * 60, seed=11)
+    X, Y, fps = generate(fly, frames=40, batch=16, seed=1, skip=2)
+    assert X.shape == (40, 16, 2048) and Y.shape == (40, 16, 8) and fps > 0
+    assert np.isfinite(Y.cpu().numpy()).all() and float(Y.sum()) > 0
+    members, curves = train_ensemble(X, Y, members=2, T=8, seqs=16, steps=60, lr=2e-3)
+    assert curves[0][-1][1] < curves[0][0][1]
+    Xt, Yt, _ = generate(fly, frames=20, batch=16, seed=5, replicas=4, skip=2)
+    res = judge(Ensemble(members), Xt, Yt, replicas=4, const=Y.mean((0, 1)))
+    for k in ("err_surrogate", "err_with_fallback", "fallback_frac", "err_single_sim_trial", "err_constant_predictor"):
+        assert np.isfinite(res[k]), (k, res)
+    assert 0.0 <= res["fallback_frac"] <= 1.0
