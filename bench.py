@@ -110,6 +110,48 @@ def measure_line(fly, p, B, engine, tile, frames, warmup, torch, gain=None, driv
     return res
 
 
+def profile_run(fly, p, spec, torch, frames=3):
+    """Counter runs (rocprofv3 --pmc): one engine configuration for a few eager frames at a fixed
+    launch shape, then a plain 1 GiB device-to-device copy whose bytes are known, which scales
+    the counters.  spec = engine:B:upd_threads:block_steps:tile_stage, e.g. tile1024:1:256:18:1."""
+    from flyhip import BatchBrain
+
+    eng, B, ut, K, stage = spec.split(":")
+    B = int(B)
+    engine, tile = ("push", 0) if eng == "push" else ("tile", int(eng[4:]))
+    rng = np.random.default_rng(5)
+    drive = rng.uniform(20.0, 120.0, B).astype(np.float32) if B > 1 else None
+    gain = rng.uniform(0.5, 2.0, B).astype(np.float32) if B > 1 else None
+    bb = BatchBrain(fly.weights, p, batch=B, seed=1 if B > 1 else 0, engine=engine, tile=tile)
+    idx = np.r_[fly.input_idx, fly.drive_idx]
+    j = bb.slots(idx)
+    probs = torch.from_numpy(frame_rates(fly, frames + 2, gain, drive)).cuda() * float(np.float32(p.dt / 1000.0))
+    bb.set_tune(upd_threads=int(ut), block_steps=int(K), tile_stage=int(stage))
+    deg = np.diff(bb.indptr).astype(np.float64)
+    steps = bb.steps_of(FRAME_MS)
+    spikes = push_bytes = 0.0
+    for f in range(frames + 2):
+        bb.p_ext[:, j] = probs[f]
+        bb.run(FRAME_MS, graph=False, copy=False)
+        if f >= 2:
+            c = bb.counts.to(torch.float64).sum(0).cpu().numpy()
+            push_bytes += float((c * (8.0 + 10.0 * deg)).sum())
+            spikes += float(c.sum())
+    torch.cuda.synchronize()
+    x = torch.ones(256 << 20, dtype=torch.int32, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    launches = frames * -(-steps // int(K))
+    upd_bytes = frames * steps * B * (26.0 * bb.n + 4.0 * len(idx))
+    print(json.dumps({"profile": spec, "frames_counted": frames, "launches_counted": launches,
+                      "algorithmic_bytes_per_launch": round((upd_bytes + push_bytes) / launches),
+                      "spikes_per_step_per_fly": round(spikes / (frames * steps * B), 1),
+                      "copy_bytes_read": x.numel() * 4, "copy_bytes_written": y.numel() * 4,
+                      "note": "the first 2 frames (warm-up) are dispatched but not counted here"}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -121,6 +163,7 @@ def main():
     ap.add_argument("--enginesB", default="tile1024,tile2048,push")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample length")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile", default="", help="counter-run mode: engine:B:upd_threads:block_steps:tile_stage")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the LIF path shards flies across GPUs with no exchange, and "
@@ -136,6 +179,9 @@ def main():
     log(f"connectome: n={fly.n} nnz={fly.weights.nnz} ({time.perf_counter() - t:.1f}s)")
     p = LIFParams()
     steps_per_frame = int(round(FRAME_MS / p.dt))
+    if args.profile:
+        profile_run(fly, p, args.profile, torch)
+        return
 
     def parse(spec):
         out = []

@@ -34,6 +34,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "errors.h"
@@ -313,6 +314,24 @@ struct PushArgs {
 __device__ __forceinline__ int wval(int16_t x, int s) { return (int)x << s; }
 __device__ __forceinline__ int wval(int32_t x, int) { return x; }
 
+// Tag for the tile engine's packed fixed edges: one int32 per edge holding the integer
+// synaptic value in the high (32 - lbits) bits and the target's offset inside its tile
+// (post % TS < 2^lbits) in the low bits, so an edge costs one 4-byte load instead of a
+// column load and a weight load from two arrays.
+struct PackedEdge {};
+
+template <typename WT>
+__global__ void tile_pack(long long nnz, const int* col, const WT* w, int frac_bits, int ts, int lbits, int* pk,
+                          int* over) {
+    const long long lim = 1ll << (31 - lbits);
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < nnz;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int v = wval(w[e], frac_bits);
+        if (v >= lim || v < -lim) *over = 1;
+        pk[e] = (int)(((unsigned)v << lbits) | (unsigned)(col[e] % ts));
+    }
+}
+
 // Scatter fly blockIdx.y's spikes of the block (flyzero/gpu.py:98-116), one
 // edge chunk per workgroup iteration.
 template <typename WT>
@@ -428,6 +447,8 @@ struct TileArgs {
     const int* col;
     const void* w;
     int frac_bits;
+    const int* pk;         // packed fixed edges (value << lbits | post % TS), or null
+    int lbits;
     const int* tb;         // [N][n_tiles + 1] first fixed edge of row i with post >= tile * TS
     const int* pl_indptr;  // null without plastic synapses
     const int* pl_post;
@@ -532,7 +553,8 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
     }
     __syncthreads();
     const int total = s_off[a.K];
-    const WT* w = static_cast<const WT*>(a.w);
+    const WT* w = static_cast<const WT*>(a.w);  // unused with packed edges
+    const int lmask = (1 << a.lbits) - 1;
     // Entries in chunks of TPB * IPT; each thread owns IPT consecutive entries so that all of
     // its list and table loads are in flight together.
     for (int base = 0; base < total; base += TPB * IPT) {
@@ -589,19 +611,25 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
                     slot[u] = st_k[lo] * TS;
                 }
             }
-            int post[UNROLL], val[UNROLL];
+            int loc[UNROLL], val[UNROLL];
 #pragma unroll
             for (int u = 0; u < UNROLL; u++) {
-                post[u] = 0;
+                loc[u] = 0;
                 val[u] = 0;
                 if (ek[u] >= 0) {
-                    post[u] = a.col[ek[u]];
-                    val[u] = wval(w[ek[u]], a.frac_bits);
+                    if constexpr (std::is_same<WT, PackedEdge>::value) {
+                        const int x = a.pk[ek[u]];  // one load: value and tile-local target
+                        loc[u] = x & lmask;
+                        val[u] = x >> a.lbits;
+                    } else {
+                        loc[u] = a.col[ek[u]] - i0;
+                        val[u] = wval(w[ek[u]], a.frac_bits);
+                    }
                 }
             }
 #pragma unroll
             for (int u = 0; u < UNROLL; u++)
-                if (val[u] != 0) atomicAdd(&acc[slot[u] + (post[u] - i0)], val[u]);
+                if (val[u] != 0) atomicAdd(&acc[slot[u] + loc[u]], val[u]);
         }
         __syncthreads();
     }
@@ -762,6 +790,8 @@ struct fz_engine {
     // tile engine (mode 2)
     int mode = 1;
     int ts = 0, n_tiles = 0;
+    int* pk = nullptr;  // packed fixed edges (null: the kernel reads col and w)
+    int lbits = 0;
     const int* tb = nullptr;
     const int* ptg_ptr = nullptr;
     const int* ptg_idx = nullptr;
@@ -912,6 +942,8 @@ TileArgs make_tile(const fz_engine* e, const fz_inputs* in, int K, int kb, int f
     a.col = e->col;
     a.w = e->w;
     a.frac_bits = e->c.frac_bits;
+    a.pk = e->pk;
+    a.lbits = e->lbits;
     a.tb = e->tb;
     a.pl_indptr = e->n_pl > 0 ? e->pl_indptr : nullptr;
     a.pl_post = e->n_pl > 0 ? e->pl_post : nullptr;
@@ -936,11 +968,14 @@ void* tile_kernel_npt(int npt) {
     }
 }
 
-// The lif_tile instantiation for a weight type, neurons per thread and staging depth.
-void* tile_kernel(int w_bits, int npt, int ipt) {
-    if (w_bits == 16) return ipt == 1 ? tile_kernel_npt<int16_t, 1>(npt) : tile_kernel_npt<int16_t, 4>(npt);
+// The lif_tile instantiation for an edge format (0: int16 col/w, 1: int32 col/w, 2: packed),
+// neurons per thread and staging depth.
+void* tile_kernel(int fmt, int npt, int ipt) {
+    if (fmt == 2) return ipt == 1 ? tile_kernel_npt<PackedEdge, 1>(npt) : tile_kernel_npt<PackedEdge, 4>(npt);
+    if (fmt == 0) return ipt == 1 ? tile_kernel_npt<int16_t, 1>(npt) : tile_kernel_npt<int16_t, 4>(npt);
     return ipt == 1 ? tile_kernel_npt<int32_t, 1>(npt) : tile_kernel_npt<int32_t, 4>(npt);
 }
+int tile_fmt(const fz_engine* e) { return e->pk ? 2 : (e->w_bits == 16 ? 0 : 1); }
 
 size_t tile_lds(const fz_engine* e, const Shape& s, int K) {
     return (size_t)K * e->ts * 4 + (size_t)3 * s.ipt * s.ut * 4;
@@ -964,7 +999,7 @@ int enqueue(fz_engine* e, const fz_inputs* in, int steps, const unsigned* hits, 
     int rc;
     if (e->mode == 2) {
         const dim3 tgrid(e->n_tiles, e->st.B);
-        void* fn = tile_kernel(e->w_bits, e->ts / s.ut, s.ipt);
+        void* fn = tile_kernel(tile_fmt(e), e->ts / s.ut, s.ipt);
         while (kb < steps) {
             const int K = (steps - kb) < s.K ? (steps - kb) : s.K;
             TileArgs ta = make_tile(e, in, K, kb, kb == 0, hits, raster);
@@ -1063,6 +1098,7 @@ int fz_destroy(fz_engine* e) {
     if (!e) return FZ_OK;
     free_worklist(e);
     (void)hipFree(e->err);
+    (void)hipFree(e->pk);
     delete e;
     return FZ_OK;
 }
@@ -1122,6 +1158,56 @@ int fz_bind(fz_engine* e, const fz_state* st) {
     return FZ_OK;
 }
 
+namespace {
+
+// Packed copy of the fixed edges for tile size `tile` (see PackedEdge).  Left null, and the
+// tile kernel reads col and w, when a value does not fit the bits left beside the offset.
+int pack_edges(fz_engine* e, int tile) {
+    (void)hipFree(e->pk);
+    e->pk = nullptr;
+    e->lbits = 0;
+    if (e->nnz == 0) return FZ_OK;
+    int lbits = 0;
+    while ((1 << lbits) < tile) lbits++;
+    int* pk = nullptr;
+    int* over = nullptr;
+    if (hipMalloc(&pk, (size_t)e->nnz * sizeof(int)) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(FZ_E_HIP, "allocating %lld packed edges failed", e->nnz);
+    }
+    if (hipMalloc(&over, sizeof(int)) != hipSuccess || hipMemset(over, 0, sizeof(int)) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipFree(pk);
+        (void)hipFree(over);
+        return fail(FZ_E_HIP, "allocating the pack flag failed");
+    }
+    const unsigned grid = (unsigned)((e->nnz + 255) / 256 < 65536 ? (e->nnz + 255) / 256 : 65536);
+    if (e->w_bits == 16)
+        hipLaunchKernelGGL(tile_pack<int16_t>, dim3(grid), dim3(256), 0, 0, e->nnz, e->col,
+                           static_cast<const int16_t*>(e->w), e->c.frac_bits, tile, lbits, pk, over);
+    else
+        hipLaunchKernelGGL(tile_pack<int32_t>, dim3(grid), dim3(256), 0, 0, e->nnz, e->col,
+                           static_cast<const int32_t*>(e->w), e->c.frac_bits, tile, lbits, pk, over);
+    int h_over = 0;
+    hipError_t r = hipGetLastError();
+    if (r == hipSuccess) r = hipMemcpy(&h_over, over, sizeof(int), hipMemcpyDeviceToHost);
+    (void)hipFree(over);
+    if (r != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipFree(pk);
+        return fail(FZ_E_HIP, "packing edges failed: %s", hipGetErrorString(r));
+    }
+    if (h_over) {
+        (void)hipFree(pk);
+        return FZ_OK;
+    }
+    e->pk = pk;
+    e->lbits = lbits;
+    return FZ_OK;
+}
+
+}  // namespace
+
 int fz_set_tiles(fz_engine* e, int tile, int n_tiles, const int* tb, const int* ptg_ptr, const int* ptg_idx) {
     if (!e) return fail(FZ_E_ARG, "null engine");
     if (tile < 64 || tile > 8192 || tile % 64) return fail(FZ_E_ARG, "tile %d: need a multiple of 64 in [64, 8192]", tile);
@@ -1132,17 +1218,20 @@ int fz_set_tiles(fz_engine* e, int tile, int n_tiles, const int* tb, const int* 
     if (lds > e->lds_max - TILE_STATIC_LDS)
         return fail(FZ_E_ARG, "tile %d needs %zu B of LDS for %d steps; the device allows %zu", tile, lds, e->kmax,
                     e->lds_max - TILE_STATIC_LDS);
-    for (int npt : {1, 2, 4, 8})
-        for (int ipt : {1, 4}) {
-            hipError_t r = hipFuncSetAttribute(tile_kernel(e->w_bits, npt, ipt),
-                                               hipFuncAttributeMaxDynamicSharedMemorySize,
-                                               (int)(e->lds_max - TILE_STATIC_LDS));
-            if (r != hipSuccess) {
-                (void)hipGetLastError();  // do not leave a sticky error for the caller's next HIP call
-                return fail(FZ_E_HIP, "hipFuncSetAttribute(max dynamic LDS %zu) failed: %s",
-                            e->lds_max - TILE_STATIC_LDS, hipGetErrorString(r));
+    for (int fmt : {0, 1, 2})
+        for (int npt : {1, 2, 4, 8})
+            for (int ipt : {1, 4}) {
+                hipError_t r = hipFuncSetAttribute(tile_kernel(fmt, npt, ipt),
+                                                   hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                   (int)(e->lds_max - TILE_STATIC_LDS));
+                if (r != hipSuccess) {
+                    (void)hipGetLastError();  // do not leave a sticky error for the caller's next HIP call
+                    return fail(FZ_E_HIP, "hipFuncSetAttribute(max dynamic LDS %zu) failed: %s",
+                                e->lds_max - TILE_STATIC_LDS, hipGetErrorString(r));
+                }
             }
-        }
+    int rc = pack_edges(e, tile);
+    if (rc) return rc;
     e->mode = 2;
     e->ts = tile;
     e->n_tiles = n_tiles;
@@ -1232,7 +1321,7 @@ int fz_graph_create(fz_engine* e, const fz_inputs* in, int steps, const fz_tune*
     int kb = 0, blk = 0;
     if (e->mode == 2) {
         const dim3 tgrid(e->n_tiles, e->st.B);
-        void* fn = tile_kernel(e->w_bits, e->ts / s.ut, s.ipt);
+        void* fn = tile_kernel(tile_fmt(e), e->ts / s.ut, s.ipt);
         while (kb < steps) {
             const int K = (steps - kb) < s.K ? (steps - kb) : s.K;
             TileArgs ta = make_tile(e, in, K, kb, kb == 0, nullptr, nullptr);

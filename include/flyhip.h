@@ -102,7 +102,9 @@ int fz_bind(fz_engine* e, const fz_state* st);
  * post neuron is >= t*tile (columns sorted within rows); ptg_ptr int32[n_tiles+1] / ptg_idx:
  * the plastic synapses' post neurons of each tile (NULL without plastic synapses).  The ring
  * then has 2*n_delay slots and carries plastic arrivals only; fixed arrivals come from the
- * per-step spike lists in fz_state.  Fixed weights must not change after this call. */
+ * per-step spike lists in fz_state.  The engine keeps a packed copy of the fixed edges
+ * (integer value and target offset inside its tile in one int32 per edge) when every value
+ * fits beside the offset; fixed weights must not change after this call. */
 int fz_set_tiles(fz_engine* e, int tile, int n_tiles, const int* tb, const int* ptg_ptr, const int* ptg_idx);
 
 /* Advance every live fly `steps` steps (one window; flyzero/gpu.py:228-254,

@@ -286,3 +286,37 @@ def test_tile_engine_rejects_shapes_over_the_lds_budget(small_fly):
     gb.set_tune(upd_threads=1024, tile_stage=4)       # 18 x 2048 x 4 B + 3 x 4 x 1024 x 4 B > 160 KB
     with pytest.raises(ValueError, match="LDS"):
         gb.run(FRAME_MS, graph=False)
+
+
+@pytest.fixture(scope="module")
+def full_fly():
+    return flywire_like()          # 138,639 neurons, 15.1 M connections
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+def test_full_connectome_raster_1000_steps_and_one_second_of_frames(full_fly, engine):
+    # the full-size workload: identical rasters over the first 1,000 steps on the same
+    # precomputed Poisson input, then 1 s of hash-RNG frames (rates resampled every 166
+    # steps) with identical per-neuron counts
+    fly = full_fly
+    gb, ob = _pair(fly, batch=1, engine=engine)
+    hits = poisson_hits(ob.p_ext[0], 1000, 1, seed=21)
+    cg, rg = gb.run_steps(1000, hits=torch.from_numpy(hits.view(np.int32)).cuda(), raster=True)
+    co, ro = ob.run_steps(1000, hits=hits, raster=True)
+    rg = _np(rg).view(np.uint32)
+    assert co.sum() > 10_000
+    first = np.flatnonzero((rg != ro).any(axis=(1, 2)))
+    assert len(first) == 0, f"first divergent step {first[:1]}"
+    np.testing.assert_array_equal(_np(cg), co)
+    idx = np.r_[fly.input_idx, fly.drive_idx]
+    tot_g = np.zeros(fly.n, np.int64)
+    tot_o = np.zeros(fly.n, np.int64)
+    for frame in range(60):                               # 60 x 166 = 9,960 steps
+        rate = np.r_[smooth_field(fly.input_xy, frame), np.float32(60.0)]
+        gb.set_input(idx, rate)
+        ob.set_input(idx, rate)
+        tot_g += _np(gb.run(FRAME_MS))[0]
+        tot_o += ob.run(FRAME_MS)[0]
+    assert tot_o.sum() > 100_000
+    np.testing.assert_array_equal(tot_g, tot_o)
+    _assert_state_equal(gb, ob)

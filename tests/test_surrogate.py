@@ -132,14 +132,14 @@ def test_training_reduces_loss_on_a_recurrent_target():
     rng = np.random.default_rng(6)
     Wt = rng.normal(0, 0.3, (n_in, 8))
     losses = []
-    for it in range(150):
+    for it in range(400):
         X = rng.normal(0, 1, (T, B, n_in))
         lin = X @ Wt
         Y = np.tanh(lin + 0.5 * np.concatenate([np.zeros((1, B, 8)), lin[:-1]], 0))   # depends on the past frame
         m.forward(_bf(X), torch.zeros((B, 64), device="cuda"), train=True)
         losses.append(float(m.loss_backward(_bf(X), torch.from_numpy(Y.astype(np.float32)).cuda()).item()))
         m.adam(lr=2e-3)
-    assert np.mean(losses[-10:]) < 0.2 * np.mean(losses[:10]), (losses[:3], losses[-3:])
+    assert np.mean(losses[-20:]) < 0.15 * np.mean(losses[:10]), (losses[:3], losses[-3:])
 
 
 def test_grid_pool_is_the_cell_mean():
