@@ -12,7 +12,9 @@ timed; ``value`` is the best at B = 1.  Also reported:
   launch / its mean launch time (HIP events on the engine's stream), against the HBM peak;
 * ``cpu_baseline``: the CPU oracle (a C restatement of the reference's time loop, one thread)
   on a bounded sample of the same workload on this host;
-* ``batched``: the same metric for B = 256 flies (per-fly drive and visual gain).
+* ``batched``: the same metric for B = 256 flies (per-fly drive and visual gain);
+* ``plastic``: B = 64 flies with the DN readout and the fused reward-plasticity kernel every
+  frame (300 frames).
 """
 
 from __future__ import annotations
@@ -110,13 +112,70 @@ def measure_line(fly, p, B, engine, tile, frames, warmup, torch, gain=None, driv
     return res
 
 
+def plastic_line(fly, p, torch, B=64, frames=300, warmup=3):
+    """Batched LIF plus reward-modulated plasticity: B flies, ``frames`` game frames, each frame
+    one window of the simulation, the DN-rate readout and the fused reward kernel (reward
+    N(0, 1) per fly and frame, eligibility tau 200 ms, weights capped at 50 synapses) on the
+    ~7,300 synapses onto the motor DN types."""
+    from flyhip import BatchBrain, autotune
+    from flyhip.batch import BatchReward, DNReadout, LearnParams
+    from flyhip.connectome import plastic_positions
+
+    pos = plastic_positions(fly)
+    rng = np.random.default_rng(7)
+    drive = rng.uniform(20.0, 120.0, B).astype(np.float32)
+    gain = rng.uniform(0.5, 2.0, B).astype(np.float32)
+    bb = BatchBrain(fly.weights, p, batch=B, seed=3, plastic_pos=pos, engine="tile", tile=1024)
+    idx = np.r_[fly.input_idx, fly.drive_idx]
+    j = bb.slots(idx)
+    probs = torch.from_numpy(frame_rates(fly, 64, gain, drive)).cuda() * float(np.float32(p.dt / 1000.0))
+    bb.p_ext[:, j] = probs[0]
+    shape = autotune.tune(bb, use_cache=False, reps=1)
+    br = BatchReward(bb, fly, LearnParams(tau_elig_ms=200.0, eta=5e-2, w_abs_max=50.0))
+    dn = DNReadout(fly, B)
+    rewards = torch.from_numpy(rng.normal(0.0, 1.0, (warmup + frames, B))).cuda()
+    count0 = br.count.clone()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * frames)]
+
+    def frame(f, timed):
+        bb.p_ext[:, j] = probs[f % 64]
+        counts = bb.run(FRAME_MS, copy=False)
+        dn.update(counts, FRAME_MS)
+        if timed:
+            ev[2 * (f - warmup)].record()
+        br.step(counts, rewards[f], True, FRAME_MS, return_adv=False)
+        if timed:
+            ev[2 * (f - warmup) + 1].record()
+
+    for f in range(warmup):
+        frame(f, False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for f in range(warmup, warmup + frames):
+        frame(f, True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    rule_ms = sum(ev[2 * i].elapsed_time(ev[2 * i + 1]) for i in range(frames)) / frames
+    edits = int((br.count != count0).sum())
+    res = {"B": B, "frames": frames, "plastic_synapses": int(len(pos)),
+           "value": round(B * frames * bb.steps_of(FRAME_MS) * p.dt / 1000.0 / wall, 3),
+           "unit": "fly-sim-seconds/wall-second", "ms_per_frame": round(wall / frames * 1000.0, 3),
+           "reward_step_ms": round(rule_ms, 4), "synapse_counts_changed": edits, "launch_shape": shape,
+           "workload": "tile engine + DN readout + fused reward kernel every frame; reward N(0,1), "
+                       "elig tau 200 ms, |w| <= 50 synapses"}
+    del bb, br, probs
+    torch.cuda.empty_cache()
+    return res
+
+
 def profile_run(fly, p, spec, torch, frames=3):
     """Counter runs (rocprofv3 --pmc): one engine configuration for a few eager frames at a fixed
     launch shape, then a plain 1 GiB device-to-device copy whose bytes are known, which scales
-    the counters.  spec = engine:B:upd_threads:block_steps:tile_stage, e.g. tile1024:1:256:18:1."""
+    the counters.  spec = engine:B:upd_threads:block_steps:tile_stage[:tile_source], e.g.
+    tile1024:1:256:18:1."""
     from flyhip import BatchBrain
 
-    eng, B, ut, K, stage = spec.split(":")
+    eng, B, ut, K, stage, *src = spec.split(":")
     B = int(B)
     engine, tile = ("push", 0) if eng == "push" else ("tile", int(eng[4:]))
     rng = np.random.default_rng(5)
@@ -126,7 +185,7 @@ def profile_run(fly, p, spec, torch, frames=3):
     idx = np.r_[fly.input_idx, fly.drive_idx]
     j = bb.slots(idx)
     probs = torch.from_numpy(frame_rates(fly, frames + 2, gain, drive)).cuda() * float(np.float32(p.dt / 1000.0))
-    bb.set_tune(upd_threads=int(ut), block_steps=int(K), tile_stage=int(stage))
+    bb.set_tune(upd_threads=int(ut), block_steps=int(K), tile_stage=int(stage), tile_source=int(src[0]) if src else 0)
     deg = np.diff(bb.indptr).astype(np.float64)
     steps = bb.steps_of(FRAME_MS)
     spikes = push_bytes = 0.0
@@ -163,7 +222,8 @@ def main():
     ap.add_argument("--enginesB", default="tile1024,tile2048,push")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample length")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--profile", default="", help="counter-run mode: engine:B:upd_threads:block_steps:tile_stage")
+    ap.add_argument("--plastic-frames", type=int, default=300, help="frames of the B=64 plasticity line (0: skip)")
+    ap.add_argument("--profile", default="", help="counter-run mode: engine:B:upd_threads:block_steps:tile_stage[:tile_source]")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the LIF path shards flies across GPUs with no exchange, and "
@@ -219,6 +279,13 @@ def main():
         batched = dict(bestB, unit="fly-sim-seconds/wall-second", vs_b1=round(bestB["value"] / value, 2),
                        candidates=[{k: r[k] for k in ("engine", "value", "ms_per_frame")} for r in linesB])
 
+    # ---- B = 64 with reward plasticity every frame ------------------------------------------
+    plastic = None
+    if args.plastic_frames > 0:
+        plastic = plastic_line(fly, p, torch, B=64, frames=args.plastic_frames)
+        log(f"B=64 plastic: {plastic['value']:.2f} fly-sim-s/s ({plastic['ms_per_frame']:.2f} ms/frame), "
+            f"reward step {plastic['reward_step_ms']} ms, {plastic['synapse_counts_changed']} count edits")
+
     # ---- CPU baseline (oracle, one thread) --------------------------------------------------
     cpu = None
     if not args.no_cpu:
@@ -253,7 +320,7 @@ def main():
                                    "rates resampled per frame", "neurons": fly.n,
                        "connections": int(fly.weights.nnz), "batch": 1, "steps_per_frame": steps_per_frame,
                        "engine": best["engine"], "parallelism": "single GPU"},
-            "roofline": best["roofline"], "cpu_baseline": cpu, "batched": batched,
+            "roofline": best["roofline"], "cpu_baseline": cpu, "batched": batched, "plastic": plastic,
             "b1_candidates": [{k: r[k] for k in ("engine", "value", "ms_per_frame")} for r in lines1]}
     print(json.dumps(line), flush=True)
 

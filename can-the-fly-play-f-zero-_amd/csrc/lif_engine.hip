@@ -459,6 +459,10 @@ struct TileArgs {
     const int* ptg_idx;    // ... (null without plastic synapses)
     int* spk_idx;          // [B][M][N]
     int* spk_cnt;          // [B][M]
+    const int2* bk;        // tile buckets (source 1): [B][bk_stride] entries
+    const long long* bk_off;  // [n_tiles + 1] offset of each tile's bucket in a fly's arena
+    long long bk_stride;
+    const int* bk_cnt;     // [B][n_tiles] entries in each bucket
     Consts c;
 };
 
@@ -504,7 +508,9 @@ __device__ __forceinline__ int block_incl_scan(int v, int* wsum, int* total) {
 }
 
 // IPT: spike-list entries staged per thread (more loads in flight, more LDS).
-template <typename WT, int NPT, int IPT>
+// BK: segments come from the tile's bucket (filled by tile_bucket) instead of being looked up
+// in the segment table for every spike of the fly.
+template <typename WT, int NPT, int IPT, bool BK>
 __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
     extern __shared__ int lds[];
     __shared__ int s_off[KMAX + 1];
@@ -537,50 +543,75 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
         return;
     }
     for (int q = tid; q < a.K * TS; q += TPB) acc[q] = 0;
-    if (tid < a.K) {
-        const long long src = t0 + tid - nd;
-        const int c = src >= 0 ? cnt_b[(lt0 + tid - nd + M) % M] : 0;
-        s_lcnt[tid] = c < a.N ? c : a.N;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        int o = 0;
-        for (int k = 0; k < a.K; k++) {
-            s_off[k] = o;
-            o += s_lcnt[k];
+    const int2* bkp = nullptr;
+    int total;
+    if constexpr (BK) {
+        const long long cap = a.bk_off[T + 1] - a.bk_off[T];
+        const int c = a.bk_cnt[(size_t)b * a.n_tiles + T];
+        total = c < cap ? c : (int)cap;
+        bkp = a.bk + (size_t)b * a.bk_stride + a.bk_off[T];
+        __syncthreads();
+    } else {
+        if (tid < a.K) {
+            const long long src = t0 + tid - nd;
+            const int c = src >= 0 ? cnt_b[(lt0 + tid - nd + M) % M] : 0;
+            s_lcnt[tid] = c < a.N ? c : a.N;
         }
-        s_off[a.K] = o;
+        __syncthreads();
+        if (tid == 0) {
+            int o = 0;
+            for (int k = 0; k < a.K; k++) {
+                s_off[k] = o;
+                o += s_lcnt[k];
+            }
+            s_off[a.K] = o;
+        }
+        __syncthreads();
+        total = s_off[a.K];
     }
-    __syncthreads();
-    const int total = s_off[a.K];
     const WT* w = static_cast<const WT*>(a.w);  // unused with packed edges
     const int lmask = (1 << a.lbits) - 1;
     // Entries in chunks of TPB * IPT; each thread owns IPT consecutive entries so that all of
     // its list and table loads are in flight together.
     for (int base = 0; base < total; base += TPB * IPT) {
         int seg[IPT], len[IPT], kk[IPT];
-        int ids[IPT];
-#pragma unroll
-        for (int r = 0; r < IPT; r++) {
-            const int e = base + tid * IPT + r;
-            kk[r] = 0;
-            ids[r] = -1;
-            if (e < total) {
-                while (s_off[kk[r] + 1] <= e) kk[r]++;
-                ids[r] = idx_b[(size_t)((lt0 + kk[r] - nd + M) % M) * a.N + (e - s_off[kk[r]])];
-            }
-        }
         int sum = 0;
+        if constexpr (BK) {
 #pragma unroll
-        for (int r = 0; r < IPT; r++) {
-            seg[r] = 0;
-            len[r] = 0;
-            if ((unsigned)ids[r] < (unsigned)a.N) {
-                const int* tr = a.tb + (size_t)ids[r] * (a.n_tiles + 1) + T;
-                seg[r] = tr[0];
-                len[r] = tr[1] - seg[r];
+            for (int r = 0; r < IPT; r++) {
+                const int e = base + tid * IPT + r;
+                seg[r] = len[r] = kk[r] = 0;
+                if (e < total) {
+                    const int2 x = bkp[e];
+                    seg[r] = x.x;
+                    len[r] = x.y >> 5;
+                    kk[r] = x.y & 31;
+                }
+                sum += len[r];
             }
-            sum += len[r];
+        } else {
+            int ids[IPT];
+#pragma unroll
+            for (int r = 0; r < IPT; r++) {
+                const int e = base + tid * IPT + r;
+                kk[r] = 0;
+                ids[r] = -1;
+                if (e < total) {
+                    while (s_off[kk[r] + 1] <= e) kk[r]++;
+                    ids[r] = idx_b[(size_t)((lt0 + kk[r] - nd + M) % M) * a.N + (e - s_off[kk[r]])];
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < IPT; r++) {
+                seg[r] = 0;
+                len[r] = 0;
+                if ((unsigned)ids[r] < (unsigned)a.N) {
+                    const int* tr = a.tb + (size_t)ids[r] * (a.n_tiles + 1) + T;
+                    seg[r] = tr[0];
+                    len[r] = tr[1] - seg[r];
+                }
+                sum += len[r];
+            }
         }
         int tot;
         int run = block_incl_scan(sum, s_wsum, &tot) - sum;
@@ -648,52 +679,93 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
         __syncthreads();
     }
 
-    // K steps of flyzero/brain.py:37-53 from LDS, NPT neurons per thread
+    // Poisson hits of the tile's input neurons for the K steps, as K-bit masks in LDS.  Input
+    // neurons are scattered, so hashing inside the step loop would run the hash on every lane
+    // of almost every wavefront; instead each wavefront compacts its input neurons (staging
+    // arrays, free after the pull) and spreads their (neuron, step) pairs over its 64 lanes.
+    // The draw is flyzero/gpu.py:66-67: lowbias32 of (neuron, step, seed + fly), or the
+    // precomputed hit bits.
+    int* hm = st_k + TPB * IPT;  // [TS]
+    __shared__ uint32_t s_sk[KMAX];
     const uint32_t hseed = mix32(a.seed + (uint32_t)b);
+    for (int q = tid; q < TS; q += TPB) hm[q] = 0;
+    if (tid < a.K) s_sk[tid] = mix32((step0 + (uint32_t)tid) * 0x9e3779b9U ^ hseed);
+    __syncthreads();
+    float u_[NPT], g_[NPT], bias_[NPT];
+    int r_[NPT];
+#pragma unroll
+    for (int p = 0; p < NPT; p++) {
+        const int li = p * TPB + tid, i = i0 + li;
+        u_[p] = g_[p] = bias_[p] = 0.f;
+        r_[p] = 0;
+        float pp = 0.f;
+        int j = -1;
+        if (i < a.N) {
+            u_[p] = a.u[bn + i];
+            g_[p] = a.g[bn + i];
+            r_[p] = a.ref[bn + i];
+            j = a.ext_idx ? a.ext_idx[i] : -1;
+            if (j >= 0) {
+                const size_t e = (size_t)b * a.n_ext + j;
+                if (a.bias_ext) bias_[p] = a.bias_ext[e];
+                if (a.p_ext) pp = a.p_ext[e];
+            }
+        }
+        const bool want = j >= 0 && (a.hits || pp > 0.f);
+        const unsigned long long wm = __ballot(want);
+        const int ne = __popcll(wm);
+        const int wbase = (tid >> 6) * 64;
+        if (want) {
+            const int rank = __popcll(wm & ((1ull << lane) - 1ull));
+            st_seg[wbase + rank] = li;
+            st_end[wbase + rank] = j;
+            st_k[wbase + rank] = __float_as_int(pp);
+        }
+        __syncthreads();
+        for (int x = lane; x < ne * a.K; x += 64) {
+            const int q = x / a.K, k = x - q * a.K;
+            const int lq = st_seg[wbase + q], jq = st_end[wbase + q];
+            bool hit;
+            if (a.hits) {
+                hit = (a.hits[((size_t)(a.kb + k) * a.B + b) * a.hw + (jq >> 5)] >> (jq & 31)) & 1u;
+            } else {
+                const uint32_t h = mix32((uint32_t)(bn + (size_t)(i0 + lq)) ^ s_sk[k]);
+                hit = (float)(h >> 8) * (1.0f / 16777216.0f) < __int_as_float(st_k[wbase + q]);
+            }
+            if (hit) atomicOr(&hm[lq], 1 << k);
+        }
+        __syncthreads();
+    }
+
+    // K steps of flyzero/brain.py:37-53 from LDS, NPT neurons per thread, without branches:
+    // every select keeps the float operations and their order exactly as written there
     uint32_t masks[NPT];
 #pragma unroll
     for (int p = 0; p < NPT; p++) {
         const int li = p * TPB + tid, i = i0 + li;
         uint32_t mask = 0;
         if (i < a.N) {
-            float u = a.u[bn + i], g = a.g[bn + i], bias = 0.f, pp = 0.f;
-            int r = a.ref[bn + i];
-            const float u_in = u, g_in = g;
-            const int r_in = r;
-            const int j = a.ext_idx ? a.ext_idx[i] : -1;
-            if (j >= 0) {
-                const size_t e = (size_t)b * a.n_ext + j;
-                if (a.bias_ext) bias = a.bias_ext[e];
-                if (a.p_ext) pp = a.p_ext[e];
-            }
-            const uint32_t tid32 = (uint32_t)(bn + (size_t)i);
+            float u = u_[p], g = g_[p];
+            int r = r_[p];
+            const float bias = bias_[p];
+            const uint32_t hits = (uint32_t)hm[li];
             for (int k = 0; k < a.K; k++) {
                 const float inj = (float)acc[k * TS + li] * a.c.in_scale;
                 g = g * a.c.decay_syn + inj;
-                if (r > 0) {
-                    r -= 1;
-                } else {
-                    u += (g + bias - u) * a.c.decay_m;
-                }
-                bool hit = false;
-                if (j >= 0) {
-                    if (a.hits) {
-                        hit = (a.hits[((size_t)(a.kb + k) * a.B + b) * a.hw + (j >> 5)] >> (j & 31)) & 1u;
-                    } else if (pp > 0.f) {
-                        const uint32_t h = mix32(tid32 ^ mix32((step0 + (uint32_t)k) * 0x9e3779b9U ^ hseed));
-                        hit = (float)(h >> 8) * (1.0f / 16777216.0f) < pp;
-                    }
-                }
-                if (hit) u += a.c.kick;
-                if (u >= a.c.u_th && r <= 0) {
-                    u = a.c.u_reset;
-                    r = a.c.n_ref;
-                    mask |= 1u << k;
-                }
+                const bool refr = r > 0;
+                const float u_int = u + (g + bias - u) * a.c.decay_m;
+                u = refr ? u : u_int;
+                r = refr ? r - 1 : r;
+                const float u_kick = u + a.c.kick;
+                u = ((hits >> k) & 1u) ? u_kick : u;
+                const bool sp = u >= a.c.u_th && r <= 0;
+                u = sp ? a.c.u_reset : u;
+                r = sp ? a.c.n_ref : r;
+                mask |= (uint32_t)sp << k;
             }
-            if (u != u_in) a.u[bn + i] = u;
-            if (g != g_in) a.g[bn + i] = g;
-            if (r != r_in) a.ref[bn + i] = (signed char)r;
+            if (u != u_[p]) a.u[bn + i] = u;
+            if (g != g_[p]) a.g[bn + i] = g;
+            if (r != r_[p]) a.ref[bn + i] = (signed char)r;
             const int ns = __popc(mask);
             if (a.first) a.counts[bn + i] = ns;
             else if (ns) a.counts[bn + i] += ns;
@@ -739,6 +811,107 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
                 }
             }
         }
+    }
+}
+
+// Bucket prologue of a tile-engine block (tile source 1): the spikes whose arrivals fall in
+// the block [t0, t0 + K) (emitted at steps [t0 - n_delay, t0 - n_delay + K)) are expanded once
+// into per-tile buckets of (first edge, length << 5 | step in block), so that each (fly, tile)
+// workgroup reads only the segments that reach it, contiguously, instead of looking every
+// spike of the fly up in the segment table.  Grid (ceil(n_tiles / 64), B): lane t of every
+// wavefront owns tile 64 * blockIdx.x + t; the wavefronts split the spikes, staged through
+// LDS in chunks; positions come from LDS counters, so a bucket's order depends on timing while
+// its contents (and the integer sums made from them) do not.
+constexpr int BK_CHUNK = 2048;
+constexpr int BK_UNROLL = 16;  // table loads in flight per lane
+
+struct BucketArgs {
+    const long long* clock;
+    int N, B, K, kb, nd, M, n_tiles;
+    const int* spk_idx;
+    const int* spk_cnt;
+    const int* tb;
+    const long long* bk_off;
+    long long bk_stride;
+    int2* bk;
+    int* bk_cnt;
+    const unsigned char* alive;
+    int* err;
+};
+
+__global__ void __launch_bounds__(MAX_THREADS) tile_bucket(BucketArgs a) {
+    __shared__ int s_off[KMAX + 1];
+    __shared__ int s_cnt[64];
+    __shared__ int s_id[BK_CHUNK];
+    __shared__ unsigned char s_k[BK_CHUNK];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, W = blockDim.x >> 6;
+    const int b = blockIdx.y, T = blockIdx.x * 64 + lane;
+    if (a.alive && !a.alive[b]) return;
+    const long long t0 = a.clock[0] + a.kb;
+    const int M = a.M, nd = a.nd;
+    const int lt0 = (int)(((t0 % M) + M) % M);
+    const int* cnt_b = a.spk_cnt + (size_t)b * M;
+    const int* idx_b = a.spk_idx + (size_t)b * M * a.N;
+    if (tid < 64) s_cnt[tid] = 0;
+    if (tid == 0) {
+        int o = 0;
+        for (int k = 0; k < a.K; k++) {
+            s_off[k] = o;
+            const int c = t0 + k - nd >= 0 ? cnt_b[(lt0 + k - nd + M) % M] : 0;
+            o += c < a.N ? c : a.N;
+        }
+        s_off[a.K] = o;
+    }
+    __syncthreads();
+    const int total = s_off[a.K];
+    const bool own = T < a.n_tiles;
+    long long base = 0, cap = 0;
+    if (own) {
+        base = (long long)b * a.bk_stride + a.bk_off[T];
+        cap = a.bk_off[T + 1] - a.bk_off[T];
+    }
+    const int nt1 = a.n_tiles + 1;
+    for (int c0 = 0; c0 < total; c0 += BK_CHUNK) {
+        const int nc = total - c0 < BK_CHUNK ? total - c0 : BK_CHUNK;
+        for (int q = tid; q < nc; q += blockDim.x) {  // stage the chunk's ids and steps
+            const int e = c0 + q;
+            int k = 0;
+            while (s_off[k + 1] <= e) k++;
+            s_id[q] = idx_b[(size_t)((lt0 + k - nd + M) % M) * a.N + (e - s_off[k])];
+            s_k[q] = (unsigned char)k;
+        }
+        __syncthreads();
+        for (int q0 = wave * BK_UNROLL; q0 < nc; q0 += W * BK_UNROLL) {
+            int lo[BK_UNROLL], hi[BK_UNROLL];
+#pragma unroll
+            for (int u = 0; u < BK_UNROLL; u++) {
+                lo[u] = hi[u] = 0;
+                const int q = q0 + u;
+                if (own && q < nc) {
+                    const int id = s_id[q];
+                    if ((unsigned)id < (unsigned)a.N) {
+                        const int* tr = a.tb + (size_t)id * nt1 + T;
+                        lo[u] = tr[0];
+                        hi[u] = tr[1];
+                    }
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < BK_UNROLL; u++)
+                if (hi[u] > lo[u]) {
+                    const int pos = atomicAdd(&s_cnt[lane], 1);
+                    if (pos < cap) a.bk[base + pos] = make_int2(lo[u], ((hi[u] - lo[u]) << 5) | (int)s_k[q0 + u]);
+                }
+        }
+        __syncthreads();
+    }
+    if (tid < 64 && own) {
+        int c = s_cnt[lane];
+        if (c > cap) {
+            atomicOr(a.err, 2);
+            c = (int)cap;
+        }
+        a.bk_cnt[(size_t)b * a.n_tiles + T] = c;
     }
 }
 
@@ -792,6 +965,13 @@ struct fz_engine {
     int ts = 0, n_tiles = 0;
     int* pk = nullptr;  // packed fixed edges (null: the kernel reads col and w)
     int lbits = 0;
+    // tile buckets (tile source 1): per-tile offsets in a fly's arena (entries), arena, counts
+    std::vector<long long> bk_off_h;
+    long long* bk_off = nullptr;
+    long long bk_stride = 0;
+    int2* bk = nullptr;
+    int* bk_cnt = nullptr;
+    int bk_B = 0;
     const int* tb = nullptr;
     const int* ptg_ptr = nullptr;
     const int* ptg_idx = nullptr;
@@ -805,7 +985,7 @@ struct fz_graph {
 namespace {
 
 struct Shape {
-    int ut, pt, chunk, grid, K, ipt;
+    int ut, pt, chunk, grid, K, ipt, src;
 };
 
 int resolve(const fz_engine* e, const fz_tune* t, Shape* s) {
@@ -816,13 +996,16 @@ int resolve(const fz_engine* e, const fz_tune* t, Shape* s) {
     s->K = (t && t->block_steps) ? t->block_steps : e->kmax;
     s->ipt = (t && t->tile_stage) ? t->tile_stage : 1;
     if (s->ipt != 1 && s->ipt != 4) return fail(FZ_E_ARG, "tile_stage %d: must be 1 or 4", s->ipt);
+    s->src = t ? t->tile_source : 0;
+    if (s->src != 0 && s->src != 1) return fail(FZ_E_ARG, "tile_source %d: must be 0 or 1", s->src);
+    if (s->src == 1 && (e->mode != 2 || !e->bk_off)) return fail(FZ_E_ARG, "tile_source 1 needs the tile engine");
     if (e->mode == 2) {
         if (!(t && t->upd_threads)) s->ut = e->ts < 256 ? e->ts : 256;
         const int npt = s->ut > 0 ? e->ts / s->ut : 0;
         if (s->ut <= 0 || e->ts % s->ut || (npt != 1 && npt != 2 && npt != 4 && npt != 8))
             return fail(FZ_E_ARG, "tile engine: upd_threads %d must divide the tile (%d neurons) into 1, 2, 4 or 8",
                         s->ut, e->ts);
-        const size_t lds = (size_t)s->K * e->ts * 4 + (size_t)3 * s->ipt * s->ut * 4;
+        const size_t lds = (size_t)s->K * e->ts * 4 + (size_t)3 * s->ipt * s->ut * 4 + (size_t)e->ts * 4;
         if (lds > e->lds_max - TILE_STATIC_LDS)
             return fail(FZ_E_ARG, "tile engine: %zu B of LDS for %d steps x %d neurons exceeds %zu", lds, s->K,
                         e->ts, e->lds_max);
@@ -945,6 +1128,10 @@ TileArgs make_tile(const fz_engine* e, const fz_inputs* in, int K, int kb, int f
     a.pk = e->pk;
     a.lbits = e->lbits;
     a.tb = e->tb;
+    a.bk = e->bk;
+    a.bk_off = e->bk_off;
+    a.bk_stride = e->bk_stride;
+    a.bk_cnt = e->bk_cnt;
     a.pl_indptr = e->n_pl > 0 ? e->pl_indptr : nullptr;
     a.pl_post = e->n_pl > 0 ? e->pl_post : nullptr;
     a.pw = e->n_pl > 0 ? in->pw : nullptr;
@@ -958,28 +1145,177 @@ TileArgs make_tile(const fz_engine* e, const fz_inputs* in, int K, int kb, int f
     return a;
 }
 
-template <typename WT, int IPT>
+template <typename WT, int IPT, bool BK>
 void* tile_kernel_npt(int npt) {
     switch (npt) {
-        case 1: return reinterpret_cast<void*>(&lif_tile<WT, 1, IPT>);
-        case 2: return reinterpret_cast<void*>(&lif_tile<WT, 2, IPT>);
-        case 4: return reinterpret_cast<void*>(&lif_tile<WT, 4, IPT>);
-        default: return reinterpret_cast<void*>(&lif_tile<WT, 8, IPT>);
+        case 1: return reinterpret_cast<void*>(&lif_tile<WT, 1, IPT, BK>);
+        case 2: return reinterpret_cast<void*>(&lif_tile<WT, 2, IPT, BK>);
+        case 4: return reinterpret_cast<void*>(&lif_tile<WT, 4, IPT, BK>);
+        default: return reinterpret_cast<void*>(&lif_tile<WT, 8, IPT, BK>);
     }
 }
 
+template <typename WT>
+void* tile_kernel_fmt(int npt, int ipt, int src) {
+    if (src == 1) return ipt == 1 ? tile_kernel_npt<WT, 1, true>(npt) : tile_kernel_npt<WT, 4, true>(npt);
+    return ipt == 1 ? tile_kernel_npt<WT, 1, false>(npt) : tile_kernel_npt<WT, 4, false>(npt);
+}
+
 // The lif_tile instantiation for an edge format (0: int16 col/w, 1: int32 col/w, 2: packed),
-// neurons per thread and staging depth.
-void* tile_kernel(int fmt, int npt, int ipt) {
-    if (fmt == 2) return ipt == 1 ? tile_kernel_npt<PackedEdge, 1>(npt) : tile_kernel_npt<PackedEdge, 4>(npt);
-    if (fmt == 0) return ipt == 1 ? tile_kernel_npt<int16_t, 1>(npt) : tile_kernel_npt<int16_t, 4>(npt);
-    return ipt == 1 ? tile_kernel_npt<int32_t, 1>(npt) : tile_kernel_npt<int32_t, 4>(npt);
+// neurons per thread, staging depth and segment source.
+void* tile_kernel(int fmt, int npt, int ipt, int src) {
+    if (fmt == 2) return tile_kernel_fmt<PackedEdge>(npt, ipt, src);
+    if (fmt == 0) return tile_kernel_fmt<int16_t>(npt, ipt, src);
+    return tile_kernel_fmt<int32_t>(npt, ipt, src);
 }
 int tile_fmt(const fz_engine* e) { return e->pk ? 2 : (e->w_bits == 16 ? 0 : 1); }
 
 size_t tile_lds(const fz_engine* e, const Shape& s, int K) {
-    return (size_t)K * e->ts * 4 + (size_t)3 * s.ipt * s.ut * 4;
+    return (size_t)K * e->ts * 4 + (size_t)3 * s.ipt * s.ut * 4 + (size_t)e->ts * 4;  // acc, staging, hit masks
 }
+
+__global__ void tile_seg_count(int n, int n_tiles, const int* tb, int* cnt) {
+    __shared__ int part[MAX_WAVES];
+    const int T = blockIdx.x;
+    int c = 0;
+    for (int i = blockIdx.y * blockDim.x + threadIdx.x; i < n; i += gridDim.y * blockDim.x) {
+        const int* tr = tb + (size_t)i * (n_tiles + 1) + T;
+        c += tr[1] > tr[0];
+    }
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += part[w];
+        if (t) atomicAdd(&cnt[T], t);
+    }
+}
+
+// Checks a caller's segment table: row i's entries run from indptr[i] to indptr[i+1] without
+// decreasing, and every edge of segment t targets a neuron of tile t.  Both tile sources index
+// edges through this table, so a bad table is rejected here instead of being read out of range.
+__global__ void tile_validate(int n, int n_tiles, int tile, const int* indptr, const int* col, const int* tb,
+                              int* bad) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int* tr = tb + (size_t)i * (n_tiles + 1);
+        if (tr[0] != indptr[i] || tr[n_tiles] != indptr[i + 1]) {
+            atomicOr(bad, 1);
+            continue;
+        }
+        const int r0 = indptr[i], r1 = indptr[i + 1];
+        for (int t = 0; t < n_tiles; t++) {
+            const int lo = tr[t], hi = tr[t + 1];
+            if (hi < lo || lo < r0 || hi > r1) {  // checked before any col[] read
+                atomicOr(bad, 1);
+                break;
+            }
+            for (int e = lo; e < hi; e++)
+                if (col[e] < t * tile || col[e] >= (t + 1) * tile) {
+                    atomicOr(bad, 1);
+                    break;
+                }
+        }
+    }
+}
+
+int validate_tiles(fz_engine* e, const int* tb, int n_tiles, int tile) {
+    int* bad = nullptr;
+    HIPCHK(hipMalloc(&bad, sizeof(int)));
+    HIPCHK(hipMemset(bad, 0, sizeof(int)));
+    const int gv = e->n / 256 + 1 < 4096 ? e->n / 256 + 1 : 4096;
+    hipLaunchKernelGGL(tile_validate, dim3(gv), dim3(256), 0, 0, e->n, n_tiles, tile, e->indptr, e->col, tb, bad);
+    int h_bad = 0;
+    hipError_t rv = hipGetLastError();
+    if (rv == hipSuccess) rv = hipMemcpy(&h_bad, bad, sizeof(int), hipMemcpyDeviceToHost);
+    (void)hipFree(bad);
+    if (rv != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(FZ_E_HIP, "checking the segment table failed: %s", hipGetErrorString(rv));
+    }
+    if (h_bad) return fail(FZ_E_ARG, "segment table does not match the fixed CSR and tile size %d", tile);
+    return FZ_OK;
+}
+
+// Bucket capacities: tile t can receive, per block, one entry per (row with an edge into t) x
+// (spikes a neuron can emit in one block: ceil(K / n_ref), K <= kmax).
+int bucket_layout(fz_engine* e, const int* tb, int n_tiles) {
+    (void)hipFree(e->bk_off);
+    (void)hipFree(e->bk);
+    (void)hipFree(e->bk_cnt);
+    e->bk_off = nullptr;
+    e->bk = nullptr;
+    e->bk_cnt = nullptr;
+    e->bk_B = 0;
+    e->bk_stride = 0;
+    e->bk_off_h.assign((size_t)n_tiles + 1, 0);
+    int* cnt = nullptr;
+    HIPCHK(hipMalloc(&cnt, sizeof(int) * (size_t)n_tiles));
+    HIPCHK(hipMemset(cnt, 0, sizeof(int) * (size_t)n_tiles));
+    const int gy = e->n / 256 + 1 < 64 ? e->n / 256 + 1 : 64;
+    hipLaunchKernelGGL(tile_seg_count, dim3(n_tiles, gy), dim3(256), 0, 0, e->n, n_tiles, tb, cnt);
+    std::vector<int> h((size_t)n_tiles);
+    hipError_t r = hipGetLastError();
+    if (r == hipSuccess) r = hipMemcpy(h.data(), cnt, sizeof(int) * (size_t)n_tiles, hipMemcpyDeviceToHost);
+    (void)hipFree(cnt);
+    if (r != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(FZ_E_HIP, "counting tile segments failed: %s", hipGetErrorString(r));
+    }
+    const int per = e->c.n_ref > 1 ? e->c.n_ref : 1;
+    const long long maxspk = (e->kmax + per - 1) / per;
+    for (int t = 0; t < n_tiles; t++) e->bk_off_h[t + 1] = e->bk_off_h[t] + (long long)h[t] * maxspk;
+    e->bk_stride = e->bk_off_h[n_tiles];
+    HIPCHK(hipMalloc(&e->bk_off, sizeof(long long) * ((size_t)n_tiles + 1)));
+    HIPCHK(hipMemcpy(e->bk_off, e->bk_off_h.data(), sizeof(long long) * ((size_t)n_tiles + 1),
+                     hipMemcpyHostToDevice));
+    return FZ_OK;
+}
+
+// The bucket arena for the bound batch, allocated on first use of tile source 1.
+int ensure_buckets(fz_engine* e) {
+    if (e->bk && e->bk_B == e->st.B) return FZ_OK;
+    (void)hipFree(e->bk);
+    (void)hipFree(e->bk_cnt);
+    e->bk = nullptr;
+    e->bk_cnt = nullptr;
+    e->bk_B = 0;
+    const size_t ne = (size_t)e->st.B * (size_t)(e->bk_stride > 0 ? e->bk_stride : 1);
+    if (hipMalloc(&e->bk, sizeof(int2) * ne) != hipSuccess ||
+        hipMalloc(&e->bk_cnt, sizeof(int) * (size_t)e->st.B * (size_t)e->n_tiles) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipFree(e->bk);
+        e->bk = nullptr;
+        return fail(FZ_E_HIP, "allocating %zu tile-bucket entries failed", ne);
+    }
+    HIPCHK(hipMemset(e->bk_cnt, 0, sizeof(int) * (size_t)e->st.B * (size_t)e->n_tiles));
+    e->bk_B = e->st.B;
+    return FZ_OK;
+}
+
+BucketArgs make_bucket(const fz_engine* e, const fz_inputs* in, int K, int kb) {
+    BucketArgs a{};
+    a.clock = e->st.clock;
+    a.N = e->n;
+    a.B = e->st.B;
+    a.K = K;
+    a.kb = kb;
+    a.nd = e->c.n_delay;
+    a.M = 3 * e->c.n_delay;
+    a.n_tiles = e->n_tiles;
+    a.spk_idx = e->st.spk_idx;
+    a.spk_cnt = e->st.spk_cnt;
+    a.tb = e->tb;
+    a.bk_off = e->bk_off;
+    a.bk_stride = e->bk_stride;
+    a.bk = e->bk;
+    a.bk_cnt = e->bk_cnt;
+    a.alive = in->alive;
+    a.err = e->err;
+    return a;
+}
+
+constexpr int BK_THREADS = 1024;
 
 // One window as a sequence of kernel launches (the graph builder mirrors it).
 // With `ev_*` non-null, events bracket every kernel.
@@ -999,9 +1335,18 @@ int enqueue(fz_engine* e, const fz_inputs* in, int steps, const unsigned* hits, 
     int rc;
     if (e->mode == 2) {
         const dim3 tgrid(e->n_tiles, e->st.B);
-        void* fn = tile_kernel(tile_fmt(e), e->ts / s.ut, s.ipt);
+        const dim3 bgrid((e->n_tiles + 63) / 64, e->st.B);
+        void* fn = tile_kernel(tile_fmt(e), e->ts / s.ut, s.ipt, s.src);
+        if (s.src == 1 && (rc = ensure_buckets(e))) return rc;
         while (kb < steps) {
             const int K = (steps - kb) < s.K ? (steps - kb) : s.K;
+            if (s.src == 1) {  // timed with the push timers: it is the spike expansion
+                BucketArgs ba = make_bucket(e, in, K, kb);
+                if ((rc = mark(ev_push))) return rc;
+                hipLaunchKernelGGL(tile_bucket, bgrid, dim3(BK_THREADS), 0, stream, ba);
+                HIPCHK(hipGetLastError());
+                if ((rc = mark(ev_push))) return rc;
+            }
             TileArgs ta = make_tile(e, in, K, kb, kb == 0, hits, raster);
             void* args[] = {&ta};
             if ((rc = mark(ev_upd))) return rc;
@@ -1099,6 +1444,9 @@ int fz_destroy(fz_engine* e) {
     free_worklist(e);
     (void)hipFree(e->err);
     (void)hipFree(e->pk);
+    (void)hipFree(e->bk_off);
+    (void)hipFree(e->bk);
+    (void)hipFree(e->bk_cnt);
     delete e;
     return FZ_OK;
 }
@@ -1206,6 +1554,7 @@ int pack_edges(fz_engine* e, int tile) {
     return FZ_OK;
 }
 
+
 }  // namespace
 
 int fz_set_tiles(fz_engine* e, int tile, int n_tiles, const int* tb, const int* ptg_ptr, const int* ptg_idx) {
@@ -1214,24 +1563,31 @@ int fz_set_tiles(fz_engine* e, int tile, int n_tiles, const int* tb, const int* 
     if (n_tiles != (e->n + tile - 1) / tile) return fail(FZ_E_ARG, "n_tiles %d != ceil(n / tile)", n_tiles);
     if (!tb) return fail(FZ_E_ARG, "null tile table");
     if (e->n_pl > 0 && (!ptg_ptr || !ptg_idx)) return fail(FZ_E_ARG, "plastic synapses need the per-tile target lists");
-    const size_t lds = (size_t)e->kmax * tile * 4 + (size_t)3 * 64 * 4;
+    const size_t lds = (size_t)e->kmax * tile * 4 + (size_t)3 * 64 * 4 + (size_t)tile * 4;
     if (lds > e->lds_max - TILE_STATIC_LDS)
         return fail(FZ_E_ARG, "tile %d needs %zu B of LDS for %d steps; the device allows %zu", tile, lds, e->kmax,
                     e->lds_max - TILE_STATIC_LDS);
     for (int fmt : {0, 1, 2})
         for (int npt : {1, 2, 4, 8})
-            for (int ipt : {1, 4}) {
-                hipError_t r = hipFuncSetAttribute(tile_kernel(fmt, npt, ipt),
-                                                   hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                   (int)(e->lds_max - TILE_STATIC_LDS));
-                if (r != hipSuccess) {
-                    (void)hipGetLastError();  // do not leave a sticky error for the caller's next HIP call
-                    return fail(FZ_E_HIP, "hipFuncSetAttribute(max dynamic LDS %zu) failed: %s",
-                                e->lds_max - TILE_STATIC_LDS, hipGetErrorString(r));
+            for (int ipt : {1, 4})
+                for (int src : {0, 1}) {
+                    hipError_t r = hipFuncSetAttribute(tile_kernel(fmt, npt, ipt, src),
+                                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                       (int)(e->lds_max - TILE_STATIC_LDS));
+                    if (r != hipSuccess) {
+                        (void)hipGetLastError();  // do not leave a sticky error for the caller's next HIP call
+                        return fail(FZ_E_HIP, "hipFuncSetAttribute(max dynamic LDS %zu) failed: %s",
+                                    e->lds_max - TILE_STATIC_LDS, hipGetErrorString(r));
+                    }
                 }
-            }
-    int rc = pack_edges(e, tile);
+    int rc = validate_tiles(e, tb, n_tiles, tile);
     if (rc) return rc;
+    // from here on a failure leaves the engine unbound, in push mode, until set up again
+    if ((rc = pack_edges(e, tile)) || (rc = bucket_layout(e, tb, n_tiles))) {
+        e->mode = 1;
+        e->bound = false;
+        return rc;
+    }
     e->mode = 2;
     e->ts = tile;
     e->n_tiles = n_tiles;
@@ -1321,9 +1677,21 @@ int fz_graph_create(fz_engine* e, const fz_inputs* in, int steps, const fz_tune*
     int kb = 0, blk = 0;
     if (e->mode == 2) {
         const dim3 tgrid(e->n_tiles, e->st.B);
-        void* fn = tile_kernel(tile_fmt(e), e->ts / s.ut, s.ipt);
+        const dim3 bgrid((e->n_tiles + 63) / 64, e->st.B);
+        void* fn = tile_kernel(tile_fmt(e), e->ts / s.ut, s.ipt, s.src);
+        if (s.src == 1 && (rc = ensure_buckets(e))) {
+            (void)hipGraphDestroy(gr->graph);
+            delete gr;
+            return rc;
+        }
         while (kb < steps) {
             const int K = (steps - kb) < s.K ? (steps - kb) : s.K;
+            if (s.src == 1) {
+                BucketArgs ba = make_bucket(e, in, K, kb);
+                void* bargs[] = {&ba};
+                if ((err = add(reinterpret_cast<void*>(&tile_bucket), bgrid, dim3(BK_THREADS), bargs)) != hipSuccess)
+                    return bail("hipGraphAddKernelNode(bucket)", err);
+            }
             TileArgs ta = make_tile(e, in, K, kb, kb == 0, nullptr, nullptr);
             void* targs[] = {&ta};
             if ((err = add(fn, tgrid, dim3(s.ut), targs, tile_lds(e, s, K))) != hipSuccess)

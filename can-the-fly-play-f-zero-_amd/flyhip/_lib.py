@@ -41,7 +41,8 @@ class FzInputs(ctypes.Structure):
 
 class FzTune(ctypes.Structure):
     _fields_ = [("upd_threads", c_int), ("push_threads", c_int), ("push_chunk", c_int),
-                ("push_grid", c_int), ("block_steps", c_int), ("tile_stage", c_int)]
+                ("push_grid", c_int), ("block_steps", c_int), ("tile_stage", c_int),
+                ("tile_source", c_int)]
 
 
 # name -> (restype, argtypes); every symbol declared in include/flyhip.h

@@ -86,19 +86,21 @@ def tune(bb, duration_ms: float = 1000.0 / 60.0988, reps: int = 2, use_cache: bo
     _time(bb, {}, steps, 1, snap)
     if bb.engine == "tile":
         # full grid (small): threads per workgroup (1, 2, 4 or 8 neurons per thread) x block length
-        # x spike-list staging depth; shapes over the LDS budget are skipped
+        # x staging depth x segment source (table lookups or buckets); shapes over the LDS budget
+        # are skipped
         best_t, cur = float("inf"), None
         for ut in (bb.tile // k for k in (8, 4, 2, 1)):
             if not 64 <= ut <= 1024:
                 continue
             for ks in sorted({bb.n_delay, max(1, bb.n_delay // 2)}):
                 for stage in (1, 4):
-                    trial = {"upd_threads": ut, "block_steps": ks, "tile_stage": stage}
-                    t = _time(bb, trial, steps, reps, snap)
-                    if verbose:
-                        print(f"autotune {trial}: {t * 1000:.1f} us", flush=True)
-                    if t < best_t:
-                        best_t, cur = t, trial
+                    for src in (0, 1):
+                        trial = {"upd_threads": ut, "block_steps": ks, "tile_stage": stage, "tile_source": src}
+                        t = _time(bb, trial, steps, reps, snap)
+                        if verbose:
+                            print(f"autotune {trial}: {t * 1000:.1f} us", flush=True)
+                        if t < best_t:
+                            best_t, cur = t, trial
         space = []
     else:
         cur = {"upd_threads": 256, "push_threads": 256, "push_chunk": 256, "push_grid": n_cu * 8}

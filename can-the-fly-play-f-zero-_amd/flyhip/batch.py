@@ -111,14 +111,18 @@ class BatchReward:
         self.noise += -a * self.noise + np.sqrt(2 * a) * self.rng.standard_normal(self.noise.shape)
         return np.maximum(self.p.noise_hz * self.noise, 0.0)
 
-    def step(self, counts: torch.Tensor, reward, learn, window_ms: float) -> np.ndarray:
+    def step(self, counts: torch.Tensor, reward, learn, window_ms: float, return_adv: bool = True):
         """One frame: traces, means and (if any slot learns) weights.  ``counts``: the window's
-        (B, n) int32 device counts; ``reward``, ``learn``: per slot.  Returns the advantage."""
+        (B, n) int32 device counts; ``reward``, ``learn``: per slot.  Returns the advantage (a
+        host copy, which waits for the device; ``return_adv=False`` returns None instead)."""
         brain = self.brain
         if counts.dtype != torch.int32 or not counts.is_contiguous() or tuple(counts.shape) != (self.B, brain.n):
             raise ValueError("counts must be the window's contiguous int32 (B, n) device counts")
         learn = np.broadcast_to(np.asarray(learn, bool), (self.B,))
-        self._reward.copy_(torch.from_numpy(np.broadcast_to(np.asarray(reward, np.float64), (self.B,)).copy()))
+        if isinstance(reward, torch.Tensor):  # device rewards stay on the device
+            self._reward.copy_(reward.to(self._reward.device, torch.float64).expand(self.B))
+        else:
+            self._reward.copy_(torch.from_numpy(np.broadcast_to(np.asarray(reward, np.float64), (self.B,)).copy()))
         self._learn.copy_(torch.from_numpy(learn.astype(np.uint8)))
         c = rule_constants(self.p, window_ms, brain.p.w_syn)
         if not brain.pw.is_contiguous() or brain.pw.dtype != torch.float32:
@@ -131,7 +135,7 @@ class BatchReward:
                          P(self.d_fly_slots), c["ae"], c["ap"], c["ar"], c["eta"], np.float32(brain.p.w_syn),
                          P(self._adv))
         _lib.check(_lib.load().fz_reward_step(ctypes.byref(a), _lib.stream_handle(brain.device)), "fz_reward_step")
-        return self._adv.cpu().numpy()
+        return self._adv.cpu().numpy() if return_adv else None
 
     def drift(self) -> np.ndarray:
         """Mean relative change from the initial weights, per slot (flyzero/batch.py:221-223)."""

@@ -9,9 +9,11 @@
   simulation's own expected rate can be estimated by their mean.
 * ``train_ensemble``: members trained by the package's own kernels on random windows of
   ``T`` consecutive frames (BPTT through the GRU).
-* ``judge``: held-out sequences, the replicas' mean as the simulation's rate; reports the
-  surrogate's error with and without fallback, the error of a constant predictor (the method
-  without the network) and of a single simulation trial (the noise floor), and frames/s.
+* ``generate_judged`` / ``judge``: held-out sequences, each simulated by several replica flies;
+  the replicas' mean is the simulation's expected rate (the judge).  Reports the surrogate's
+  error with and without fallback, the error of a constant predictor (the method without the
+  network) and of a single simulation trial (the simulation's own trial-to-trial noise), and
+  frames/s with and without fallback.
 
 ``python -m flyhip.surrogate_pipeline`` runs all of it and prints one JSON report.
 """
@@ -139,30 +141,60 @@ def train_ensemble(X, Y, members=3, T=16, seqs=256, steps=600, lr=1e-3, rate_sca
     return out, curves
 
 
-def judge(ens, X, Y, replicas: int, threshold: float = 0.15, const=None):
-    """X, Y from ``generate`` with ``replicas`` flies per input sequence.  The simulation's rate
-    of a sequence is the replicas' mean; the surrogate runs on one replica's (shared) input."""
-    F, B, n_in = X.shape
-    S = B // replicas
-    Ys = Y.view(F, S, replicas, 8)
-    ref = Ys.mean(2)                                                   # [F, S, 8]
-    trial = Ys[:, :, 0]
-    # leave-one-out noise floor: one trial against the mean of the others
-    others = (Ys.sum(2) - trial) / max(replicas - 1, 1)
+def generate_judged(fly, frames: int, sequences: int, replicas: int, batch: int = 256, seed: int = 77, log=None,
+                    **kw):
+    """Held-out data: ``sequences`` input sequences, each simulated by ``replicas`` flies with
+    independent spiking noise (``batch // replicas`` sequences per batched run).  Returns the
+    shared inputs X [F, S, 2048], the replicas' mean rate (the simulation's expected rate, the
+    judge), one trial, the mean of the other trials, and sim frames/s."""
+    per = batch // replicas
+    if per < 1 or batch % replicas:
+        raise ValueError("batch must be a multiple of replicas")
+    xs, mean, trial, others, fps = [], [], [], [], []
+    done = 0
+    k = 0
+    while done < sequences:
+        X, Y, f = generate(fly, frames, batch, seed=seed + 1000 * k, replicas=replicas, **kw)
+        Ys = Y.view(frames, per, replicas, 8)
+        xs.append(X.view(frames, per, replicas, -1)[:, :, 0].clone())
+        mean.append(Ys.mean(2))
+        trial.append(Ys[:, :, 0].clone())
+        others.append((Ys.sum(2) - Ys[:, :, 0]) / max(replicas - 1, 1))
+        fps.append(f)
+        del X, Y, Ys
+        done += per
+        k += 1
+        if log:
+            log(f"  held-out: {done}/{sequences} sequences")
+    cat = lambda a: torch.cat(a, 1)[:, :sequences].contiguous()  # noqa: E731
+    return cat(xs), cat(mean), cat(trial), cat(others), float(np.mean(fps))
+
+
+def judge(ens, X, ref, trial, others, replicas: int, threshold: float = 0.15, const=None):
+    """X [F, S, n_in] inputs, ref [F, S, 8] the simulation's expected rate (replica mean),
+    trial one simulation trial, others the mean of the remaining replicas.  Errors are
+    mean |a - ref| / mean |ref| over frames, sequences and the 8 groups."""
+    F, S, n_in = X.shape
     ens.reset(S)
     pred = torch.empty_like(ref)
     spread = torch.empty((F, S), dtype=torch.float32, device=X.device)
-    xs = X.view(F, S, replicas, n_in)[:, :, 0]
     for f in range(F):
-        pred[f], spread[f] = ens.predict(xs[f].contiguous())
+        pred[f], spread[f] = ens.predict(X[f].contiguous())
     fb = spread > threshold
     mixed = torch.where(fb[..., None], trial, pred)
     denom = ref.abs().mean()
     err = lambda a, b: float((a - b).abs().mean() / denom)  # noqa: E731
-    res = {"frames": F * S, "sequences": S, "replicas": replicas, "mean_rate_hz": float(ref.mean()),
-           "err_surrogate": err(pred, ref), "err_with_fallback": err(mixed, ref),
+    noise = err(trial, others)
+    res = {"frames": F * S, "sequences": S, "frames_per_sequence": F, "replicas": replicas,
+           "mean_rate_hz": float(ref.mean()), "err_surrogate": err(pred, ref), "err_with_fallback": err(mixed, ref),
            "fallback_frac": float(fb.float().mean()), "err_single_sim_trial": err(trial, ref),
-           "err_sim_trial_vs_other_trials": err(trial, others), "threshold": threshold}
+           "err_sim_trial_vs_other_trials": noise,
+           # a trial's deviation from the mean of R-1 others is ~sqrt(R/(R-1)) x its deviation from
+           # the expected rate; the judge (mean of R) deviates ~1/sqrt(R) of that
+           "judge_noise_est": noise * math.sqrt((replicas - 1) / replicas) / math.sqrt(replicas),
+           "err_per_group": [float((pred[..., g] - ref[..., g]).abs().mean() / ref[..., g].abs().mean().clamp_min(1e-6))
+                             for g in range(ref.shape[-1])],
+           "threshold": threshold}
     if const is not None:
         res["err_constant_predictor"] = err(const.expand_as(ref), ref)
     return res
@@ -184,12 +216,13 @@ def inference_fps(ens, batch: int = 4096, frames: int = 20):
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--train-frames", type=int, default=4000, help="frames per fly (256 flies)")
-    ap.add_argument("--test-frames", type=int, default=200, help="frames per held-out sequence")
+    ap.add_argument("--train-frames", type=int, default=7800, help="frames per fly (x --batch flies)")
     ap.add_argument("--batch", type=int, default=256)
-    ap.add_argument("--replicas", type=int, default=8)
+    ap.add_argument("--test-frames", type=int, default=200, help="frames per held-out sequence")
+    ap.add_argument("--test-sequences", type=int, default=250)
+    ap.add_argument("--replicas", type=int, default=16)
     ap.add_argument("--members", type=int, default=3)
-    ap.add_argument("--steps", type=int, default=1500)
+    ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--threshold", type=float, default=0.15)
     ap.add_argument("--n", type=int, default=0, help="neurons (0: the full 138,639)")
@@ -212,15 +245,19 @@ def main(argv=None):
     const = Y.mean((0, 1))
     del X, Y
     torch.cuda.empty_cache()
-    Xt, Yt, _ = generate(fly, args.test_frames, args.batch, seed=77, replicas=args.replicas, log=log)
+    Xt, ref, trial, others, _ = generate_judged(fly, args.test_frames, args.test_sequences, args.replicas,
+                                                args.batch, log=log)
     ens = Ensemble(members)
-    res = judge(ens, Xt, Yt, args.replicas, args.threshold, const)
+    res = judge(ens, Xt, ref, trial, others, args.replicas, args.threshold, const)
     fps = inference_fps(ens)
-    res.update({"train_frames": args.train_frames * args.batch, "train_steps": args.steps, "train_s": round(train_s, 1),
-                "members": args.members, "loss_curves": curves, "surrogate_frames_per_s": round(fps),
+    f = res["fallback_frac"]
+    # every frame runs the ensemble; a fallback frame also costs one simulated frame
+    eff = 1.0 / (1.0 / fps + f / sim_fps)
+    res.update({"train_frames": args.train_frames * args.batch, "train_steps": args.steps,
+                "train_s": round(train_s, 1), "members": args.members,
+                "loss_curves": [c[::6] + [c[-1]] for c in curves], "surrogate_frames_per_s": round(fps),
                 "sim_frames_per_s": round(sim_fps), "speedup_vs_sim": round(fps / sim_fps, 1),
-                "speedup_with_fallback": round(1.0 / ((1 - res["fallback_frac"]) / fps + res["fallback_frac"] / sim_fps)
-                                               / sim_fps, 1)})
+                "frames_per_s_with_fallback": round(eff), "speedup_with_fallback": round(eff / sim_fps, 1)})
     print(json.dumps(res), flush=True)
     return res
 

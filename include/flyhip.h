@@ -77,6 +77,11 @@ typedef struct fz_tune {
     int push_grid;     /* push workgroups (grid-stride) */
     int block_steps;   /* steps per temporal block, 1..min(n_delay, 32) */
     int tile_stage;    /* tile engine: spike-list entries staged per thread, 1 or 4 */
+    int tile_source;   /* tile engine: 0 = every tile looks its segments up in the segment table;
+                        * 1 = a prologue kernel per block expands each spike once into per-tile
+                        * buckets of (first edge, length, step) (engine scratch, allocated on
+                        * first use: 8 B x nonempty (row, tile) pairs x spikes per block bound
+                        * per fly) */
 } fz_tune;
 
 int fz_version(void);

@@ -266,7 +266,11 @@ def test_extreme_launch_shapes_are_exact(small_fly, shape):
 @pytest.mark.parametrize("tile,shape", [(512, dict(upd_threads=64, block_steps=5)),
                                         (2048, dict(upd_threads=256)), (256, dict(upd_threads=256)),
                                         (1024, dict(upd_threads=1024, block_steps=9, tile_stage=4)),
-                                        (1024, dict(upd_threads=128, tile_stage=4))])
+                                        (1024, dict(upd_threads=128, tile_stage=4)),
+                                        (1024, dict(upd_threads=256, tile_source=1)),
+                                        (512, dict(upd_threads=64, block_steps=5, tile_source=1)),
+                                        (2048, dict(upd_threads=512, block_steps=9, tile_stage=4, tile_source=1)),
+                                        (256, dict(upd_threads=256, block_steps=7, tile_source=1))])
 def test_tile_engine_shapes_are_exact(small_fly, tile, shape):
     # shapes that the LDS budget admits, incl. 8 neurons per thread and 5-step blocks
     fly = small_fly
@@ -320,3 +324,42 @@ def test_full_connectome_raster_1000_steps_and_one_second_of_frames(full_fly, en
     assert tot_o.sum() > 100_000
     np.testing.assert_array_equal(tot_g, tot_o)
     _assert_state_equal(gb, ob)
+
+
+@pytest.mark.parametrize("source", [0, 1])
+def test_tile_sources_give_identical_rasters_and_plastic_arrivals(small_fly, source):
+    # per-tile buckets versus segment-table lookups: same rasters, counts and state as the
+    # oracle, with plastic synapses (pushed through the ring) and a retired fly
+    fly = small_fly
+    rng = np.random.default_rng(13)
+    pos = np.sort(rng.choice(fly.weights.nnz, 3000, replace=False))
+    gb, ob = _pair(fly, batch=3, seed=4, plastic_pos=pos, engine="tile", tile=1024)
+    gb.set_tune(upd_threads=512, tile_source=source)
+    gb.pw[1] *= 1.5
+    ob.pw[1] *= 1.5
+    gb.alive[2] = 0
+    ob.alive = np.array([1, 1, 0], np.uint8)
+    steps = 300
+    hits = poisson_hits(ob.p_ext[0], steps, 3, seed=8)
+    cg, rg = gb.run_steps(steps, hits=torch.from_numpy(hits.view(np.int32)).cuda(), raster=True)
+    co, ro = ob.run_steps(steps, hits=hits, raster=True)
+    rg = _np(rg).view(np.uint32)
+    assert co[:2].sum() > 1000
+    first = np.flatnonzero((rg[:, :2] != ro[:, :2]).any(axis=(1, 2)))
+    assert len(first) == 0, f"first divergent step {first[:1]}"
+    np.testing.assert_array_equal(_np(cg), co)
+    for frame in range(3):
+        np.testing.assert_array_equal(_np(gb.run(FRAME_MS)), ob.run(FRAME_MS), err_msg=f"frame {frame}")
+    np.testing.assert_array_equal(_np(gb.u)[:2], ob.u[:2])
+    np.testing.assert_array_equal(_np(gb.g)[:2], ob.g[:2])
+
+
+def test_tile_engine_rejects_a_bad_segment_table(small_fly):
+    from flyhip import _lib
+
+    gb = BatchBrain(small_fly.weights, LIFParams(), batch=1, engine="tile", tile=1024)
+    bad = gb.d_tb.clone()
+    bad[5, 3] += 1                                      # an edge moved to the wrong tile
+    with pytest.raises(ValueError, match="segment table"):
+        _lib.check(gb._lib.fz_set_tiles(gb._h, gb.tile, bad.shape[1] - 1, bad.data_ptr(), None, None),
+                   "fz_set_tiles")

@@ -162,7 +162,7 @@ def test_grid_pool_is_the_cell_mean():
 def test_surrogate_pipeline_end_to_end_small():
     from flyhip.connectome import flywire_like
     from flyhip.surrogate import Ensemble
-    from flyhip.surrogate_pipeline import generate, judge, train_ensemble
+    from flyhip.surrogate_pipeline import generate, generate_judged, judge, train_ensemble
 
     fly = flywire_like(n=4000, n_edges=4000 * 60, seed=11)
     X, Y, fps = generate(fly, frames=40, batch=16, seed=1, skip=2)
@@ -170,8 +170,9 @@ def test_surrogate_pipeline_end_to_end_small():
     assert np.isfinite(Y.cpu().numpy()).all() and float(Y.sum()) > 0
     members, curves = train_ensemble(X, Y, members=2, T=8, seqs=16, steps=60, lr=2e-3)
     assert curves[0][-1][1] < curves[0][0][1]
-    Xt, Yt, _ = generate(fly, frames=20, batch=16, seed=5, replicas=4, skip=2)
-    res = judge(Ensemble(members), Xt, Yt, replicas=4, const=Y.mean((0, 1)))
+    Xt, ref, trial, others, _ = generate_judged(fly, frames=20, sequences=6, replicas=4, batch=16, skip=2)
+    assert Xt.shape == (20, 6, 2048) and ref.shape == (20, 6, 8)
+    res = judge(Ensemble(members), Xt, ref, trial, others, replicas=4, const=Y.mean((0, 1)))
     for k in ("err_surrogate", "err_with_fallback", "fallback_frac", "err_single_sim_trial", "err_constant_predictor"):
         assert np.isfinite(res[k]), (k, res)
     assert 0.0 <= res["fallback_frac"] <= 1.0
