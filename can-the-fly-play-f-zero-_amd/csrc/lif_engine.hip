@@ -68,6 +68,7 @@ constexpr int KMAX = 32;          // steps per block held in registers
 constexpr int MAX_THREADS = 1024;
 constexpr int MAX_WAVES = MAX_THREADS / 64;
 constexpr int QMIN = 64;          // smallest push chunk (sizes the chunk-start table)
+constexpr int BK_PIECE = 4;  // edges per tile-bucket entry at most (longer segments are split)
 constexpr size_t TILE_STATIC_LDS = 1024;  // room left for lif_tile's static shared arrays
 constexpr int UNROLL = 4;         // tile engine: edge loads in flight per thread
 
@@ -571,25 +572,53 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
     }
     const WT* w = static_cast<const WT*>(a.w);  // unused with packed edges
     const int lmask = (1 << a.lbits) - 1;
-    // Entries in chunks of TPB * IPT; each thread owns IPT consecutive entries so that all of
-    // its list and table loads are in flight together.
-    for (int base = 0; base < total; base += TPB * IPT) {
-        int seg[IPT], len[IPT], kk[IPT];
-        int sum = 0;
-        if constexpr (BK) {
-#pragma unroll
-            for (int r = 0; r < IPT; r++) {
-                const int e = base + tid * IPT + r;
-                seg[r] = len[r] = kk[r] = 0;
-                if (e < total) {
-                    const int2 x = bkp[e];
-                    seg[r] = x.x;
-                    len[r] = x.y >> 5;
-                    kk[r] = x.y & 31;
-                }
-                sum += len[r];
-            }
+    auto edge = [&](int ek, int* loc, int* val) {
+        if constexpr (std::is_same<WT, PackedEdge>::value) {
+            const int x = a.pk[ek];  // one load: value and tile-local target
+            *loc = x & lmask;
+            *val = x >> a.lbits;
         } else {
+            *loc = a.col[ek] - i0;
+            *val = wval(w[ek], a.frac_bits);
+        }
+    };
+    if constexpr (BK) {
+        // Bucket pieces hold at most BK_PIECE edges (tile segments are short: ~1 edge per
+        // (row, tile) on a FlyWire-shaped graph), so each thread takes whole pieces: no scan
+        // and no search, BK_PIECE x 2 edge loads in flight per thread.
+        for (int e0 = tid; e0 < total; e0 += 2 * TPB) {
+            int2 x[2];
+#pragma unroll
+            for (int r = 0; r < 2; r++) {
+                const int e = e0 + r * TPB;
+                x[r] = e < total ? bkp[e] : make_int2(0, 0);
+            }
+            int loc[2][BK_PIECE], val[2][BK_PIECE];
+#pragma unroll
+            for (int r = 0; r < 2; r++) {
+                const int len = x[r].y >> 5;
+#pragma unroll
+                for (int j = 0; j < BK_PIECE; j++) {
+                    loc[r][j] = val[r][j] = 0;
+                    if (j < len) edge(x[r].x + j, &loc[r][j], &val[r][j]);
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < 2; r++) {
+                const int slot = (x[r].y & 31) * TS;
+#pragma unroll
+                for (int j = 0; j < BK_PIECE; j++)
+                    if (val[r][j] != 0) atomicAdd(&acc[slot + loc[r][j]], val[r][j]);
+            }
+        }
+        __syncthreads();
+    } else {
+        // Entries in chunks of TPB * IPT; each thread owns IPT consecutive entries so that all of
+        // its list and table loads are in flight together; segments (any length) are staged with
+        // their prefix sums and the edges split evenly over the threads.
+        for (int base = 0; base < total; base += TPB * IPT) {
+            int seg[IPT], len[IPT], kk[IPT];
+            int sum = 0;
             int ids[IPT];
 #pragma unroll
             for (int r = 0; r < IPT; r++) {
@@ -612,57 +641,48 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
                 }
                 sum += len[r];
             }
-        }
-        int tot;
-        int run = block_incl_scan(sum, s_wsum, &tot) - sum;
+            int tot;
+            int run = block_incl_scan(sum, s_wsum, &tot) - sum;
 #pragma unroll
-        for (int r = 0; r < IPT; r++) {
-            run += len[r];
-            st_seg[tid * IPT + r] = seg[r];
-            st_end[tid * IPT + r] = run;
-            st_k[tid * IPT + r] = kk[r];
-        }
-        __syncthreads();
-        const int ns = TPB * IPT;
-        for (int q0 = tid; q0 < tot; q0 += TPB * UNROLL) {
-            int ek[UNROLL], slot[UNROLL];
-#pragma unroll
-            for (int u = 0; u < UNROLL; u++) {
-                const int q = q0 + u * TPB;
-                ek[u] = -1;
-                slot[u] = 0;
-                if (q < tot) {
-                    int lo = 0, hi = ns - 1;  // first j with st_end[j] > q
-                    while (lo < hi) {
-                        const int mid = (lo + hi) >> 1;
-                        if (st_end[mid] > q) hi = mid;
-                        else lo = mid + 1;
-                    }
-                    ek[u] = st_seg[lo] + (q - (lo ? st_end[lo - 1] : 0));
-                    slot[u] = st_k[lo] * TS;
-                }
+            for (int r = 0; r < IPT; r++) {
+                run += len[r];
+                st_seg[tid * IPT + r] = seg[r];
+                st_end[tid * IPT + r] = run;
+                st_k[tid * IPT + r] = kk[r];
             }
-            int loc[UNROLL], val[UNROLL];
+            __syncthreads();
+            const int ns = TPB * IPT;
+            for (int q0 = tid; q0 < tot; q0 += TPB * UNROLL) {
+                int ek[UNROLL], slot[UNROLL];
 #pragma unroll
-            for (int u = 0; u < UNROLL; u++) {
-                loc[u] = 0;
-                val[u] = 0;
-                if (ek[u] >= 0) {
-                    if constexpr (std::is_same<WT, PackedEdge>::value) {
-                        const int x = a.pk[ek[u]];  // one load: value and tile-local target
-                        loc[u] = x & lmask;
-                        val[u] = x >> a.lbits;
-                    } else {
-                        loc[u] = a.col[ek[u]] - i0;
-                        val[u] = wval(w[ek[u]], a.frac_bits);
+                for (int u = 0; u < UNROLL; u++) {
+                    const int q = q0 + u * TPB;
+                    ek[u] = -1;
+                    slot[u] = 0;
+                    if (q < tot) {
+                        int lo = 0, hi = ns - 1;  // first j with st_end[j] > q
+                        while (lo < hi) {
+                            const int mid = (lo + hi) >> 1;
+                            if (st_end[mid] > q) hi = mid;
+                            else lo = mid + 1;
+                        }
+                        ek[u] = st_seg[lo] + (q - (lo ? st_end[lo - 1] : 0));
+                        slot[u] = st_k[lo] * TS;
                     }
                 }
-            }
+                int loc[UNROLL], val[UNROLL];
 #pragma unroll
-            for (int u = 0; u < UNROLL; u++)
-                if (val[u] != 0) atomicAdd(&acc[slot[u] + loc[u]], val[u]);
+                for (int u = 0; u < UNROLL; u++) {
+                    loc[u] = 0;
+                    val[u] = 0;
+                    if (ek[u] >= 0) edge(ek[u], &loc[u], &val[u]);
+                }
+#pragma unroll
+                for (int u = 0; u < UNROLL; u++)
+                    if (val[u] != 0) atomicAdd(&acc[slot[u] + loc[u]], val[u]);
+            }
+            __syncthreads();
         }
-        __syncthreads();
     }
     if (a.ptg_ptr) {  // plastic arrivals (pushed at spike time), tile's targets only
         const int p0 = a.ptg_ptr[T], np_ = a.ptg_ptr[T + 1] - p0;
@@ -722,8 +742,10 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
             st_k[wbase + rank] = __float_as_int(pp);
         }
         __syncthreads();
+        // pairs x = q * K + k, x = lane, lane + 64, ...: one division, then increments
+        int q = lane / a.K, k = lane - q * a.K;
+        const int dq = 64 / a.K, dk = 64 - dq * a.K;
         for (int x = lane; x < ne * a.K; x += 64) {
-            const int q = x / a.K, k = x - q * a.K;
             const int lq = st_seg[wbase + q], jq = st_end[wbase + q];
             bool hit;
             if (a.hits) {
@@ -733,6 +755,12 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
                 hit = (float)(h >> 8) * (1.0f / 16777216.0f) < __int_as_float(st_k[wbase + q]);
             }
             if (hit) atomicOr(&hm[lq], 1 << k);
+            q += dq;
+            k += dk;
+            if (k >= a.K) {
+                k -= a.K;
+                q++;
+            }
         }
         __syncthreads();
     }
@@ -759,9 +787,11 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
                 const float u_kick = u + a.c.kick;
                 u = ((hits >> k) & 1u) ? u_kick : u;
                 const bool sp = u >= a.c.u_th && r <= 0;
-                u = sp ? a.c.u_reset : u;
-                r = sp ? a.c.n_ref : r;
-                mask |= (uint32_t)sp << k;
+                if (__any(sp)) {  // spikes are rare: most wavefronts skip the resets
+                    u = sp ? a.c.u_reset : u;
+                    r = sp ? a.c.n_ref : r;
+                    mask |= (uint32_t)sp << k;
+                }
             }
             if (u != u_[p]) a.u[bn + i] = u;
             if (g != g_[p]) a.g[bn + i] = g;
@@ -816,7 +846,8 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
 
 // Bucket prologue of a tile-engine block (tile source 1): the spikes whose arrivals fall in
 // the block [t0, t0 + K) (emitted at steps [t0 - n_delay, t0 - n_delay + K)) are expanded once
-// into per-tile buckets of (first edge, length << 5 | step in block), so that each (fly, tile)
+// into per-tile buckets of (first edge, length << 5 | step in block), segments longer than
+// BK_PIECE edges split into pieces, so that each (fly, tile)
 // workgroup reads only the segments that reach it, contiguously, instead of looking every
 // spike of the fly up in the segment table.  Grid (ceil(n_tiles / 64), B): lane t of every
 // wavefront owns tile 64 * blockIdx.x + t; the wavefronts split the spikes, staged through
@@ -899,8 +930,13 @@ __global__ void __launch_bounds__(MAX_THREADS) tile_bucket(BucketArgs a) {
 #pragma unroll
             for (int u = 0; u < BK_UNROLL; u++)
                 if (hi[u] > lo[u]) {
-                    const int pos = atomicAdd(&s_cnt[lane], 1);
-                    if (pos < cap) a.bk[base + pos] = make_int2(lo[u], ((hi[u] - lo[u]) << 5) | (int)s_k[q0 + u]);
+                    const int len = hi[u] - lo[u], np = (len + BK_PIECE - 1) / BK_PIECE;
+                    const int pos = atomicAdd(&s_cnt[lane], np);
+                    const int k = (int)s_k[q0 + u];
+                    for (int c = 0; c < np; c++) {
+                        const int l = len - c * BK_PIECE < BK_PIECE ? len - c * BK_PIECE : BK_PIECE;
+                        if (pos + c < cap) a.bk[base + pos + c] = make_int2(lo[u] + c * BK_PIECE, (l << 5) | k);
+                    }
                 }
         }
         __syncthreads();
@@ -1180,7 +1216,7 @@ __global__ void tile_seg_count(int n, int n_tiles, const int* tb, int* cnt) {
     int c = 0;
     for (int i = blockIdx.y * blockDim.x + threadIdx.x; i < n; i += gridDim.y * blockDim.x) {
         const int* tr = tb + (size_t)i * (n_tiles + 1) + T;
-        c += tr[1] > tr[0];
+        c += (tr[1] - tr[0] + BK_PIECE - 1) / BK_PIECE;  // bucket entries of this segment
     }
     for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o);
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
@@ -1237,8 +1273,9 @@ int validate_tiles(fz_engine* e, const int* tb, int n_tiles, int tile) {
     return FZ_OK;
 }
 
-// Bucket capacities: tile t can receive, per block, one entry per (row with an edge into t) x
-// (spikes a neuron can emit in one block: ceil(K / n_ref), K <= kmax).
+// Bucket capacities: tile t can receive, per block, ceil(segment length / BK_PIECE) entries per
+// row with an edge into t, times the spikes a neuron can emit in one block (ceil(K / n_ref),
+// K <= kmax).
 int bucket_layout(fz_engine* e, const int* tb, int n_tiles) {
     (void)hipFree(e->bk_off);
     (void)hipFree(e->bk);

@@ -220,11 +220,12 @@ def main(argv=None):
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--test-frames", type=int, default=200, help="frames per held-out sequence")
     ap.add_argument("--test-sequences", type=int, default=250)
-    ap.add_argument("--replicas", type=int, default=16)
+    ap.add_argument("--replicas", type=int, default=32)
     ap.add_argument("--members", type=int, default=3)
-    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--steps", type=int, default=6000)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--threshold", type=float, default=0.15)
+    ap.add_argument("--rate-scale", type=float, default=2.0, help="Hz per unit of the network's output")
     ap.add_argument("--n", type=int, default=0, help="neurons (0: the full 138,639)")
     args = ap.parse_args(argv)
 
@@ -240,7 +241,8 @@ def main(argv=None):
     log(f"training data: {tuple(X.shape)} in {time.perf_counter() - t0:.1f}s ({sim_fps:.0f} sim frames/s); "
         f"DN rate mean {float(Y.mean()):.2f} Hz, sd {float(Y.std()):.2f}")
     t0 = time.perf_counter()
-    members, curves = train_ensemble(X, Y, members=args.members, steps=args.steps, lr=args.lr, log=log)
+    members, curves = train_ensemble(X, Y, members=args.members, steps=args.steps, lr=args.lr, log=log,
+                                     rate_scale=args.rate_scale)
     train_s = time.perf_counter() - t0
     const = Y.mean((0, 1))
     del X, Y
