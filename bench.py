@@ -53,6 +53,30 @@ def frame_rates(fly, frames: int, gain=None, drive=None) -> np.ndarray:
     return np.minimum(out, 1e4).astype(np.float32)
 
 
+PROFILE = os.path.join(REPO, "profiles", "lif_tile_flywire138k.md")
+
+
+def measured_traffic(spec: str):
+    """HBM bytes per launch of the dominant kernel for a launch shape, from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary in profiles/ (lines 'traffic-json: {...}'), or None."""
+    try:
+        for line in open(PROFILE):
+            if line.startswith("traffic-json:"):
+                rec = json.loads(line.split(":", 1)[1])
+                if rec.get("spec") == spec:
+                    return rec
+    except OSError:
+        pass
+    return None
+
+
+def shape_spec(engine, tile, B, shape):
+    if engine != "tile":
+        return None
+    return (f"tile{tile}:{B}:{shape.get('upd_threads', 0)}:{shape.get('block_steps', 0)}:"
+            f"{shape.get('tile_stage', 0)}:{shape.get('tile_source', 0)}")
+
+
 def measure_line(fly, p, B, engine, tile, frames, warmup, torch, gain=None, drive=None, seed=0,
                  kernel_frames=0):
     """Build, autotune and time one engine configuration on ``frames`` game frames."""
@@ -101,8 +125,13 @@ def measure_line(fly, p, B, engine, tile, frames, warmup, torch, gain=None, driv
         avg_ms = ms / max(launches, 1)
         bpl = nbytes / max(launches, 1)
         achieved = bpl / (avg_ms / 1000.0) / 1e9
+        tr = measured_traffic(shape_spec(engine, tile, B, shape))
         res["roofline"] = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                           "frac": round(achieved / HBM_PEAK_GBS, 4),
+                           "traffic": tr["bytes_per_launch"] if tr else None,
+                           "traffic_source": (f"rocprofv3 FETCH_SIZE x {tr['fetch_factor']} + WRITE_SIZE x "
+                                              f"{tr['write_factor']} (copy-kernel scaled), {PROFILE[len(REPO) + 1:]}")
+                           if tr else "no counter run for this launch shape", "kernel": dom,
                            "avg_launch_us": round(avg_ms * 1000.0, 3), "bytes_per_launch": round(bpl),
                            "launches": launches,
                            "kernel_ms_per_frame": round((tim["update_ms"] + tim["push_ms"]) / kernel_frames, 4),

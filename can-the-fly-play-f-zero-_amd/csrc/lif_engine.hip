@@ -416,7 +416,7 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_push(PushArgs<WT> a) {
 // For each listed spike the workgroup reads the segment of the spiking row
 // that targets its tile (a per-(row, tile) start table; columns are sorted
 // within rows), and adds the signed counts into an LDS accumulator
-// acc[K][TS] with LDS atomics, after a block-wide prefix sum that spreads the
+// acc[TS][K|1] with LDS atomics, after a block-wide prefix sum that spreads the
 // segments' edges evenly over the threads.  Plastic synapses are still pushed
 // at spike time (their weights may change between windows) into a ring of
 // 2*n_delay slots that is read only for the tile's plastic targets; pushes go
@@ -429,7 +429,7 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_push(PushArgs<WT> a) {
 
 struct TileArgs {
     const long long* clock;
-    int N, B, K, kb, first, TS, n_tiles, nd, M, ring_slots;
+    int N, B, K, KP, kb, first, TS, n_tiles, nd, M, ring_slots;
     float* u;
     float* g;
     signed char* ref;
@@ -460,6 +460,9 @@ struct TileArgs {
     const int* ptg_idx;    // ... (null without plastic synapses)
     int* spk_idx;          // [B][M][N]
     int* spk_cnt;          // [B][M]
+    const uint16_t* tb16;  // segment table as offsets from the row start, or null
+    const int* indptr;
+    int bk4;               // bucket entries are 4 bytes (first edge << 7 | (len - 1) << 5 | step)
     const int2* bk;        // tile buckets (source 1): [B][bk_stride] entries
     const long long* bk_off;  // [n_tiles + 1] offset of each tile's bucket in a fly's arena
     long long bk_stride;
@@ -519,8 +522,12 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
     __shared__ int s_wsum[MAX_WAVES + 1];
     const int TPB = blockDim.x, tid = threadIdx.x, lane = tid & 63;
     const int TS = TPB * NPT;
-    int* acc = lds;                      // [K][TS]
-    int* st_seg = acc + a.K * TS;        // [TPB * IPT]
+    // acc[neuron][KP]: a neuron's K arrivals are adjacent (two steps per ds_read2 with immediate
+    // offsets in the step loop); KP = K | 1 is odd, so the 64 lanes of a wavefront hit 64
+    // different banks
+    const int KP = a.KP;
+    int* acc = lds;                      // [TS][KP]
+    int* st_seg = acc + KP * TS;         // [TPB * IPT]
     int* st_end = st_seg + TPB * IPT;    // [TPB * IPT] inclusive prefix of segment lengths
     int* st_k = st_end + TPB * IPT;      // [TPB * IPT]
     int b, T;
@@ -543,14 +550,17 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
             }
         return;
     }
-    for (int q = tid; q < a.K * TS; q += TPB) acc[q] = 0;
-    const int2* bkp = nullptr;
+    for (int q = tid; q < KP * TS; q += TPB) acc[q] = 0;
+    const int2* bkp = nullptr;  // this tile's bucket: entry e at bkp[e] (8 bytes) or bkp4[e]
+    const int* bkp4 = nullptr;
     int total;
     if constexpr (BK) {
         const long long cap = a.bk_off[T + 1] - a.bk_off[T];
         const int c = a.bk_cnt[(size_t)b * a.n_tiles + T];
         total = c < cap ? c : (int)cap;
-        bkp = a.bk + (size_t)b * a.bk_stride + a.bk_off[T];
+        const size_t first = (size_t)b * a.bk_stride + a.bk_off[T];  // entry index, either width
+        bkp = a.bk + first;
+        bkp4 = reinterpret_cast<const int*>(a.bk) + first;
         __syncthreads();
     } else {
         if (tid < a.K) {
@@ -591,7 +601,15 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
 #pragma unroll
             for (int r = 0; r < 2; r++) {
                 const int e = e0 + r * TPB;
-                x[r] = e < total ? bkp[e] : make_int2(0, 0);
+                x[r] = make_int2(0, 0);
+                if (e < total) {
+                    if (a.bk4) {  // first edge << 7 | (len - 1) << 5 | step
+                        const int v = bkp4[e];
+                        x[r] = make_int2(v >> 7, ((((v >> 5) & 3) + 1) << 5) | (v & 31));
+                    } else {
+                        x[r] = bkp[e];
+                    }
+                }
             }
             int loc[2][BK_PIECE], val[2][BK_PIECE];
 #pragma unroll
@@ -605,10 +623,10 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
             }
 #pragma unroll
             for (int r = 0; r < 2; r++) {
-                const int slot = (x[r].y & 31) * TS;
+                const int slot = x[r].y & 31;
 #pragma unroll
                 for (int j = 0; j < BK_PIECE; j++)
-                    if (val[r][j] != 0) atomicAdd(&acc[slot + loc[r][j]], val[r][j]);
+                    if (val[r][j] != 0) atomicAdd(&acc[loc[r][j] * KP + slot], val[r][j]);
             }
         }
         __syncthreads();
@@ -635,9 +653,15 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
                 seg[r] = 0;
                 len[r] = 0;
                 if ((unsigned)ids[r] < (unsigned)a.N) {
-                    const int* tr = a.tb + (size_t)ids[r] * (a.n_tiles + 1) + T;
-                    seg[r] = tr[0];
-                    len[r] = tr[1] - seg[r];
+                    if (a.tb16) {
+                        const uint16_t* tr = a.tb16 + (size_t)ids[r] * (a.n_tiles + 1) + T;
+                        seg[r] = a.indptr[ids[r]] + tr[0];
+                        len[r] = (int)tr[1] - (int)tr[0];
+                    } else {
+                        const int* tr = a.tb + (size_t)ids[r] * (a.n_tiles + 1) + T;
+                        seg[r] = tr[0];
+                        len[r] = tr[1] - seg[r];
+                    }
                 }
                 sum += len[r];
             }
@@ -667,7 +691,7 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
                             else lo = mid + 1;
                         }
                         ek[u] = st_seg[lo] + (q - (lo ? st_end[lo - 1] : 0));
-                        slot[u] = st_k[lo] * TS;
+                        slot[u] = st_k[lo];
                     }
                 }
                 int loc[UNROLL], val[UNROLL];
@@ -679,7 +703,7 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
                 }
 #pragma unroll
                 for (int u = 0; u < UNROLL; u++)
-                    if (val[u] != 0) atomicAdd(&acc[slot[u] + loc[u]], val[u]);
+                    if (val[u] != 0) atomicAdd(&acc[loc[u] * KP + slot[u]], val[u]);
             }
             __syncthreads();
         }
@@ -692,7 +716,7 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
             int* r = a.ring + ((size_t)b * a.ring_slots + (rs0 + k) % a.ring_slots) * a.N + i;
             const int v = *r;
             if (v != 0) {
-                atomicAdd(&acc[k * TS + (i - i0)], v);
+                atomicAdd(&acc[(i - i0) * KP + k], v);
                 *r = 0;
             }
         }
@@ -772,13 +796,13 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
     for (int p = 0; p < NPT; p++) {
         const int li = p * TPB + tid, i = i0 + li;
         uint32_t mask = 0;
+        const uint32_t hits = i < a.N ? (uint32_t)hm[li] : 0u;
         if (i < a.N) {
             float u = u_[p], g = g_[p];
             int r = r_[p];
             const float bias = bias_[p];
-            const uint32_t hits = (uint32_t)hm[li];
             for (int k = 0; k < a.K; k++) {
-                const float inj = (float)acc[k * TS + li] * a.c.in_scale;
+                const float inj = (float)acc[li * KP + k] * a.c.in_scale;
                 g = g * a.c.decay_syn + inj;
                 const bool refr = r > 0;
                 const float u_int = u + (g + bias - u) * a.c.decay_m;
@@ -787,11 +811,9 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
                 const float u_kick = u + a.c.kick;
                 u = ((hits >> k) & 1u) ? u_kick : u;
                 const bool sp = u >= a.c.u_th && r <= 0;
-                if (__any(sp)) {  // spikes are rare: most wavefronts skip the resets
-                    u = sp ? a.c.u_reset : u;
-                    r = sp ? a.c.n_ref : r;
-                    mask |= (uint32_t)sp << k;
-                }
+                u = sp ? a.c.u_reset : u;
+                r = sp ? a.c.n_ref : r;
+                mask |= (uint32_t)sp << k;
             }
             if (u != u_[p]) a.u[bn + i] = u;
             if (g != g_[p]) a.g[bn + i] = g;
@@ -862,6 +884,9 @@ struct BucketArgs {
     const int* spk_idx;
     const int* spk_cnt;
     const int* tb;
+    const uint16_t* tb16;
+    const int* indptr;
+    int bk4;
     const long long* bk_off;
     long long bk_stride;
     int2* bk;
@@ -921,9 +946,16 @@ __global__ void __launch_bounds__(MAX_THREADS) tile_bucket(BucketArgs a) {
                 if (own && q < nc) {
                     const int id = s_id[q];
                     if ((unsigned)id < (unsigned)a.N) {
-                        const int* tr = a.tb + (size_t)id * nt1 + T;
-                        lo[u] = tr[0];
-                        hi[u] = tr[1];
+                        if (a.tb16) {  // half the bytes of the int32 table
+                            const uint16_t* tr = a.tb16 + (size_t)id * nt1 + T;
+                            const int r0 = a.indptr[id];
+                            lo[u] = r0 + tr[0];
+                            hi[u] = r0 + tr[1];
+                        } else {
+                            const int* tr = a.tb + (size_t)id * nt1 + T;
+                            lo[u] = tr[0];
+                            hi[u] = tr[1];
+                        }
                     }
                 }
             }
@@ -935,7 +967,12 @@ __global__ void __launch_bounds__(MAX_THREADS) tile_bucket(BucketArgs a) {
                     const int k = (int)s_k[q0 + u];
                     for (int c = 0; c < np; c++) {
                         const int l = len - c * BK_PIECE < BK_PIECE ? len - c * BK_PIECE : BK_PIECE;
-                        if (pos + c < cap) a.bk[base + pos + c] = make_int2(lo[u] + c * BK_PIECE, (l << 5) | k);
+                        if (pos + c >= cap) break;
+                        if (a.bk4)
+                            reinterpret_cast<int*>(a.bk)[base + pos + c] =
+                                ((lo[u] + c * BK_PIECE) << 7) | ((l - 1) << 5) | k;
+                        else
+                            a.bk[base + pos + c] = make_int2(lo[u] + c * BK_PIECE, (l << 5) | k);
                     }
                 }
         }
@@ -1002,6 +1039,8 @@ struct fz_engine {
     int* pk = nullptr;  // packed fixed edges (null: the kernel reads col and w)
     int lbits = 0;
     // tile buckets (tile source 1): per-tile offsets in a fly's arena (entries), arena, counts
+    uint16_t* tb16 = nullptr;  // segment table as uint16 offsets from each row's start
+    int bk4 = 0;
     std::vector<long long> bk_off_h;
     long long* bk_off = nullptr;
     long long bk_stride = 0;
@@ -1129,13 +1168,14 @@ dim3 push_grid(const fz_engine* e, const Shape& s) {
     return dim3(gx, e->st.B);
 }
 
-TileArgs make_tile(const fz_engine* e, const fz_inputs* in, int K, int kb, int first, const unsigned* hits,
+TileArgs make_tile(const fz_engine* e, const fz_inputs* in, int K, int KP, int kb, int first, const unsigned* hits,
                    unsigned* raster) {
     TileArgs a{};
     a.clock = e->st.clock;
     a.N = e->n;
     a.B = e->st.B;
     a.K = K;
+    a.KP = KP;
     a.kb = kb;
     a.first = first;
     a.TS = e->ts;
@@ -1164,6 +1204,9 @@ TileArgs make_tile(const fz_engine* e, const fz_inputs* in, int K, int kb, int f
     a.pk = e->pk;
     a.lbits = e->lbits;
     a.tb = e->tb;
+    a.tb16 = e->tb16;
+    a.indptr = e->indptr;
+    a.bk4 = e->bk4;
     a.bk = e->bk;
     a.bk_off = e->bk_off;
     a.bk_stride = e->bk_stride;
@@ -1206,8 +1249,15 @@ void* tile_kernel(int fmt, int npt, int ipt, int src) {
 }
 int tile_fmt(const fz_engine* e) { return e->pk ? 2 : (e->w_bits == 16 ? 0 : 1); }
 
+// Row stride of the LDS accumulator: K | 1 (odd: no bank conflicts in the step loop) when the
+// extra column fits the LDS budget, else K.
+int tile_kp(const fz_engine* e, const Shape& s, int K) {
+    const size_t rest = (size_t)3 * s.ipt * s.ut * 4 + (size_t)e->ts * 4;
+    return (size_t)(K | 1) * e->ts * 4 + rest <= e->lds_max - TILE_STATIC_LDS ? (K | 1) : K;
+}
+
 size_t tile_lds(const fz_engine* e, const Shape& s, int K) {
-    return (size_t)K * e->ts * 4 + (size_t)3 * s.ipt * s.ut * 4 + (size_t)e->ts * 4;  // acc, staging, hit masks
+    return (size_t)tile_kp(e, s, K) * e->ts * 4 + (size_t)3 * s.ipt * s.ut * 4 + (size_t)e->ts * 4;  // acc, staging, hit masks
 }
 
 __global__ void tile_seg_count(int n, int n_tiles, const int* tb, int* cnt) {
@@ -1273,6 +1323,17 @@ int validate_tiles(fz_engine* e, const int* tb, int n_tiles, int tile) {
     return FZ_OK;
 }
 
+__global__ void tile_rel16(int n, int n_tiles, const int* indptr, const int* tb, uint16_t* tb16, int* over) {
+    const long long total = (long long)n * (n_tiles + 1);
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+         q += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(q / (n_tiles + 1));
+        const int d = tb[q] - indptr[i];
+        if (d < 0 || d > 65535) *over = 1;
+        tb16[q] = (uint16_t)d;
+    }
+}
+
 // Bucket capacities: tile t can receive, per block, ceil(segment length / BK_PIECE) entries per
 // row with an edge into t, times the spikes a neuron can emit in one block (ceil(K / n_ref),
 // K <= kmax).
@@ -1303,6 +1364,29 @@ int bucket_layout(fz_engine* e, const int* tb, int n_tiles) {
     const long long maxspk = (e->kmax + per - 1) / per;
     for (int t = 0; t < n_tiles; t++) e->bk_off_h[t + 1] = e->bk_off_h[t] + (long long)h[t] * maxspk;
     e->bk_stride = e->bk_off_h[n_tiles];
+    e->bk4 = e->nnz < (1ll << 24) && e->kmax <= 32;  // first edge fits 24 bits beside length and step
+    // uint16 copy of the table (row offsets) when every row has fewer than 65,536 fixed edges
+    (void)hipFree(e->tb16);
+    e->tb16 = nullptr;
+    {
+        uint16_t* t16 = nullptr;
+        int* over = nullptr;
+        const size_t nt = (size_t)e->n * ((size_t)n_tiles + 1);
+        if (hipMalloc(&t16, nt * sizeof(uint16_t)) == hipSuccess && hipMalloc(&over, sizeof(int)) == hipSuccess &&
+            hipMemset(over, 0, sizeof(int)) == hipSuccess) {
+            const unsigned g = (unsigned)(nt / 256 + 1 < 65536 ? nt / 256 + 1 : 65536);
+            hipLaunchKernelGGL(tile_rel16, dim3(g), dim3(256), 0, 0, e->n, n_tiles, e->indptr, tb, t16, over);
+            int h = 1;
+            if (hipGetLastError() == hipSuccess && hipMemcpy(&h, over, sizeof(int), hipMemcpyDeviceToHost) == hipSuccess &&
+                h == 0) {
+                e->tb16 = t16;
+                t16 = nullptr;
+            }
+        }
+        (void)hipGetLastError();
+        (void)hipFree(t16);
+        (void)hipFree(over);
+    }
     HIPCHK(hipMalloc(&e->bk_off, sizeof(long long) * ((size_t)n_tiles + 1)));
     HIPCHK(hipMemcpy(e->bk_off, e->bk_off_h.data(), sizeof(long long) * ((size_t)n_tiles + 1),
                      hipMemcpyHostToDevice));
@@ -1343,6 +1427,9 @@ BucketArgs make_bucket(const fz_engine* e, const fz_inputs* in, int K, int kb) {
     a.spk_idx = e->st.spk_idx;
     a.spk_cnt = e->st.spk_cnt;
     a.tb = e->tb;
+    a.tb16 = e->tb16;
+    a.indptr = e->indptr;
+    a.bk4 = e->bk4;
     a.bk_off = e->bk_off;
     a.bk_stride = e->bk_stride;
     a.bk = e->bk;
@@ -1384,7 +1471,7 @@ int enqueue(fz_engine* e, const fz_inputs* in, int steps, const unsigned* hits, 
                 HIPCHK(hipGetLastError());
                 if ((rc = mark(ev_push))) return rc;
             }
-            TileArgs ta = make_tile(e, in, K, kb, kb == 0, hits, raster);
+            TileArgs ta = make_tile(e, in, K, tile_kp(e, s, K), kb, kb == 0, hits, raster);
             void* args[] = {&ta};
             if ((rc = mark(ev_upd))) return rc;
             HIPCHK(hipLaunchKernel(fn, tgrid, dim3(s.ut), args, tile_lds(e, s, K), stream));
@@ -1484,6 +1571,7 @@ int fz_destroy(fz_engine* e) {
     (void)hipFree(e->bk_off);
     (void)hipFree(e->bk);
     (void)hipFree(e->bk_cnt);
+    (void)hipFree(e->tb16);
     delete e;
     return FZ_OK;
 }
@@ -1729,7 +1817,7 @@ int fz_graph_create(fz_engine* e, const fz_inputs* in, int steps, const fz_tune*
                 if ((err = add(reinterpret_cast<void*>(&tile_bucket), bgrid, dim3(BK_THREADS), bargs)) != hipSuccess)
                     return bail("hipGraphAddKernelNode(bucket)", err);
             }
-            TileArgs ta = make_tile(e, in, K, kb, kb == 0, nullptr, nullptr);
+            TileArgs ta = make_tile(e, in, K, tile_kp(e, s, K), kb, kb == 0, nullptr, nullptr);
             void* targs[] = {&ta};
             if ((err = add(fn, tgrid, dim3(s.ut), targs, tile_lds(e, s, K))) != hipSuccess)
                 return bail("hipGraphAddKernelNode(tile)", err);

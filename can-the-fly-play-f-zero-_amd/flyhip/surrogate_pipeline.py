@@ -220,7 +220,9 @@ def main(argv=None):
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--test-frames", type=int, default=200, help="frames per held-out sequence")
     ap.add_argument("--test-sequences", type=int, default=250)
-    ap.add_argument("--replicas", type=int, default=32)
+    ap.add_argument("--replicas", type=int, default=32, help="simulated flies per held-out sequence")
+    ap.add_argument("--train-replicas", type=int, default=1,
+                    help="flies per training sequence; targets are their mean rate (same simulated frames)")
     ap.add_argument("--members", type=int, default=3)
     ap.add_argument("--steps", type=int, default=6000)
     ap.add_argument("--lr", type=float, default=1e-3)
@@ -237,7 +239,12 @@ def main(argv=None):
 
     fly = flywire_like() if not args.n else flywire_like(n=args.n, n_edges=args.n * 109, seed=11)
     t0 = time.perf_counter()
-    X, Y, sim_fps = generate(fly, args.train_frames, args.batch, seed=1, log=log)
+    X, Y, sim_fps = generate(fly, args.train_frames, args.batch, seed=1, replicas=args.train_replicas, log=log)
+    if args.train_replicas > 1:  # one input per sequence, the replicas' mean rate as its target
+        R, F = args.train_replicas, X.shape[0]
+        X = X.view(F, -1, R, X.shape[-1])[:, :, 0].contiguous()
+        Y = Y.view(F, -1, R, 8).mean(2).contiguous()
+        torch.cuda.empty_cache()
     log(f"training data: {tuple(X.shape)} in {time.perf_counter() - t0:.1f}s ({sim_fps:.0f} sim frames/s); "
         f"DN rate mean {float(Y.mean()):.2f} Hz, sd {float(Y.std()):.2f}")
     t0 = time.perf_counter()
@@ -255,7 +262,8 @@ def main(argv=None):
     f = res["fallback_frac"]
     # every frame runs the ensemble; a fallback frame also costs one simulated frame
     eff = 1.0 / (1.0 / fps + f / sim_fps)
-    res.update({"train_frames": args.train_frames * args.batch, "train_steps": args.steps,
+    res.update({"train_frames": args.train_frames * args.batch, "train_replicas": args.train_replicas,
+                "train_steps": args.steps, "rate_scale": args.rate_scale,
                 "train_s": round(train_s, 1), "members": args.members,
                 "loss_curves": [c[::6] + [c[-1]] for c in curves], "surrogate_frames_per_s": round(fps),
                 "sim_frames_per_s": round(sim_fps), "speedup_vs_sim": round(fps / sim_fps, 1),

@@ -363,3 +363,56 @@ def test_tile_engine_rejects_a_bad_segment_table(small_fly):
     with pytest.raises(ValueError, match="segment table"):
         _lib.check(gb._lib.fz_set_tiles(gb._h, gb.tile, bad.shape[1] - 1, bad.data_ptr(), None, None),
                    "fz_set_tiles")
+
+
+def _random_graph(n, deg, seed, hub_deg=0):
+    # Dale's law per pre neuron, counts 1..4; optional hub row 0 with hub_deg targets
+    rng = np.random.default_rng(seed)
+    rows = np.repeat(np.arange(n, dtype=np.int64), deg)
+    cols = rng.integers(0, n, rows.size)
+    if hub_deg:
+        rows = np.r_[rows, np.zeros(hub_deg, np.int64)]
+        cols = np.r_[cols, rng.choice(n, hub_deg, replace=False)]
+    sign = np.where(rng.random(n) < 0.55, 1.0, -1.0)
+    vals = sign[rows] * rng.integers(1, 5, rows.size)
+    w = sp.csr_matrix((vals.astype(np.float32), (rows, cols)), shape=(n, n))
+    w.sum_duplicates()
+    return w
+
+
+@pytest.mark.parametrize("source", [0, 1])
+def test_tile_engine_row_longer_than_65535_edges(source):
+    # the uint16 segment-table copy cannot hold this row: the int32 table is used instead
+    w = _random_graph(80_000, 20, seed=31, hub_deg=70_000)
+    assert np.diff(w.indptr).max() >= 65_536
+    p = LIFParams()
+    gb = BatchBrain(w, p, batch=2, seed=3, engine="tile", tile=1024)
+    ob = OracleBrain(w, p, batch=2, seed=3)
+    idx = np.r_[0, np.arange(1, 4000)]
+    rate = np.r_[np.float32(400.0), np.full(3999, 30.0, np.float32)]
+    for b in (gb, ob):
+        b.set_input(idx, rate)
+    gb.set_tune(upd_threads=512, tile_source=source)
+    hub = 0
+    for frame in range(3):
+        co = ob.run(FRAME_MS)
+        np.testing.assert_array_equal(_np(gb.run(FRAME_MS)), co, err_msg=f"frame {frame}")
+        hub += int(co[:, 0].sum())
+    assert hub > 0                                       # the hub row was pushed
+    _assert_state_equal(gb, ob)
+
+
+def test_tile_engine_with_more_than_2_24_edges():
+    # bucket entries cannot pack a first-edge index of 25 bits: 8-byte entries are used
+    w = _random_graph(60_000, 285, seed=32)
+    assert w.nnz >= 1 << 24
+    p = LIFParams()
+    gb = BatchBrain(w, p, batch=1, seed=4, engine="tile", tile=1024)
+    ob = OracleBrain(w, p, batch=1, seed=4)
+    idx = np.arange(0, 6000)
+    for b in (gb, ob):
+        b.set_input(idx, 40.0)
+    gb.set_tune(upd_threads=512, tile_source=1)
+    for frame in range(2):
+        np.testing.assert_array_equal(_np(gb.run(FRAME_MS)), ob.run(FRAME_MS), err_msg=f"frame {frame}")
+    _assert_state_equal(gb, ob)
