@@ -247,8 +247,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--batch", type=int, default=256, help="flies in the batched line (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=10)
-    ap.add_argument("--engines1", default="push,tile256,tile512,tile1024")
-    ap.add_argument("--enginesB", default="tile1024,tile2048,push")
+    ap.add_argument("--engines1", default="push,tile512,tile1024")
+    ap.add_argument("--enginesB", default="tile512,tile1024")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample length")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plastic-frames", type=int, default=300, help="frames of the B=64 plasticity line (0: skip)")

@@ -868,15 +868,24 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
 
 // Bucket prologue of a tile-engine block (tile source 1): the spikes whose arrivals fall in
 // the block [t0, t0 + K) (emitted at steps [t0 - n_delay, t0 - n_delay + K)) are expanded once
-// into per-tile buckets of (first edge, length << 5 | step in block), segments longer than
-// BK_PIECE edges split into pieces, so that each (fly, tile)
-// workgroup reads only the segments that reach it, contiguously, instead of looking every
-// spike of the fly up in the segment table.  Grid (ceil(n_tiles / 64), B): lane t of every
-// wavefront owns tile 64 * blockIdx.x + t; the wavefronts split the spikes, staged through
-// LDS in chunks; positions come from LDS counters, so a bucket's order depends on timing while
-// its contents (and the integer sums made from them) do not.
-constexpr int BK_CHUNK = 2048;
-constexpr int BK_UNROLL = 16;  // table loads in flight per lane
+// into per-tile buckets of pieces of at most BK_PIECE edges (first edge, length, step in block),
+// so that each (fly, tile) workgroup reads only the pieces that reach it, contiguously, instead
+// of looking every spike of the fly up in the segment table.
+//
+// Grid (ceil(n_tiles / 64), B), BK_WAVES wavefronts: lane t of every wavefront owns tile
+// 64 * blockIdx.x + t, so the workgroup alone writes these 64 buckets of its fly and needs no
+// atomics.  Spikes go in chunks of BK_CH; wavefront w takes BK_CH / BK_WAVES consecutive
+// spikes of a chunk.  Per chunk: (1) each lane counts its tile's pieces, keeping the table
+// entries in registers; (2) one wavefront turns the counts into per-wavefront offsets, staging
+// offsets (tile-major) and bucket positions; (3) the pieces are placed in LDS; (4) the
+// workgroup copies them to the buckets, consecutive threads to consecutive entries of a bucket.
+// A bucket's order is deterministic (chunk, wavefront, spike).  8-byte entries, or chunks with
+// more than BK_STAGE pieces, are written directly in step (3).
+constexpr int BK_WAVES = 16;
+constexpr int BK_THREADS = 64 * BK_WAVES;
+constexpr int BK_CH = 256;                 // spikes per chunk
+constexpr int BK_PER = BK_CH / BK_WAVES;   // spikes per wavefront and chunk
+constexpr int BK_STAGE = 8192;             // staged pieces per chunk
 
 struct BucketArgs {
     const long long* clock;
@@ -895,12 +904,17 @@ struct BucketArgs {
     int* err;
 };
 
-__global__ void __launch_bounds__(MAX_THREADS) tile_bucket(BucketArgs a) {
+__global__ void __launch_bounds__(BK_THREADS) tile_bucket(BucketArgs a) {
     __shared__ int s_off[KMAX + 1];
-    __shared__ int s_cnt[64];
-    __shared__ int s_id[BK_CHUNK];
-    __shared__ unsigned char s_k[BK_CHUNK];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, W = blockDim.x >> 6;
+    __shared__ int s_id[BK_CH];
+    __shared__ unsigned char s_k[BK_CH];
+    __shared__ int s_cw[BK_WAVES][64];  // pieces per (wavefront, tile) -> exclusive prefix over wavefronts
+    __shared__ int s_stg[65];           // staging offset of each tile's run; [64] = chunk total
+    __shared__ int s_gold[64];          // bucket fill of each tile before this chunk
+    __shared__ int s_gpos[64];          // bucket fill of each tile
+    __shared__ int s_val[BK_STAGE];
+    __shared__ int s_dst[BK_STAGE];     // entry index within the fly's arena, or -1 (over capacity)
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int b = blockIdx.y, T = blockIdx.x * 64 + lane;
     if (a.alive && !a.alive[b]) return;
     const long long t0 = a.clock[0] + a.kb;
@@ -908,7 +922,7 @@ __global__ void __launch_bounds__(MAX_THREADS) tile_bucket(BucketArgs a) {
     const int lt0 = (int)(((t0 % M) + M) % M);
     const int* cnt_b = a.spk_cnt + (size_t)b * M;
     const int* idx_b = a.spk_idx + (size_t)b * M * a.N;
-    if (tid < 64) s_cnt[tid] = 0;
+    if (tid < 64) s_gpos[tid] = 0;
     if (tid == 0) {
         int o = 0;
         for (int k = 0; k < a.K; k++) {
@@ -921,15 +935,13 @@ __global__ void __launch_bounds__(MAX_THREADS) tile_bucket(BucketArgs a) {
     __syncthreads();
     const int total = s_off[a.K];
     const bool own = T < a.n_tiles;
-    long long base = 0, cap = 0;
-    if (own) {
-        base = (long long)b * a.bk_stride + a.bk_off[T];
-        cap = a.bk_off[T + 1] - a.bk_off[T];
-    }
+    const long long off_T = own ? a.bk_off[T] : 0;
+    const long long cap = own ? a.bk_off[T + 1] - off_T : 0;
+    const size_t fly0 = (size_t)b * a.bk_stride;  // the fly's arena, in entries
     const int nt1 = a.n_tiles + 1;
-    for (int c0 = 0; c0 < total; c0 += BK_CHUNK) {
-        const int nc = total - c0 < BK_CHUNK ? total - c0 : BK_CHUNK;
-        for (int q = tid; q < nc; q += blockDim.x) {  // stage the chunk's ids and steps
+    for (int c0 = 0; c0 < total; c0 += BK_CH) {
+        const int nc = total - c0 < BK_CH ? total - c0 : BK_CH;
+        for (int q = tid; q < nc; q += BK_THREADS) {  // stage the chunk's ids and steps
             const int e = c0 + q;
             int k = 0;
             while (s_off[k + 1] <= e) k++;
@@ -937,49 +949,92 @@ __global__ void __launch_bounds__(MAX_THREADS) tile_bucket(BucketArgs a) {
             s_k[q] = (unsigned char)k;
         }
         __syncthreads();
-        for (int q0 = wave * BK_UNROLL; q0 < nc; q0 += W * BK_UNROLL) {
-            int lo[BK_UNROLL], hi[BK_UNROLL];
+        // (1) this lane's tile segments of the wavefront's spikes, and their piece count
+        int lo[BK_PER], hi[BK_PER];
+        int cnt = 0;
 #pragma unroll
-            for (int u = 0; u < BK_UNROLL; u++) {
-                lo[u] = hi[u] = 0;
-                const int q = q0 + u;
-                if (own && q < nc) {
-                    const int id = s_id[q];
-                    if ((unsigned)id < (unsigned)a.N) {
-                        if (a.tb16) {  // half the bytes of the int32 table
-                            const uint16_t* tr = a.tb16 + (size_t)id * nt1 + T;
-                            const int r0 = a.indptr[id];
-                            lo[u] = r0 + tr[0];
-                            hi[u] = r0 + tr[1];
-                        } else {
-                            const int* tr = a.tb + (size_t)id * nt1 + T;
-                            lo[u] = tr[0];
-                            hi[u] = tr[1];
-                        }
+        for (int u = 0; u < BK_PER; u++) {
+            lo[u] = hi[u] = 0;
+            const int q = wave * BK_PER + u;
+            if (own && q < nc) {
+                const int id = s_id[q];
+                if ((unsigned)id < (unsigned)a.N) {
+                    if (a.tb16) {  // half the bytes of the int32 table
+                        const uint16_t* tr = a.tb16 + (size_t)id * nt1 + T;
+                        const int r0 = a.indptr[id];
+                        lo[u] = r0 + tr[0];
+                        hi[u] = r0 + tr[1];
+                    } else {
+                        const int* tr = a.tb + (size_t)id * nt1 + T;
+                        lo[u] = tr[0];
+                        hi[u] = tr[1];
                     }
                 }
             }
+        }
 #pragma unroll
-            for (int u = 0; u < BK_UNROLL; u++)
-                if (hi[u] > lo[u]) {
-                    const int len = hi[u] - lo[u], np = (len + BK_PIECE - 1) / BK_PIECE;
-                    const int pos = atomicAdd(&s_cnt[lane], np);
-                    const int k = (int)s_k[q0 + u];
-                    for (int c = 0; c < np; c++) {
-                        const int l = len - c * BK_PIECE < BK_PIECE ? len - c * BK_PIECE : BK_PIECE;
-                        if (pos + c >= cap) break;
+        for (int u = 0; u < BK_PER; u++)
+            if (hi[u] > lo[u]) cnt += (hi[u] - lo[u] + BK_PIECE - 1) / BK_PIECE;
+        s_cw[wave][lane] = cnt;
+        __syncthreads();
+        // (2) offsets: over wavefronts per tile, then over tiles (staging is tile-major)
+        if (wave == 0) {
+            int run = 0;
+            for (int w = 0; w < BK_WAVES; w++) {
+                const int c = s_cw[w][lane];
+                s_cw[w][lane] = run;
+                run += c;
+            }
+            int x = run;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int y = __shfl_up(x, o);
+                if (lane >= o) x += y;
+            }
+            s_stg[lane] = x - run;
+            if (lane == 63) s_stg[64] = x;
+            s_gold[lane] = s_gpos[lane];
+            s_gpos[lane] += run;
+        }
+        __syncthreads();
+        const int chunk = s_stg[64];
+        const bool stage = a.bk4 && chunk <= BK_STAGE;
+        // (3) place the pieces
+        {
+            int sp = s_stg[lane] + s_cw[wave][lane];       // staging slot
+            long long gp = s_gold[lane] + s_cw[wave][lane];  // position in the tile's bucket
+#pragma unroll
+            for (int u = 0; u < BK_PER; u++) {
+                if (hi[u] <= lo[u]) continue;
+                const int k = (int)s_k[wave * BK_PER + u];
+                for (int e = lo[u]; e < hi[u]; e += BK_PIECE, sp++, gp++) {
+                    const int l = hi[u] - e < BK_PIECE ? hi[u] - e : BK_PIECE;
+                    const bool fits = gp < cap;
+                    if (stage) {
+                        s_val[sp] = (e << 7) | ((l - 1) << 5) | k;
+                        s_dst[sp] = fits ? (int)(off_T + gp) : -1;
+                    } else if (fits) {
                         if (a.bk4)
-                            reinterpret_cast<int*>(a.bk)[base + pos + c] =
-                                ((lo[u] + c * BK_PIECE) << 7) | ((l - 1) << 5) | k;
+                            reinterpret_cast<int*>(a.bk)[fly0 + off_T + gp] = (e << 7) | ((l - 1) << 5) | k;
                         else
-                            a.bk[base + pos + c] = make_int2(lo[u] + c * BK_PIECE, (l << 5) | k);
+                            a.bk[fly0 + off_T + gp] = make_int2(e, (l << 5) | k);
                     }
                 }
+            }
+        }
+        __syncthreads();
+        // (4) copy the staged pieces out, tile by tile
+        if (stage) {
+            int* out = reinterpret_cast<int*>(a.bk) + fly0;
+            for (int i = tid; i < chunk; i += BK_THREADS) {
+                const int d = s_dst[i];
+                if (d >= 0) out[d] = s_val[i];
+            }
         }
         __syncthreads();
     }
     if (tid < 64 && own) {
-        int c = s_cnt[lane];
+        int c = s_gpos[lane];
         if (c > cap) {
             atomicOr(a.err, 2);
             c = (int)cap;
@@ -1439,7 +1494,6 @@ BucketArgs make_bucket(const fz_engine* e, const fz_inputs* in, int K, int kb) {
     return a;
 }
 
-constexpr int BK_THREADS = 1024;
 
 // One window as a sequence of kernel launches (the graph builder mirrors it).
 // With `ev_*` non-null, events bracket every kernel.
