@@ -429,7 +429,7 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_push(PushArgs<WT> a) {
 
 struct TileArgs {
     const long long* clock;
-    int N, B, K, KP, kb, first, TS, n_tiles, nd, M, ring_slots;
+    int N, B, K, KP, kb, blk, first, TS, n_tiles, nd, M, ring_slots;
     float* u;
     float* g;
     signed char* ref;
@@ -555,10 +555,11 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
     const int* bkp4 = nullptr;
     int total;
     if constexpr (BK) {
+        const int par = (int)((a.clock[2] + a.blk) & 1);
         const long long cap = a.bk_off[T + 1] - a.bk_off[T];
-        const int c = a.bk_cnt[(size_t)b * a.n_tiles + T];
+        const int c = a.bk_cnt[((size_t)par * a.B + b) * a.n_tiles + T];
         total = c < cap ? c : (int)cap;
-        const size_t first = (size_t)b * a.bk_stride + a.bk_off[T];  // entry index, either width
+        const size_t first = ((size_t)par * a.B + b) * a.bk_stride + a.bk_off[T];  // entry index, either width
         bkp = a.bk + first;
         bkp4 = reinterpret_cast<const int*>(a.bk) + first;
         __syncthreads();
@@ -868,28 +869,20 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
 
 // Bucket prologue of a tile-engine block (tile source 1): the spikes whose arrivals fall in
 // the block [t0, t0 + K) (emitted at steps [t0 - n_delay, t0 - n_delay + K)) are expanded once
-// into per-tile buckets of pieces of at most BK_PIECE edges (first edge, length, step in block),
-// so that each (fly, tile) workgroup reads only the pieces that reach it, contiguously, instead
-// of looking every spike of the fly up in the segment table.
-//
-// Grid (ceil(n_tiles / 64), B), BK_WAVES wavefronts: lane t of every wavefront owns tile
-// 64 * blockIdx.x + t, so the workgroup alone writes these 64 buckets of its fly and needs no
-// atomics.  Spikes go in chunks of BK_CH; wavefront w takes BK_CH / BK_WAVES consecutive
-// spikes of a chunk.  Per chunk: (1) each lane counts its tile's pieces, keeping the table
-// entries in registers; (2) one wavefront turns the counts into per-wavefront offsets, staging
-// offsets (tile-major) and bucket positions; (3) the pieces are placed in LDS; (4) the
-// workgroup copies them to the buckets, consecutive threads to consecutive entries of a bucket.
-// A bucket's order is deterministic (chunk, wavefront, spike).  8-byte entries, or chunks with
-// more than BK_STAGE pieces, are written directly in step (3).
-constexpr int BK_WAVES = 16;
-constexpr int BK_THREADS = 64 * BK_WAVES;
-constexpr int BK_CH = 256;                 // spikes per chunk
-constexpr int BK_PER = BK_CH / BK_WAVES;   // spikes per wavefront and chunk
-constexpr int BK_STAGE = 8192;             // staged pieces per chunk
+// into per-tile buckets of (first edge, length << 5 | step in block), segments longer than
+// BK_PIECE edges split into pieces, so that each (fly, tile)
+// workgroup reads only the segments that reach it, contiguously, instead of looking every
+// spike of the fly up in the segment table.  Grid (ceil(n_tiles / 64), B): lane t of every
+// wavefront owns tile 64 * blockIdx.x + t; the wavefronts split the spikes, staged through
+// LDS in chunks; positions come from LDS counters, so a bucket's order depends on timing while
+// its contents (and the integer sums made from them) do not.
+constexpr int BK_CHUNK = 2048;
+constexpr int BK_UNROLL = 16;  // table loads in flight per lane
+constexpr int BK_THREADS = 1024;
 
 struct BucketArgs {
     const long long* clock;
-    int N, B, K, kb, nd, M, n_tiles;
+    int N, B, K, kb, blk, nd, M, n_tiles;
     const int* spk_idx;
     const int* spk_cnt;
     const int* tb;
@@ -904,17 +897,12 @@ struct BucketArgs {
     int* err;
 };
 
-__global__ void __launch_bounds__(BK_THREADS) tile_bucket(BucketArgs a) {
+__global__ void __launch_bounds__(MAX_THREADS) tile_bucket(BucketArgs a) {
     __shared__ int s_off[KMAX + 1];
-    __shared__ int s_id[BK_CH];
-    __shared__ unsigned char s_k[BK_CH];
-    __shared__ int s_cw[BK_WAVES][64];  // pieces per (wavefront, tile) -> exclusive prefix over wavefronts
-    __shared__ int s_stg[65];           // staging offset of each tile's run; [64] = chunk total
-    __shared__ int s_gold[64];          // bucket fill of each tile before this chunk
-    __shared__ int s_gpos[64];          // bucket fill of each tile
-    __shared__ int s_val[BK_STAGE];
-    __shared__ int s_dst[BK_STAGE];     // entry index within the fly's arena, or -1 (over capacity)
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    __shared__ int s_cnt[64];
+    __shared__ int s_id[BK_CHUNK];
+    __shared__ unsigned char s_k[BK_CHUNK];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, W = blockDim.x >> 6;
     const int b = blockIdx.y, T = blockIdx.x * 64 + lane;
     if (a.alive && !a.alive[b]) return;
     const long long t0 = a.clock[0] + a.kb;
@@ -922,7 +910,7 @@ __global__ void __launch_bounds__(BK_THREADS) tile_bucket(BucketArgs a) {
     const int lt0 = (int)(((t0 % M) + M) % M);
     const int* cnt_b = a.spk_cnt + (size_t)b * M;
     const int* idx_b = a.spk_idx + (size_t)b * M * a.N;
-    if (tid < 64) s_gpos[tid] = 0;
+    if (tid < 64) s_cnt[tid] = 0;
     if (tid == 0) {
         int o = 0;
         for (int k = 0; k < a.K; k++) {
@@ -935,13 +923,18 @@ __global__ void __launch_bounds__(BK_THREADS) tile_bucket(BucketArgs a) {
     __syncthreads();
     const int total = s_off[a.K];
     const bool own = T < a.n_tiles;
-    const long long off_T = own ? a.bk_off[T] : 0;
-    const long long cap = own ? a.bk_off[T + 1] - off_T : 0;
-    const size_t fly0 = (size_t)b * a.bk_stride;  // the fly's arena, in entries
+    // buffers alternate with the block parity, so the bucket of block j can be written while
+    // the tile kernel of block j - 1 reads the other one
+    const int par = (int)((a.clock[2] + a.blk) & 1);
+    long long base = 0, cap = 0;
+    if (own) {
+        base = ((long long)par * a.B + b) * a.bk_stride + a.bk_off[T];
+        cap = a.bk_off[T + 1] - a.bk_off[T];
+    }
     const int nt1 = a.n_tiles + 1;
-    for (int c0 = 0; c0 < total; c0 += BK_CH) {
-        const int nc = total - c0 < BK_CH ? total - c0 : BK_CH;
-        for (int q = tid; q < nc; q += BK_THREADS) {  // stage the chunk's ids and steps
+    for (int c0 = 0; c0 < total; c0 += BK_CHUNK) {
+        const int nc = total - c0 < BK_CHUNK ? total - c0 : BK_CHUNK;
+        for (int q = tid; q < nc; q += blockDim.x) {  // stage the chunk's ids and steps
             const int e = c0 + q;
             int k = 0;
             while (s_off[k + 1] <= e) k++;
@@ -949,97 +942,54 @@ __global__ void __launch_bounds__(BK_THREADS) tile_bucket(BucketArgs a) {
             s_k[q] = (unsigned char)k;
         }
         __syncthreads();
-        // (1) this lane's tile segments of the wavefront's spikes, and their piece count
-        int lo[BK_PER], hi[BK_PER];
-        int cnt = 0;
+        for (int q0 = wave * BK_UNROLL; q0 < nc; q0 += W * BK_UNROLL) {
+            int lo[BK_UNROLL], hi[BK_UNROLL];
 #pragma unroll
-        for (int u = 0; u < BK_PER; u++) {
-            lo[u] = hi[u] = 0;
-            const int q = wave * BK_PER + u;
-            if (own && q < nc) {
-                const int id = s_id[q];
-                if ((unsigned)id < (unsigned)a.N) {
-                    if (a.tb16) {  // half the bytes of the int32 table
-                        const uint16_t* tr = a.tb16 + (size_t)id * nt1 + T;
-                        const int r0 = a.indptr[id];
-                        lo[u] = r0 + tr[0];
-                        hi[u] = r0 + tr[1];
-                    } else {
-                        const int* tr = a.tb + (size_t)id * nt1 + T;
-                        lo[u] = tr[0];
-                        hi[u] = tr[1];
+            for (int u = 0; u < BK_UNROLL; u++) {
+                lo[u] = hi[u] = 0;
+                const int q = q0 + u;
+                if (own && q < nc) {
+                    const int id = s_id[q];
+                    if ((unsigned)id < (unsigned)a.N) {
+                        if (a.tb16) {  // half the bytes of the int32 table
+                            const uint16_t* tr = a.tb16 + (size_t)id * nt1 + T;
+                            const int r0 = a.indptr[id];
+                            lo[u] = r0 + tr[0];
+                            hi[u] = r0 + tr[1];
+                        } else {
+                            const int* tr = a.tb + (size_t)id * nt1 + T;
+                            lo[u] = tr[0];
+                            hi[u] = tr[1];
+                        }
                     }
                 }
             }
-        }
 #pragma unroll
-        for (int u = 0; u < BK_PER; u++)
-            if (hi[u] > lo[u]) cnt += (hi[u] - lo[u] + BK_PIECE - 1) / BK_PIECE;
-        s_cw[wave][lane] = cnt;
-        __syncthreads();
-        // (2) offsets: over wavefronts per tile, then over tiles (staging is tile-major)
-        if (wave == 0) {
-            int run = 0;
-            for (int w = 0; w < BK_WAVES; w++) {
-                const int c = s_cw[w][lane];
-                s_cw[w][lane] = run;
-                run += c;
-            }
-            int x = run;
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const int y = __shfl_up(x, o);
-                if (lane >= o) x += y;
-            }
-            s_stg[lane] = x - run;
-            if (lane == 63) s_stg[64] = x;
-            s_gold[lane] = s_gpos[lane];
-            s_gpos[lane] += run;
-        }
-        __syncthreads();
-        const int chunk = s_stg[64];
-        const bool stage = a.bk4 && chunk <= BK_STAGE;
-        // (3) place the pieces
-        {
-            int sp = s_stg[lane] + s_cw[wave][lane];       // staging slot
-            long long gp = s_gold[lane] + s_cw[wave][lane];  // position in the tile's bucket
-#pragma unroll
-            for (int u = 0; u < BK_PER; u++) {
-                if (hi[u] <= lo[u]) continue;
-                const int k = (int)s_k[wave * BK_PER + u];
-                for (int e = lo[u]; e < hi[u]; e += BK_PIECE, sp++, gp++) {
-                    const int l = hi[u] - e < BK_PIECE ? hi[u] - e : BK_PIECE;
-                    const bool fits = gp < cap;
-                    if (stage) {
-                        s_val[sp] = (e << 7) | ((l - 1) << 5) | k;
-                        s_dst[sp] = fits ? (int)(off_T + gp) : -1;
-                    } else if (fits) {
+            for (int u = 0; u < BK_UNROLL; u++)
+                if (hi[u] > lo[u]) {
+                    const int len = hi[u] - lo[u], np = (len + BK_PIECE - 1) / BK_PIECE;
+                    const int pos = atomicAdd(&s_cnt[lane], np);
+                    const int k = (int)s_k[q0 + u];
+                    for (int c = 0; c < np; c++) {
+                        const int l = len - c * BK_PIECE < BK_PIECE ? len - c * BK_PIECE : BK_PIECE;
+                        if (pos + c >= cap) break;
                         if (a.bk4)
-                            reinterpret_cast<int*>(a.bk)[fly0 + off_T + gp] = (e << 7) | ((l - 1) << 5) | k;
+                            reinterpret_cast<int*>(a.bk)[base + pos + c] =
+                                ((lo[u] + c * BK_PIECE) << 7) | ((l - 1) << 5) | k;
                         else
-                            a.bk[fly0 + off_T + gp] = make_int2(e, (l << 5) | k);
+                            a.bk[base + pos + c] = make_int2(lo[u] + c * BK_PIECE, (l << 5) | k);
                     }
                 }
-            }
-        }
-        __syncthreads();
-        // (4) copy the staged pieces out, tile by tile
-        if (stage) {
-            int* out = reinterpret_cast<int*>(a.bk) + fly0;
-            for (int i = tid; i < chunk; i += BK_THREADS) {
-                const int d = s_dst[i];
-                if (d >= 0) out[d] = s_val[i];
-            }
         }
         __syncthreads();
     }
     if (tid < 64 && own) {
-        int c = s_gpos[lane];
+        int c = s_cnt[lane];
         if (c > cap) {
             atomicOr(a.err, 2);
             c = (int)cap;
         }
-        a.bk_cnt[(size_t)b * a.n_tiles + T] = c;
+        a.bk_cnt[((size_t)par * a.B + b) * a.n_tiles + T] = c;
     }
 }
 
@@ -1223,9 +1173,10 @@ dim3 push_grid(const fz_engine* e, const Shape& s) {
     return dim3(gx, e->st.B);
 }
 
-TileArgs make_tile(const fz_engine* e, const fz_inputs* in, int K, int KP, int kb, int first, const unsigned* hits,
-                   unsigned* raster) {
+TileArgs make_tile(const fz_engine* e, const fz_inputs* in, int K, int KP, int kb, int blk, int first,
+                   const unsigned* hits, unsigned* raster) {
     TileArgs a{};
+    a.blk = blk;
     a.clock = e->st.clock;
     a.N = e->n;
     a.B = e->st.B;
@@ -1456,21 +1407,23 @@ int ensure_buckets(fz_engine* e) {
     e->bk = nullptr;
     e->bk_cnt = nullptr;
     e->bk_B = 0;
-    const size_t ne = (size_t)e->st.B * (size_t)(e->bk_stride > 0 ? e->bk_stride : 1);
+    // two parity buffers of B x stride entries: 4-byte entries fit both in B x stride int2
+    const size_t ne = (size_t)(e->bk4 ? 1 : 2) * (size_t)e->st.B * (size_t)(e->bk_stride > 0 ? e->bk_stride : 1);
     if (hipMalloc(&e->bk, sizeof(int2) * ne) != hipSuccess ||
-        hipMalloc(&e->bk_cnt, sizeof(int) * (size_t)e->st.B * (size_t)e->n_tiles) != hipSuccess) {
+        hipMalloc(&e->bk_cnt, sizeof(int) * 2 * (size_t)e->st.B * (size_t)e->n_tiles) != hipSuccess) {
         (void)hipGetLastError();
         (void)hipFree(e->bk);
         e->bk = nullptr;
         return fail(FZ_E_HIP, "allocating %zu tile-bucket entries failed", ne);
     }
-    HIPCHK(hipMemset(e->bk_cnt, 0, sizeof(int) * (size_t)e->st.B * (size_t)e->n_tiles));
+    HIPCHK(hipMemset(e->bk_cnt, 0, sizeof(int) * 2 * (size_t)e->st.B * (size_t)e->n_tiles));
     e->bk_B = e->st.B;
     return FZ_OK;
 }
 
-BucketArgs make_bucket(const fz_engine* e, const fz_inputs* in, int K, int kb) {
+BucketArgs make_bucket(const fz_engine* e, const fz_inputs* in, int K, int kb, int blk) {
     BucketArgs a{};
+    a.blk = blk;
     a.clock = e->st.clock;
     a.N = e->n;
     a.B = e->st.B;
@@ -1519,13 +1472,13 @@ int enqueue(fz_engine* e, const fz_inputs* in, int steps, const unsigned* hits, 
         while (kb < steps) {
             const int K = (steps - kb) < s.K ? (steps - kb) : s.K;
             if (s.src == 1) {  // timed with the push timers: it is the spike expansion
-                BucketArgs ba = make_bucket(e, in, K, kb);
+                BucketArgs ba = make_bucket(e, in, K, kb, blk);
                 if ((rc = mark(ev_push))) return rc;
                 hipLaunchKernelGGL(tile_bucket, bgrid, dim3(BK_THREADS), 0, stream, ba);
                 HIPCHK(hipGetLastError());
                 if ((rc = mark(ev_push))) return rc;
             }
-            TileArgs ta = make_tile(e, in, K, tile_kp(e, s, K), kb, kb == 0, hits, raster);
+            TileArgs ta = make_tile(e, in, K, tile_kp(e, s, K), kb, blk, kb == 0, hits, raster);
             void* args[] = {&ta};
             if ((rc = mark(ev_upd))) return rc;
             HIPCHK(hipLaunchKernel(fn, tgrid, dim3(s.ut), args, tile_lds(e, s, K), stream));
@@ -1863,19 +1816,56 @@ int fz_graph_create(fz_engine* e, const fz_inputs* in, int steps, const fz_tune*
             delete gr;
             return rc;
         }
+        // The bucket node of block j needs the spikes of steps [kb - n_delay, kb - n_delay + K)
+        // and its parity buffer free (last read by block j - 2): with K <= n_delay / 2 both come
+        // from block j - 2 or earlier, so it runs beside the tile kernel of block j - 1.
+        std::vector<hipGraphNode_t> tiles;
+        std::vector<int> tile_end;  // window step after each tile block
         while (kb < steps) {
             const int K = (steps - kb) < s.K ? (steps - kb) : s.K;
+            hipGraphNode_t bnode = nullptr;
             if (s.src == 1) {
-                BucketArgs ba = make_bucket(e, in, K, kb);
+                BucketArgs ba = make_bucket(e, in, K, kb, blk);
                 void* bargs[] = {&ba};
-                if ((err = add(reinterpret_cast<void*>(&tile_bucket), bgrid, dim3(BK_THREADS), bargs)) != hipSuccess)
+                const int need = kb - e->c.n_delay + K - 1;  // last spike step it reads
+                int dep = blk - 2;
+                for (int i = 0; i < (int)tiles.size(); i++)
+                    if (need >= 0 && tile_end[i] > need) {
+                        dep = i > dep ? i : dep;
+                        break;
+                    }
+                if (dep >= blk) dep = blk - 1;
+                hipKernelNodeParams kp{};
+                kp.func = reinterpret_cast<void*>(&tile_bucket);
+                kp.gridDim = bgrid;
+                kp.blockDim = dim3(BK_THREADS);
+                kp.sharedMemBytes = 0;
+                kp.kernelParams = bargs;
+                kp.extra = nullptr;
+                hipGraphNode_t d = dep >= 0 ? tiles[dep] : nullptr;
+                if ((err = hipGraphAddKernelNode(&bnode, gr->graph, d ? &d : nullptr, d ? 1 : 0, &kp)) != hipSuccess)
                     return bail("hipGraphAddKernelNode(bucket)", err);
             }
-            TileArgs ta = make_tile(e, in, K, tile_kp(e, s, K), kb, kb == 0, nullptr, nullptr);
+            TileArgs ta = make_tile(e, in, K, tile_kp(e, s, K), kb, blk, kb == 0, nullptr, nullptr);
             void* targs[] = {&ta};
-            if ((err = add(fn, tgrid, dim3(s.ut), targs, tile_lds(e, s, K))) != hipSuccess)
+            hipKernelNodeParams kp{};
+            kp.func = fn;
+            kp.gridDim = tgrid;
+            kp.blockDim = dim3(s.ut);
+            kp.sharedMemBytes = (unsigned)tile_lds(e, s, K);
+            kp.kernelParams = targs;
+            kp.extra = nullptr;
+            hipGraphNode_t deps[2];
+            int nd = 0;
+            if (prev) deps[nd++] = prev;
+            if (bnode) deps[nd++] = bnode;
+            hipGraphNode_t tnode;
+            if ((err = hipGraphAddKernelNode(&tnode, gr->graph, nd ? deps : nullptr, nd, &kp)) != hipSuccess)
                 return bail("hipGraphAddKernelNode(tile)", err);
+            prev = tnode;
+            tiles.push_back(tnode);
             kb += K;
+            tile_end.push_back(kb);
             blk++;
         }
     }

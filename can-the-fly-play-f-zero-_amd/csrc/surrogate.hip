@@ -8,7 +8,8 @@
 //   y  = [h2, s'] Wo^T + bo           1280 -> 8
 //
 // sg_gemm: C[M][N] = sum_k A[m][k] B[k][n], A and B bf16 (each optionally stored transposed),
-// FP32 accumulation on v_mfma_f32_16x16x32_bf16.  Workgroup tile 64 x 64 x 32 staged in LDS
+// FP32 accumulation on v_mfma_f32_16x16x32_bf16.  Workgroup tile 64 x 64 x 32 (128 x 128 in
+// sg_gemm_big for the large, aligned products) staged in LDS
 // (k-contiguous rows for both operands), four wavefronts of 32 x 32, bounds-checked
 // epilogues: FP32 store, FP32 accumulate (gradient sums), BF16 store, bias + GELU (FP32
 // pre-activation and BF16 activation).  K must be a multiple of 32.
@@ -139,6 +140,127 @@ __global__ void __launch_bounds__(256) sg_gemm(GemmArgs g) {
             }
 }
 
+// The same product with 128 x 128 workgroup tiles for the large GEMMs (M, N >= 128; operands
+// 16-byte aligned with leading dimensions a multiple of 8): four wavefronts of 64 x 64 (4 x 4
+// MFMA tiles), 16-byte global loads staged through registers (the next k-slice is loaded while
+// the current one is multiplied), 16-byte LDS fragment reads.  Operands stored along k (A not
+// transposed, B transposed) go to LDS as 16-byte rows; the others are scattered into k-rows.
+constexpr int GB_M = 128, GB_N = 128;
+
+typedef __attribute__((__vector_size__(4 * sizeof(unsigned)))) unsigned u32x4;
+
+template <bool KROWS>  // KROWS: the operand is stored [rows][K]; else [K][rows]
+struct TileLoad {
+    u32x4 v[2];
+    // rows r0 .. r0+127 of the operand, k0 .. k0+31
+    __device__ __forceinline__ void load(const uint16_t* p, int ld, int rows, int r0, int k0, int tid) {
+        if (KROWS) {
+            const int r = tid >> 1, kc = (tid & 1) * 16;
+            if (r0 + r < rows) {
+                const u32x4* q = reinterpret_cast<const u32x4*>(p + (size_t)(r0 + r) * ld + k0 + kc);
+                v[0] = q[0];
+                v[1] = q[1];
+            } else {
+                v[0] = v[1] = u32x4{0u, 0u, 0u, 0u};
+            }
+        } else {
+            const int kr = tid >> 3, rc = (tid & 7) * 16;
+            const uint16_t* row = p + (size_t)(k0 + kr) * ld + r0 + rc;
+            if (r0 + rc + 16 <= rows) {
+                const u32x4* q = reinterpret_cast<const u32x4*>(row);
+                v[0] = q[0];
+                v[1] = q[1];
+            } else {
+                uint16_t t[16];
+#pragma unroll
+                for (int j = 0; j < 16; j++) t[j] = r0 + rc + j < rows ? row[j] : 0;
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    v[0][j] = (unsigned)t[2 * j] | ((unsigned)t[2 * j + 1] << 16);
+                    v[1][j] = (unsigned)t[8 + 2 * j] | ((unsigned)t[8 + 2 * j + 1] << 16);
+                }
+            }
+        }
+    }
+    __device__ __forceinline__ void store(uint16_t (*S)[LDK], int tid) const {
+        if (KROWS) {
+            const int r = tid >> 1, kc = (tid & 1) * 16;
+            u32x4* q = reinterpret_cast<u32x4*>(&S[r][kc]);
+            q[0] = v[0];
+            q[1] = v[1];
+        } else {
+            const int kr = tid >> 3, rc = (tid & 7) * 16;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                S[rc + 2 * j][kr] = (uint16_t)(v[0][j] & 0xffffu);
+                S[rc + 2 * j + 1][kr] = (uint16_t)(v[0][j] >> 16);
+                S[rc + 8 + 2 * j][kr] = (uint16_t)(v[1][j] & 0xffffu);
+                S[rc + 8 + 2 * j + 1][kr] = (uint16_t)(v[1][j] >> 16);
+            }
+        }
+    }
+};
+
+template <bool TA, bool TB>
+__global__ void __launch_bounds__(256) sg_gemm_big(GemmArgs g) {
+    __shared__ __attribute__((aligned(16))) uint16_t As[GB_M][LDK];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[GB_N][LDK];  // B staged as [n][k]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m0 = blockIdx.y * GB_M, n0 = blockIdx.x * GB_N;
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    f32x4 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+    TileLoad<!TA> la;
+    TileLoad<TB> lb;
+    la.load(g.A, g.lda, g.M, m0, 0, tid);
+    lb.load(g.B, g.ldb, g.N, n0, 0, tid);
+    const int fr = lane & 15, fk = (lane >> 4) * 8;
+    for (int k0 = 0; k0 < g.K; k0 += BK) {
+        la.store(As, tid);
+        lb.store(Bs, tid);
+        __syncthreads();
+        if (k0 + BK < g.K) {  // next k-slice in flight during the MFMAs
+            la.load(g.A, g.lda, g.M, m0, k0 + BK, tid);
+            lb.load(g.B, g.ldb, g.N, n0, k0 + BK, tid);
+        }
+        bf16x8 af[4], bfm[4];
+#pragma unroll
+        for (int a = 0; a < 4; a++) af[a] = *reinterpret_cast<const bf16x8*>(&As[wm + a * 16 + fr][fk]);
+#pragma unroll
+        for (int b = 0; b < 4; b++) bfm[b] = *reinterpret_cast<const bf16x8*>(&Bs[wn + b * 16 + fr][fk]);
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int b = 0; b < 4; b++)
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bfm[b], acc[a][b], 0, 0, 0);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int m = m0 + wm + a * 16 + (lane >> 4) * 4 + i;
+                const int n = n0 + wn + b * 16 + (lane & 15);
+                if (m >= g.M || n >= g.N) continue;
+                float v = acc[a][b][i];
+                if (g.bias) v += g.bias[n];
+                const size_t o = (size_t)m * g.ldc + n;
+                switch (g.mode) {
+                    case FZ_SG_STORE32: g.C32[o] = v; break;
+                    case FZ_SG_ACC32: g.C32[o] += v; break;
+                    case FZ_SG_STORE16: g.C16[o] = f2bf(v); break;
+                    default:
+                        g.C32[o] = v;
+                        g.C16[o] = f2bf(gelu(v));
+                }
+            }
+}
+
 // dpre = dact * gelu'(pre); written as FP32 and as BF16 (the next GEMM operand)
 __global__ void sg_gelu_bwd_k(long long n, const float* dact, const float* pre, float* dpre32, uint16_t* dpre16) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -254,8 +376,19 @@ int fz_sg_gemm(int M, int N, int K, const void* A, int lda, int trans_a, const v
     if (mode < FZ_SG_STORE32 || mode > FZ_SG_GELU) return fzi_fail(FZ_E_ARG, "gemm: unknown mode %d", mode);
     GemmArgs g{M, N, K, static_cast<const uint16_t*>(A), static_cast<const uint16_t*>(B), lda, ldb, ldc, mode, C32,
                static_cast<uint16_t*>(C16), bias};
-    const dim3 grid((N + BN - 1) / BN, (M + BM - 1) / BM);
     hipStream_t s = static_cast<hipStream_t>(stream);
+    const bool big = M >= GB_M && N >= GB_N && lda % 8 == 0 && ldb % 8 == 0 &&
+                     (reinterpret_cast<uintptr_t>(A) & 15) == 0 && (reinterpret_cast<uintptr_t>(B) & 15) == 0;
+    if (big) {
+        const dim3 gridb((N + GB_N - 1) / GB_N, (M + GB_M - 1) / GB_M);
+        if (!trans_a && !trans_b) hipLaunchKernelGGL((sg_gemm_big<false, false>), gridb, dim3(256), 0, s, g);
+        else if (!trans_a && trans_b) hipLaunchKernelGGL((sg_gemm_big<false, true>), gridb, dim3(256), 0, s, g);
+        else if (trans_a && !trans_b) hipLaunchKernelGGL((sg_gemm_big<true, false>), gridb, dim3(256), 0, s, g);
+        else hipLaunchKernelGGL((sg_gemm_big<true, true>), gridb, dim3(256), 0, s, g);
+        if (hipGetLastError() != hipSuccess) return fzi_fail(FZ_E_HIP, "sg_gemm launch failed");
+        return FZ_OK;
+    }
+    const dim3 grid((N + BN - 1) / BN, (M + BM - 1) / BM);
     if (!trans_a && !trans_b) hipLaunchKernelGGL((sg_gemm<false, false>), grid, dim3(256), 0, s, g);
     else if (!trans_a && trans_b) hipLaunchKernelGGL((sg_gemm<false, true>), grid, dim3(256), 0, s, g);
     else if (trans_a && !trans_b) hipLaunchKernelGGL((sg_gemm<true, false>), grid, dim3(256), 0, s, g);

@@ -245,7 +245,8 @@ class Ensemble:
 
     def predict(self, X16: torch.Tensor):
         """One frame for B sequences: X16 [B, n_in] bf16.  Returns (mean rates [B, n_out] in Hz,
-        relative disagreement [B]: max over groups of (max - min) / (|mean| + 1 Hz))."""
+        relative disagreement [B]: the members' standard deviation over the rate vector relative
+        to the mean rate vector, ||std|| / (||mean|| + 1 Hz))."""
         ys = []
         for k, m in enumerate(self.members):
             y, s = m.forward(X16.unsqueeze(0), self.state[k])
@@ -253,5 +254,6 @@ class Ensemble:
             ys.append(y[0] * m.rate_scale)
         Y = torch.stack(ys)
         mean = Y.mean(0)
-        spread = (Y.max(0).values - Y.min(0).values) / (mean.abs() + 1.0)
-        return mean, spread.max(1).values
+        std = Y.std(0, unbiased=False) if len(ys) > 1 else torch.zeros_like(mean)
+        spread = std.norm(dim=1) / (mean.norm(dim=1) + 1.0)
+        return mean, spread

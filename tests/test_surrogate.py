@@ -61,14 +61,16 @@ def _bf(t):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
-def test_mfma_gemm_matches_numpy(ta, tb):
+@pytest.mark.parametrize("M,N,K", [(70, 45, 96), (200, 136, 96), (256, 384, 64), (136, 130, 32)])
+def test_mfma_gemm_matches_numpy(ta, tb, M, N, K):
+    # 64 x 64 tiles for small or unaligned operands, 128 x 128 tiles from M, N >= 128 with
+    # leading dimensions a multiple of 8 (N = 130 is neither for the [K][N] layouts)
     import torch
 
     from flyhip import _lib
 
     lib = _lib.load()
     rng = np.random.default_rng(2)
-    M, N, K = 70, 45, 96                                  # ragged M, N; K a multiple of 32
     A = bf16(rng.normal(0, 1, (M, K)))
     Bm = bf16(rng.normal(0, 1, (K, N)))
     bias = rng.normal(0, 1, N).astype(np.float32)
