@@ -524,7 +524,8 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
     const int TS = TPB * NPT;
     // acc[neuron][KP]: a neuron's K arrivals are adjacent (two steps per ds_read2 with immediate
     // offsets in the step loop); KP = K | 1 is odd, so the 64 lanes of a wavefront hit 64
-    // different banks
+    // different banks.  Offsets are 24-bit products (neuron < 8192, KP <= 33): full-rate
+    // multiplies.
     const int KP = a.KP;
     int* acc = lds;                      // [TS][KP]
     int* st_seg = acc + KP * TS;         // [TPB * IPT]
@@ -549,6 +550,27 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
                 if (i < a.N) a.counts[bn + i] = 0;
             }
         return;
+    }
+    // the tile's neuron state, loaded first so that its latency overlaps the pull
+    float u_[NPT], g_[NPT], bias_[NPT], pp_[NPT];
+    int r_[NPT], j_[NPT];
+#pragma unroll
+    for (int p = 0; p < NPT; p++) {
+        const int i = i0 + p * TPB + tid;
+        u_[p] = g_[p] = bias_[p] = pp_[p] = 0.f;
+        r_[p] = 0;
+        j_[p] = -1;
+        if (i < a.N) {
+            u_[p] = a.u[bn + i];
+            g_[p] = a.g[bn + i];
+            r_[p] = a.ref[bn + i];
+            j_[p] = a.ext_idx ? a.ext_idx[i] : -1;
+            if (j_[p] >= 0) {
+                const size_t e = (size_t)b * a.n_ext + j_[p];
+                if (a.bias_ext) bias_[p] = a.bias_ext[e];
+                if (a.p_ext) pp_[p] = a.p_ext[e];
+            }
+        }
     }
     for (int q = tid; q < KP * TS; q += TPB) acc[q] = 0;
     const int2* bkp = nullptr;  // this tile's bucket: entry e at bkp[e] (8 bytes) or bkp4[e]
@@ -627,7 +649,7 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
                 const int slot = x[r].y & 31;
 #pragma unroll
                 for (int j = 0; j < BK_PIECE; j++)
-                    if (val[r][j] != 0) atomicAdd(&acc[loc[r][j] * KP + slot], val[r][j]);
+                    if (val[r][j] != 0) atomicAdd(&acc[__umul24(loc[r][j], KP) + slot], val[r][j]);
             }
         }
         __syncthreads();
@@ -704,7 +726,7 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
                 }
 #pragma unroll
                 for (int u = 0; u < UNROLL; u++)
-                    if (val[u] != 0) atomicAdd(&acc[loc[u] * KP + slot[u]], val[u]);
+                    if (val[u] != 0) atomicAdd(&acc[__umul24(loc[u], KP) + slot[u]], val[u]);
             }
             __syncthreads();
         }
@@ -717,7 +739,7 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
             int* r = a.ring + ((size_t)b * a.ring_slots + (rs0 + k) % a.ring_slots) * a.N + i;
             const int v = *r;
             if (v != 0) {
-                atomicAdd(&acc[(i - i0) * KP + k], v);
+                atomicAdd(&acc[__umul24(i - i0, KP) + k], v);
                 *r = 0;
             }
         }
@@ -736,26 +758,11 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
     for (int q = tid; q < TS; q += TPB) hm[q] = 0;
     if (tid < a.K) s_sk[tid] = mix32((step0 + (uint32_t)tid) * 0x9e3779b9U ^ hseed);
     __syncthreads();
-    float u_[NPT], g_[NPT], bias_[NPT];
-    int r_[NPT];
 #pragma unroll
     for (int p = 0; p < NPT; p++) {
-        const int li = p * TPB + tid, i = i0 + li;
-        u_[p] = g_[p] = bias_[p] = 0.f;
-        r_[p] = 0;
-        float pp = 0.f;
-        int j = -1;
-        if (i < a.N) {
-            u_[p] = a.u[bn + i];
-            g_[p] = a.g[bn + i];
-            r_[p] = a.ref[bn + i];
-            j = a.ext_idx ? a.ext_idx[i] : -1;
-            if (j >= 0) {
-                const size_t e = (size_t)b * a.n_ext + j;
-                if (a.bias_ext) bias_[p] = a.bias_ext[e];
-                if (a.p_ext) pp = a.p_ext[e];
-            }
-        }
+        const int li = p * TPB + tid;
+        const float pp = pp_[p];
+        const int j = j_[p];
         const bool want = j >= 0 && (a.hits || pp > 0.f);
         const unsigned long long wm = __ballot(want);
         const int ne = __popcll(wm);

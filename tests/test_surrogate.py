@@ -178,3 +178,24 @@ def test_surrogate_pipeline_end_to_end_small():
     for k in ("err_surrogate", "err_with_fallback", "fallback_frac", "err_single_sim_trial", "err_constant_predictor"):
         assert np.isfinite(res[k]), (k, res)
     assert 0.0 <= res["fallback_frac"] <= 1.0
+
+
+@pytest.mark.gpu
+def test_trained_surrogate_beats_the_method_without_it():
+    # what the network gains: on held-out sequences, its DN-rate error against the simulation's
+    # expected rate (replica mean) is well below that of the best constant prediction (the
+    # training mean rate), the method without the network
+    from flyhip.connectome import flywire_like
+    from flyhip.surrogate import Ensemble
+    from flyhip.surrogate_pipeline import generate, generate_judged, judge, train_ensemble
+
+    fly = flywire_like(n=8000, n_edges=8000 * 109, seed=11)
+    X, Y, _ = generate(fly, frames=400, batch=64, seed=1, replicas=4, skip=5)
+    X = X.view(400, 16, 4, -1)[:, :, 0].contiguous()
+    Y = Y.view(400, 16, 4, 8).mean(2).contiguous()
+    members, _ = train_ensemble(X, Y, members=2, T=16, seqs=64, steps=500, lr=1e-3, rate_scale=2.0)
+    Xt, ref, trial, others, _ = generate_judged(fly, frames=60, sequences=8, replicas=8, batch=64, seed=9, skip=5)
+    res = judge(Ensemble(members), Xt, ref, trial, others, replicas=8, const=Y.mean((0, 1)))
+    print(res)
+    assert res["mean_rate_hz"] > 0
+    assert res["err_surrogate"] < 0.5 * res["err_constant_predictor"], res
