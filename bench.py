@@ -129,6 +129,7 @@ def measure_line(fly, p, B, engine, tile, frames, warmup, torch, gain=None, driv
         res["roofline"] = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": round(achieved / HBM_PEAK_GBS, 4),
                            "traffic": tr["bytes_per_launch"] if tr else None,
+                           "traffic_gbs": round(tr["bytes_per_launch"] / (avg_ms / 1000.0) / 1e9, 1) if tr else None,
                            "traffic_source": (f"rocprofv3 FETCH_SIZE x {tr['fetch_factor']} + WRITE_SIZE x "
                                               f"{tr['write_factor']} (copy-kernel scaled), {PROFILE[len(REPO) + 1:]}")
                            if tr else "no counter run for this launch shape", "kernel": dom,
@@ -139,6 +140,23 @@ def measure_line(fly, p, B, engine, tile, frames, warmup, torch, gain=None, driv
     del bb, probs
     torch.cuda.empty_cache()
     return res
+
+
+def copy_bandwidth(torch, nbytes=1 << 30, reps=5):
+    """A STREAM-like device-to-device copy (bytes read + written per second), for context."""
+    x = torch.ones(nbytes // 4, dtype=torch.int32, device="cuda")
+    y = torch.empty_like(x)
+    y.copy_(x)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        y.copy_(x)
+    e1.record()
+    torch.cuda.synchronize()
+    gbs = 2 * nbytes * reps / (e0.elapsed_time(e1) / 1000.0) / 1e9
+    del x, y
+    torch.cuda.empty_cache()
+    return gbs
 
 
 def plastic_line(fly, p, torch, B=64, frames=300, warmup=3):
@@ -289,6 +307,9 @@ def main():
         lines1.append(r)
     best = max(lines1, key=lambda r: r["value"])
     value, ms_per_frame = best["value"], best["ms_per_frame"]
+    copy_gbs = copy_bandwidth(torch)
+    best["roofline"]["copy_gbs_measured"] = round(copy_gbs, 1)
+    log(f"1 GiB device copy: {copy_gbs:.0f} GB/s (read + write)")
 
     # ---- B = 256 (batched line) -------------------------------------------------------------
     batched = None
@@ -334,12 +355,23 @@ def main():
             ob.run(FRAME_MS)
         cw = time.perf_counter() - t0
         cv = nf * steps_per_frame * p.dt / 1000.0 / cw
+        # the same restatement with its OpenMP sweep on all host threads (the faster CPU path)
+        nthr = min(os.cpu_count() or 1, 16)
+        ob2 = OracleBrain(fly.weights, p, batch=1, seed=0)
+        ob2.set_input(idx, rr[0, 0])
+        ob2.run(FRAME_MS, nthreads=nthr)
+        t0 = time.perf_counter()
+        for f in range(nf):
+            ob2.set_input(idx, rr[(f + 1) % 64, 0])
+            ob2.run(FRAME_MS, nthreads=nthr)
+        cv2 = nf * steps_per_frame * p.dt / 1000.0 / (time.perf_counter() - t0)
         cpu = {"value": round(cv, 5), "unit": "fly-sim-seconds/wall-second", "cores": 1, "kind": "port",
                "sample": f"{nf} game frames ({nf * steps_per_frame} LIF steps) of the same B=1 workload, "
                          f"C oracle -O2 single thread, {platform.processor() or platform.machine()}, "
                          f"{os.cpu_count()} logical CPUs on host",
-               "gpu_over_cpu": round(value / cv, 1)}
-        log(f"CPU oracle: {cv:.4f} fly-sim-s/s ({cw / nf * 1000:.1f} ms/frame)")
+               "gpu_over_cpu": round(value / cv, 1),
+               "openmp": {"value": round(cv2, 5), "cores": nthr, "gpu_over_cpu": round(value / cv2, 1)}}
+        log(f"CPU oracle: {cv:.4f} fly-sim-s/s ({cw / nf * 1000:.1f} ms/frame); {nthr} threads: {cv2:.4f}")
 
     line = {"metric": METRIC, "value": round(value, 4), "unit": "fly-sim-seconds/wall-second", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_frame, 4),

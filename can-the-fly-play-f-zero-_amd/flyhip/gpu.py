@@ -169,7 +169,7 @@ class BatchBrain:
                                 self.ring.data_ptr(), self.counts.data_ptr(), self.clock.data_ptr(),
                                 self.ring_slots, _lib.ptr(self.spk_idx), _lib.ptr(self.spk_cnt))
         _lib.check(lib.fz_bind(h, ctypes.byref(self._st)), "fz_bind")
-        self.tune = _lib.FzTune(0, 0, 0, 0, 0, 0)
+        self.tune = _lib.FzTune()  # all zero: engine defaults
         self._graphs: dict = {}
 
     # ------------------------------------------------------------------ clock

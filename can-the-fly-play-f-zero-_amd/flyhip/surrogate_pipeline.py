@@ -229,7 +229,16 @@ def main(argv=None):
     ap.add_argument("--threshold", type=float, default=0.15)
     ap.add_argument("--rate-scale", type=float, default=2.0, help="Hz per unit of the network's output")
     ap.add_argument("--n", type=int, default=0, help="neurons (0: the full 138,639)")
+    ap.add_argument("--inference-only", action="store_true",
+                    help="time ensemble inference at B = 4,096 with freshly initialised members (for profiling)")
     args = ap.parse_args(argv)
+    if args.inference_only:
+        from .surrogate import Ensemble, RateSurrogate
+
+        ens = Ensemble([RateSurrogate(seed=k) for k in range(args.members)])
+        fps = inference_fps(ens)
+        print(json.dumps({"members": args.members, "batch": 4096, "surrogate_frames_per_s": round(fps)}), flush=True)
+        return {"surrogate_frames_per_s": fps}
 
     def log(*a):
         print(*a, file=sys.stderr, flush=True)
