@@ -462,8 +462,8 @@ struct TileArgs {
     int* spk_cnt;          // [B][M]
     const uint16_t* tb16;  // segment table as offsets from the row start, or null
     const int* indptr;
-    int bk4;               // bucket entries are 4 bytes (first edge << 7 | (len - 1) << 5 | step)
-    const int2* bk;        // tile buckets (source 1): [B][bk_stride] entries
+    const int* bk;         // tile buckets (source 1): [2][B][bk_stride] entries (first edge << 7 |
+                           // (len - 1) << 5 | step), two buffers alternating with the block parity
     const long long* bk_off;  // [n_tiles + 1] offset of each tile's bucket in a fly's arena
     long long bk_stride;
     const int* bk_cnt;     // [B][n_tiles] entries in each bucket
@@ -573,17 +573,14 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
         }
     }
     for (int q = tid; q < KP * TS; q += TPB) acc[q] = 0;
-    const int2* bkp = nullptr;  // this tile's bucket: entry e at bkp[e] (8 bytes) or bkp4[e]
-    const int* bkp4 = nullptr;
+    const int* bkp4 = nullptr;  // this tile's bucket: entry e at bkp4[e]
     int total;
     if constexpr (BK) {
         const int par = (int)((a.clock[2] + a.blk) & 1);
         const long long cap = a.bk_off[T + 1] - a.bk_off[T];
         const int c = a.bk_cnt[((size_t)par * a.B + b) * a.n_tiles + T];
         total = c < cap ? c : (int)cap;
-        const size_t first = ((size_t)par * a.B + b) * a.bk_stride + a.bk_off[T];  // entry index, either width
-        bkp = a.bk + first;
-        bkp4 = reinterpret_cast<const int*>(a.bk) + first;
+        bkp4 = a.bk + ((size_t)par * a.B + b) * a.bk_stride + a.bk_off[T];
         __syncthreads();
     } else {
         if (tid < a.K) {
@@ -618,35 +615,37 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
     if constexpr (BK) {
         // Bucket pieces hold at most BK_PIECE edges (tile segments are short: ~1 edge per
         // (row, tile) on a FlyWire-shaped graph), so each thread takes whole pieces: no scan
-        // and no search, BK_PIECE x 2 edge loads in flight per thread.
+        // and no search.  The two entry loads and the first edge of each piece (most pieces
+        // are one edge) are issued unconditionally from clamped, valid indices and masked
+        // afterwards, so they are in flight together; the rare further edges are loaded on
+        // demand.
         for (int e0 = tid; e0 < total; e0 += 2 * TPB) {
-            int2 x[2];
+            int v[2];
+            bool ok[2];
 #pragma unroll
             for (int r = 0; r < 2; r++) {
                 const int e = e0 + r * TPB;
-                x[r] = make_int2(0, 0);
-                if (e < total) {
-                    if (a.bk4) {  // first edge << 7 | (len - 1) << 5 | step
-                        const int v = bkp4[e];
-                        x[r] = make_int2(v >> 7, ((((v >> 5) & 3) + 1) << 5) | (v & 31));
-                    } else {
-                        x[r] = bkp[e];
-                    }
-                }
+                ok[r] = e < total;
+                v[r] = bkp4[ok[r] ? e : e0];  // first edge << 7 | (len - 1) << 5 | step
             }
             int loc[2][BK_PIECE], val[2][BK_PIECE];
 #pragma unroll
             for (int r = 0; r < 2; r++) {
-                const int len = x[r].y >> 5;
+                edge(v[r] >> 7, &loc[r][0], &val[r][0]);
+                if (!ok[r]) val[r][0] = 0;
+            }
 #pragma unroll
-                for (int j = 0; j < BK_PIECE; j++) {
+            for (int r = 0; r < 2; r++) {
+                const int seg = v[r] >> 7, len = ok[r] ? ((v[r] >> 5) & 3) + 1 : 0;
+#pragma unroll
+                for (int j = 1; j < BK_PIECE; j++) {
                     loc[r][j] = val[r][j] = 0;
-                    if (j < len) edge(x[r].x + j, &loc[r][j], &val[r][j]);
+                    if (j < len) edge(seg + j, &loc[r][j], &val[r][j]);
                 }
             }
 #pragma unroll
             for (int r = 0; r < 2; r++) {
-                const int slot = x[r].y & 31;
+                const int slot = v[r] & 31;
 #pragma unroll
                 for (int j = 0; j < BK_PIECE; j++)
                     if (val[r][j] != 0) atomicAdd(&acc[__umul24(loc[r][j], KP) + slot], val[r][j]);
@@ -661,33 +660,45 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
             int seg[IPT], len[IPT], kk[IPT];
             int sum = 0;
             int ids[IPT];
+            // loads from clamped, valid positions (results masked afterwards) so that the IPT
+            // list reads, then the IPT table reads, of a thread are in flight together
+            bool ok[IPT];
+            size_t li_[IPT];
 #pragma unroll
-            for (int r = 0; r < IPT; r++) {
+            for (int r = 0; r < IPT; r++) {  // list positions first (LDS only) ...
                 const int e = base + tid * IPT + r;
+                ok[r] = e < total;
+                const int ec = ok[r] ? e : base;
                 kk[r] = 0;
-                ids[r] = -1;
-                if (e < total) {
-                    while (s_off[kk[r] + 1] <= e) kk[r]++;
-                    ids[r] = idx_b[(size_t)((lt0 + kk[r] - nd + M) % M) * a.N + (e - s_off[kk[r]])];
+                while (s_off[kk[r] + 1] <= ec) kk[r]++;
+                li_[r] = (size_t)((lt0 + kk[r] - nd + M) % M) * a.N + (ec - s_off[kk[r]]);
+            }
+#pragma unroll
+            for (int r = 0; r < IPT; r++) ids[r] = idx_b[li_[r]];  // ... then all list loads at once
+            const unsigned nt1 = (unsigned)(a.n_tiles + 1);
+            if (a.tb16) {
+#pragma unroll
+                for (int r = 0; r < IPT; r++) {
+                    ok[r] = ok[r] && (unsigned)ids[r] < (unsigned)a.N;
+                    const int idc = ok[r] ? ids[r] : 0;
+                    const unsigned at = __umul24((unsigned)idc, nt1) + (unsigned)T;  // < 2^31 (checked)
+                    const int r0 = a.indptr[idc];
+                    const int l16 = a.tb16[at], h16 = a.tb16[at + 1];
+                    seg[r] = r0 + l16;
+                    len[r] = (h16 - l16) & -(int)ok[r];
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < IPT; r++) {
+                    ok[r] = ok[r] && (unsigned)ids[r] < (unsigned)a.N;
+                    const int idc = ok[r] ? ids[r] : 0;
+                    const unsigned at = __umul24((unsigned)idc, nt1) + (unsigned)T;
+                    seg[r] = a.tb[at];
+                    len[r] = (a.tb[at + 1] - seg[r]) & -(int)ok[r];  // masked, not branched on
                 }
             }
 #pragma unroll
-            for (int r = 0; r < IPT; r++) {
-                seg[r] = 0;
-                len[r] = 0;
-                if ((unsigned)ids[r] < (unsigned)a.N) {
-                    if (a.tb16) {
-                        const uint16_t* tr = a.tb16 + (size_t)ids[r] * (a.n_tiles + 1) + T;
-                        seg[r] = a.indptr[ids[r]] + tr[0];
-                        len[r] = (int)tr[1] - (int)tr[0];
-                    } else {
-                        const int* tr = a.tb + (size_t)ids[r] * (a.n_tiles + 1) + T;
-                        seg[r] = tr[0];
-                        len[r] = tr[1] - seg[r];
-                    }
-                }
-                sum += len[r];
-            }
+            for (int r = 0; r < IPT; r++) sum += len[r];
             int tot;
             int run = block_incl_scan(sum, s_wsum, &tot) - sum;
 #pragma unroll
@@ -719,10 +730,9 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
                 }
                 int loc[UNROLL], val[UNROLL];
 #pragma unroll
-                for (int u = 0; u < UNROLL; u++) {
-                    loc[u] = 0;
-                    val[u] = 0;
-                    if (ek[u] >= 0) edge(ek[u], &loc[u], &val[u]);
+                for (int u = 0; u < UNROLL; u++) {  // all UNROLL edge loads in flight
+                    edge(ek[u] >= 0 ? ek[u] : ek[0], &loc[u], &val[u]);
+                    if (ek[u] < 0) val[u] = 0;
                 }
 #pragma unroll
                 for (int u = 0; u < UNROLL; u++)
@@ -895,15 +905,16 @@ struct BucketArgs {
     const int* tb;
     const uint16_t* tb16;
     const int* indptr;
-    int bk4;
     const long long* bk_off;
     long long bk_stride;
-    int2* bk;
+    int* bk;
     int* bk_cnt;
     const unsigned char* alive;
     int* err;
 };
 
+// T16: read the uint16 copy of the segment table (offsets from the row start).
+template <bool T16>
 __global__ void __launch_bounds__(MAX_THREADS) tile_bucket(BucketArgs a) {
     __shared__ int s_off[KMAX + 1];
     __shared__ int s_cnt[64];
@@ -950,27 +961,30 @@ __global__ void __launch_bounds__(MAX_THREADS) tile_bucket(BucketArgs a) {
         }
         __syncthreads();
         for (int q0 = wave * BK_UNROLL; q0 < nc; q0 += W * BK_UNROLL) {
+            // all BK_UNROLL table reads in flight at once: every lane loads from a clamped,
+            // valid row and tile, and invalid results are masked afterwards
             int lo[BK_UNROLL], hi[BK_UNROLL];
+            bool ok[BK_UNROLL];
+            const int Tc = own ? T : 0;
 #pragma unroll
             for (int u = 0; u < BK_UNROLL; u++) {
-                lo[u] = hi[u] = 0;
                 const int q = q0 + u;
-                if (own && q < nc) {
-                    const int id = s_id[q];
-                    if ((unsigned)id < (unsigned)a.N) {
-                        if (a.tb16) {  // half the bytes of the int32 table
-                            const uint16_t* tr = a.tb16 + (size_t)id * nt1 + T;
-                            const int r0 = a.indptr[id];
-                            lo[u] = r0 + tr[0];
-                            hi[u] = r0 + tr[1];
-                        } else {
-                            const int* tr = a.tb + (size_t)id * nt1 + T;
-                            lo[u] = tr[0];
-                            hi[u] = tr[1];
-                        }
-                    }
+                const int id = s_id[q < nc ? q : q0];
+                ok[u] = own && q < nc && (unsigned)id < (unsigned)a.N;
+                const int idc = (unsigned)id < (unsigned)a.N ? id : 0;
+                const unsigned at = __umul24((unsigned)idc, (unsigned)nt1) + (unsigned)Tc;  // < 2^31 (checked)
+                if constexpr (T16) {
+                    const int r0 = a.indptr[idc];
+                    lo[u] = r0 + a.tb16[at];
+                    hi[u] = r0 + a.tb16[at + 1];
+                } else {
+                    lo[u] = a.tb[at];
+                    hi[u] = a.tb[at + 1];
                 }
             }
+#pragma unroll
+            for (int u = 0; u < BK_UNROLL; u++)
+                if (!ok[u]) lo[u] = hi[u] = 0;
 #pragma unroll
             for (int u = 0; u < BK_UNROLL; u++)
                 if (hi[u] > lo[u]) {
@@ -980,11 +994,7 @@ __global__ void __launch_bounds__(MAX_THREADS) tile_bucket(BucketArgs a) {
                     for (int c = 0; c < np; c++) {
                         const int l = len - c * BK_PIECE < BK_PIECE ? len - c * BK_PIECE : BK_PIECE;
                         if (pos + c >= cap) break;
-                        if (a.bk4)
-                            reinterpret_cast<int*>(a.bk)[base + pos + c] =
-                                ((lo[u] + c * BK_PIECE) << 7) | ((l - 1) << 5) | k;
-                        else
-                            a.bk[base + pos + c] = make_int2(lo[u] + c * BK_PIECE, (l << 5) | k);
+                        a.bk[base + pos + c] = ((lo[u] + c * BK_PIECE) << 7) | ((l - 1) << 5) | k;
                     }
                 }
         }
@@ -1056,7 +1066,7 @@ struct fz_engine {
     std::vector<long long> bk_off_h;
     long long* bk_off = nullptr;
     long long bk_stride = 0;
-    int2* bk = nullptr;
+    int* bk = nullptr;
     int* bk_cnt = nullptr;
     int bk_B = 0;
     const int* tb = nullptr;
@@ -1086,6 +1096,9 @@ int resolve(const fz_engine* e, const fz_tune* t, Shape* s) {
     s->src = t ? t->tile_source : 0;
     if (s->src != 0 && s->src != 1) return fail(FZ_E_ARG, "tile_source %d: must be 0 or 1", s->src);
     if (s->src == 1 && (e->mode != 2 || !e->bk_off)) return fail(FZ_E_ARG, "tile_source 1 needs the tile engine");
+    if (s->src == 1 && !e->bk4)
+        return fail(FZ_E_ARG, "tile_source 1 needs fewer than 2^24 fixed edges (bucket entries hold a 24-bit edge "
+                    "index); use tile_source 0");
     if (e->mode == 2) {
         if (!(t && t->upd_threads)) s->ut = e->ts < 256 ? e->ts : 256;
         const int npt = s->ut > 0 ? e->ts / s->ut : 0;
@@ -1219,7 +1232,6 @@ TileArgs make_tile(const fz_engine* e, const fz_inputs* in, int K, int KP, int k
     a.tb = e->tb;
     a.tb16 = e->tb16;
     a.indptr = e->indptr;
-    a.bk4 = e->bk4;
     a.bk = e->bk;
     a.bk_off = e->bk_off;
     a.bk_stride = e->bk_stride;
@@ -1414,9 +1426,9 @@ int ensure_buckets(fz_engine* e) {
     e->bk = nullptr;
     e->bk_cnt = nullptr;
     e->bk_B = 0;
-    // two parity buffers of B x stride entries: 4-byte entries fit both in B x stride int2
-    const size_t ne = (size_t)(e->bk4 ? 1 : 2) * (size_t)e->st.B * (size_t)(e->bk_stride > 0 ? e->bk_stride : 1);
-    if (hipMalloc(&e->bk, sizeof(int2) * ne) != hipSuccess ||
+    // two parity buffers of B x stride 4-byte entries
+    const size_t ne = 2 * (size_t)e->st.B * (size_t)(e->bk_stride > 0 ? e->bk_stride : 1);
+    if (hipMalloc(&e->bk, sizeof(int) * ne) != hipSuccess ||
         hipMalloc(&e->bk_cnt, sizeof(int) * 2 * (size_t)e->st.B * (size_t)e->n_tiles) != hipSuccess) {
         (void)hipGetLastError();
         (void)hipFree(e->bk);
@@ -1444,7 +1456,6 @@ BucketArgs make_bucket(const fz_engine* e, const fz_inputs* in, int K, int kb, i
     a.tb = e->tb;
     a.tb16 = e->tb16;
     a.indptr = e->indptr;
-    a.bk4 = e->bk4;
     a.bk_off = e->bk_off;
     a.bk_stride = e->bk_stride;
     a.bk = e->bk;
@@ -1481,7 +1492,10 @@ int enqueue(fz_engine* e, const fz_inputs* in, int steps, const unsigned* hits, 
             if (s.src == 1) {  // timed with the push timers: it is the spike expansion
                 BucketArgs ba = make_bucket(e, in, K, kb, blk);
                 if ((rc = mark(ev_push))) return rc;
-                hipLaunchKernelGGL(tile_bucket, bgrid, dim3(BK_THREADS), 0, stream, ba);
+                if (e->tb16)
+                    hipLaunchKernelGGL(tile_bucket<true>, bgrid, dim3(BK_THREADS), 0, stream, ba);
+                else
+                    hipLaunchKernelGGL(tile_bucket<false>, bgrid, dim3(BK_THREADS), 0, stream, ba);
                 HIPCHK(hipGetLastError());
                 if ((rc = mark(ev_push))) return rc;
             }
@@ -1701,6 +1715,8 @@ int fz_set_tiles(fz_engine* e, int tile, int n_tiles, const int* tb, const int* 
     if (tile < 64 || tile > 8192 || tile % 64) return fail(FZ_E_ARG, "tile %d: need a multiple of 64 in [64, 8192]", tile);
     if (n_tiles != (e->n + tile - 1) / tile) return fail(FZ_E_ARG, "n_tiles %d != ceil(n / tile)", n_tiles);
     if (!tb) return fail(FZ_E_ARG, "null tile table");
+    if ((long long)e->n * (n_tiles + 1) >= (1ll << 31))
+        return fail(FZ_E_ARG, "segment table of %d x %d entries is too large (limit 2^31)", e->n, n_tiles + 1);
     if (e->n_pl > 0 && (!ptg_ptr || !ptg_idx)) return fail(FZ_E_ARG, "plastic synapses need the per-tile target lists");
     const size_t lds = (size_t)e->kmax * tile * 4 + (size_t)3 * 64 * 4 + (size_t)tile * 4;
     if (lds > e->lds_max - TILE_STATIC_LDS)
@@ -1843,7 +1859,8 @@ int fz_graph_create(fz_engine* e, const fz_inputs* in, int steps, const fz_tune*
                     }
                 if (dep >= blk) dep = blk - 1;
                 hipKernelNodeParams kp{};
-                kp.func = reinterpret_cast<void*>(&tile_bucket);
+                kp.func = e->tb16 ? reinterpret_cast<void*>(&tile_bucket<true>)
+                                  : reinterpret_cast<void*>(&tile_bucket<false>);
                 kp.gridDim = bgrid;
                 kp.blockDim = dim3(BK_THREADS);
                 kp.sharedMemBytes = 0;

@@ -79,9 +79,9 @@ typedef struct fz_tune {
     int tile_stage;    /* tile engine: spike-list entries staged per thread, 1 or 4 */
     int tile_source;   /* tile engine: 0 = every tile looks its segments up in the segment table;
                         * 1 = a prologue kernel per block expands each spike once into per-tile
-                        * buckets of (first edge, length, step) (engine scratch, allocated on
-                        * first use: 8 B x nonempty (row, tile) pairs x spikes per block bound
-                        * per fly) */
+                        * buckets of pieces (first edge, length <= 4, step): engine scratch,
+                        * allocated on first use, 2 x 4 B x pieces bound per fly; needs fewer
+                        * than 2^24 fixed edges */
 } fz_tune;
 
 int fz_version(void);

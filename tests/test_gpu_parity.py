@@ -403,7 +403,8 @@ def test_tile_engine_row_longer_than_65535_edges(source):
 
 
 def test_tile_engine_with_more_than_2_24_edges():
-    # bucket entries cannot pack a first-edge index of 25 bits: 8-byte entries are used
+    # bucket entries hold a 24-bit first-edge index: a larger graph is refused the bucket source
+    # with a clear error and runs exactly on segment-table lookups
     w = _random_graph(60_000, 285, seed=32)
     assert w.nnz >= 1 << 24
     p = LIFParams()
@@ -413,6 +414,9 @@ def test_tile_engine_with_more_than_2_24_edges():
     for b in (gb, ob):
         b.set_input(idx, 40.0)
     gb.set_tune(upd_threads=512, tile_source=1)
+    with pytest.raises(ValueError, match="2\\^24"):
+        gb.run(FRAME_MS, graph=False)
+    gb.set_tune(upd_threads=512, tile_source=0)
     for frame in range(2):
         np.testing.assert_array_equal(_np(gb.run(FRAME_MS)), ob.run(FRAME_MS), err_msg=f"frame {frame}")
     _assert_state_equal(gb, ob)
