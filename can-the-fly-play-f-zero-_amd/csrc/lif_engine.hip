@@ -516,19 +516,19 @@ __device__ __forceinline__ int block_incl_scan(int v, int* wsum, int* total) {
 // in the segment table for every spike of the fly.
 template <typename WT, int NPT, int IPT, bool BK>
 __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
-    extern __shared__ int lds[];
+    extern __shared__ __attribute__((aligned(16))) int lds[];
     __shared__ int s_off[KMAX + 1];
     __shared__ int s_lcnt[KMAX];
     __shared__ int s_wsum[MAX_WAVES + 1];
     const int TPB = blockDim.x, tid = threadIdx.x, lane = tid & 63;
     const int TS = TPB * NPT;
     // acc[neuron][KP]: a neuron's K arrivals are adjacent (two steps per ds_read2 with immediate
-    // offsets in the step loop); KP = K | 1 is odd, so the 64 lanes of a wavefront hit 64
-    // different banks.  Offsets are 24-bit products (neuron < 8192, KP <= 33): full-rate
-    // multiplies.
+    // offsets in the step loop), and column K holds its Poisson hits as a K-bit mask.  KP >= K + 1
+    // is odd when that fits, so the 64 lanes of a wavefront hit 64 different banks.  Offsets are
+    // 24-bit products (neuron < 8192, KP <= 33): full-rate multiplies.
     const int KP = a.KP;
     int* acc = lds;                      // [TS][KP]
-    int* st_seg = acc + KP * TS;         // [TPB * IPT]
+    int* st_seg = acc + KP * TS;         // [TPB * IPT] (segment-table source only)
     int* st_end = st_seg + TPB * IPT;    // [TPB * IPT] inclusive prefix of segment lengths
     int* st_k = st_end + TPB * IPT;      // [TPB * IPT]
     int b, T;
@@ -572,7 +572,12 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
             }
         }
     }
-    for (int q = tid; q < KP * TS; q += TPB) acc[q] = 0;
+    // KP * TS is a multiple of 4 ints (TS is a multiple of 64): 16-byte stores
+    for (int q = tid; q < (KP * TS) >> 2; q += TPB) reinterpret_cast<int4*>(acc)[q] = make_int4(0, 0, 0, 0);
+    // per-step keys of the Poisson draw (read after the next barrier)
+    __shared__ uint32_t s_sk[KMAX];
+    const uint32_t hseed = mix32(a.seed + (uint32_t)b);
+    if (tid < a.K) s_sk[tid] = mix32((step0 + (uint32_t)tid) * 0x9e3779b9U ^ hseed);
     const int* bkp4 = nullptr;  // this tile's bucket: entry e at bkp4[e]
     int total;
     if constexpr (BK) {
@@ -651,7 +656,8 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
                     if (val[r][j] != 0) atomicAdd(&acc[__umul24(loc[r][j], KP) + slot], val[r][j]);
             }
         }
-        __syncthreads();
+        // no barrier: the plastic arrivals and Poisson hits below are LDS atomics too, and one
+        // barrier before the step loop orders them all
     } else {
         // Entries in chunks of TPB * IPT; each thread owns IPT consecutive entries so that all of
         // its list and table loads are in flight together; segments (any length) are staged with
@@ -753,50 +759,44 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
                 *r = 0;
             }
         }
-        __syncthreads();
     }
 
-    // Poisson hits of the tile's input neurons for the K steps, as K-bit masks in LDS.  Input
-    // neurons are scattered, so hashing inside the step loop would run the hash on every lane
-    // of almost every wavefront; instead each wavefront compacts its input neurons (staging
-    // arrays, free after the pull) and spreads their (neuron, step) pairs over its 64 lanes.
-    // The draw is flyzero/gpu.py:66-67: lowbias32 of (neuron, step, seed + fly), or the
-    // precomputed hit bits.
-    int* hm = st_k + TPB * IPT;  // [TS]
-    __shared__ uint32_t s_sk[KMAX];
-    const uint32_t hseed = mix32(a.seed + (uint32_t)b);
-    for (int q = tid; q < TS; q += TPB) hm[q] = 0;
-    if (tid < a.K) s_sk[tid] = mix32((step0 + (uint32_t)tid) * 0x9e3779b9U ^ hseed);
-    __syncthreads();
+    // Poisson hits of the tile's input neurons for the K steps, as K-bit masks in column K of
+    // acc.  Input neurons are scattered, so hashing inside the step loop would run the hash on
+    // every lane of almost every wavefront; instead each wavefront spreads the (neuron, step)
+    // pairs of its input neurons over its 64 lanes, in registers: a lane permutation lists the
+    // input lanes in order (src[q] = lane of the q-th input neuron) and each pair fetches its
+    // neuron's payload from that lane.  No LDS staging and no barrier; the column-K atomics
+    // touch no word the pull adds into.  The draw is flyzero/gpu.py:66-67: lowbias32 of
+    // (neuron, step, seed + fly), or the precomputed hit bits.
 #pragma unroll
     for (int p = 0; p < NPT; p++) {
-        const int li = p * TPB + tid;
-        const float pp = pp_[p];
-        const int j = j_[p];
-        const bool want = j >= 0 && (a.hits || pp > 0.f);
+        const bool want = j_[p] >= 0 && (a.hits || pp_[p] > 0.f);
         const unsigned long long wm = __ballot(want);
-        const int ne = __popcll(wm);
-        const int wbase = (tid >> 6) * 64;
-        if (want) {
-            const int rank = __popcll(wm & ((1ull << lane) - 1ull));
-            st_seg[wbase + rank] = li;
-            st_end[wbase + rank] = j;
-            st_k[wbase + rank] = __float_as_int(pp);
-        }
-        __syncthreads();
+        const int ne = __popcll(wm);  // wavefront-uniform
+        if (ne == 0) continue;
+        const int below = __popcll(wm & ((1ull << lane) - 1ull));
+        // input lanes go to positions 0 .. ne - 1 in lane order, the others after them: a permutation
+        const int src = __builtin_amdgcn_ds_permute((want ? below : ne + lane - below) << 2, lane);
+        const int payload = a.hits ? j_[p] : __float_as_int(pp_[p]);
+        const int lbase = p * TPB + (tid & ~63);
         // pairs x = q * K + k, x = lane, lane + 64, ...: one division, then increments
         int q = lane / a.K, k = lane - q * a.K;
         const int dq = 64 / a.K, dk = 64 - dq * a.K;
-        for (int x = lane; x < ne * a.K; x += 64) {
-            const int lq = st_seg[wbase + q], jq = st_end[wbase + q];
-            bool hit;
-            if (a.hits) {
-                hit = (a.hits[((size_t)(a.kb + k) * a.B + b) * a.hw + (jq >> 5)] >> (jq & 31)) & 1u;
-            } else {
-                const uint32_t h = mix32((uint32_t)(bn + (size_t)(i0 + lq)) ^ s_sk[k]);
-                hit = (float)(h >> 8) * (1.0f / 16777216.0f) < __int_as_float(st_k[wbase + q]);
+        for (int x0 = 0; x0 < ne * a.K; x0 += 64) {  // uniform trip count: every lane permutes
+            const int sl = __builtin_amdgcn_ds_bpermute((q < ne ? q : 0) << 2, src);
+            const int pl = __builtin_amdgcn_ds_bpermute(sl << 2, payload);
+            if (x0 + lane < ne * a.K) {
+                const int lq = lbase + sl;
+                bool hit;
+                if (a.hits) {
+                    hit = (a.hits[((size_t)(a.kb + k) * a.B + b) * a.hw + (pl >> 5)] >> (pl & 31)) & 1u;
+                } else {
+                    const uint32_t h = mix32((uint32_t)(bn + (size_t)(i0 + lq)) ^ s_sk[k]);
+                    hit = (float)(h >> 8) * (1.0f / 16777216.0f) < __int_as_float(pl);
+                }
+                if (hit) atomicOr(&acc[__umul24(lq, KP) + a.K], 1 << k);
             }
-            if (hit) atomicOr(&hm[lq], 1 << k);
             q += dq;
             k += dk;
             if (k >= a.K) {
@@ -804,34 +804,59 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
                 q++;
             }
         }
-        __syncthreads();
     }
+    __syncthreads();
 
     // K steps of flyzero/brain.py:37-53 from LDS, NPT neurons per thread, without branches:
-    // every select keeps the float operations and their order exactly as written there
+    // every select keeps the float operations and their order exactly as written there.
+    // When n_ref >= K (the model's 2.2 ms refractory period against a block of at most
+    // n_delay = 1.8 ms) a neuron fires at most once per block, and the refractory counter is
+    // carried as the block step at which it runs out, rend: the counter before step k is
+    // max(rend - k, 0), so "r > 0" is k < rend, "r <= 0 after the decrement" is k + 1 >= rend,
+    // and a spike at step k sets rend = k + 1 + n_ref (strictly larger than before, so the
+    // spike and its step are read back from rend after the loop, without a per-step mask).
+    const bool once = a.c.n_ref >= a.K;
     uint32_t masks[NPT];
 #pragma unroll
     for (int p = 0; p < NPT; p++) {
         const int li = p * TPB + tid, i = i0 + li;
+        const int* row = acc + __umul24(li, KP);
         uint32_t mask = 0;
-        const uint32_t hits = i < a.N ? (uint32_t)hm[li] : 0u;
         if (i < a.N) {
+            const uint32_t hits = (uint32_t)row[a.K];
             float u = u_[p], g = g_[p];
             int r = r_[p];
             const float bias = bias_[p];
-            for (int k = 0; k < a.K; k++) {
-                const float inj = (float)acc[li * KP + k] * a.c.in_scale;
-                g = g * a.c.decay_syn + inj;
-                const bool refr = r > 0;
-                const float u_int = u + (g + bias - u) * a.c.decay_m;
-                u = refr ? u : u_int;
-                r = refr ? r - 1 : r;
-                const float u_kick = u + a.c.kick;
-                u = ((hits >> k) & 1u) ? u_kick : u;
-                const bool sp = u >= a.c.u_th && r <= 0;
-                u = sp ? a.c.u_reset : u;
-                r = sp ? a.c.n_ref : r;
-                mask |= (uint32_t)sp << k;
+            if (once) {
+                int rend = r;
+                for (int k = 0; k < a.K; k++) {
+                    const float inj = (float)row[k] * a.c.in_scale;
+                    g = g * a.c.decay_syn + inj;
+                    const float u_int = u + (g + bias - u) * a.c.decay_m;
+                    u = k < rend ? u : u_int;
+                    const float u_kick = u + a.c.kick;
+                    u = ((hits >> k) & 1u) ? u_kick : u;
+                    const bool sp = u >= a.c.u_th && k + 1 >= rend;
+                    u = sp ? a.c.u_reset : u;
+                    rend = sp ? k + 1 + a.c.n_ref : rend;
+                }
+                if (rend != r) mask = 1u << (rend - 1 - a.c.n_ref);
+                r = rend - min(max(rend, 0), a.K);  // the counter after the last step
+            } else {
+                for (int k = 0; k < a.K; k++) {
+                    const float inj = (float)row[k] * a.c.in_scale;
+                    g = g * a.c.decay_syn + inj;
+                    const bool refr = r > 0;
+                    const float u_int = u + (g + bias - u) * a.c.decay_m;
+                    u = refr ? u : u_int;
+                    r = refr ? r - 1 : r;
+                    const float u_kick = u + a.c.kick;
+                    u = ((hits >> k) & 1u) ? u_kick : u;
+                    const bool sp = u >= a.c.u_th && r <= 0;
+                    u = sp ? a.c.u_reset : u;
+                    r = sp ? a.c.n_ref : r;
+                    mask |= (uint32_t)sp << k;
+                }
             }
             if (u != u_[p]) a.u[bn + i] = u;
             if (g != g_[p]) a.g[bn + i] = g;
@@ -919,7 +944,6 @@ __global__ void __launch_bounds__(MAX_THREADS) tile_bucket(BucketArgs a) {
     __shared__ int s_off[KMAX + 1];
     __shared__ int s_cnt[64];
     __shared__ int s_id[BK_CHUNK];
-    __shared__ unsigned char s_k[BK_CHUNK];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, W = blockDim.x >> 6;
     const int b = blockIdx.y, T = blockIdx.x * 64 + lane;
     if (a.alive && !a.alive[b]) return;
@@ -944,59 +968,79 @@ __global__ void __launch_bounds__(MAX_THREADS) tile_bucket(BucketArgs a) {
     // buffers alternate with the block parity, so the bucket of block j can be written while
     // the tile kernel of block j - 1 reads the other one
     const int par = (int)((a.clock[2] + a.blk) & 1);
-    long long base = 0, cap = 0;
+    long long cap = 0;
+    int* dst = a.bk;  // this lane's bucket: entry p at dst[p]
     if (own) {
-        base = ((long long)par * a.B + b) * a.bk_stride + a.bk_off[T];
+        dst = a.bk + ((long long)par * a.B + b) * a.bk_stride + a.bk_off[T];
         cap = a.bk_off[T + 1] - a.bk_off[T];
     }
+    const int capi = cap < 0x7fffffff ? (int)cap : 0x7fffffff;
     const int nt1 = a.n_tiles + 1;
+    const int Tc = own ? T : 0;
     for (int c0 = 0; c0 < total; c0 += BK_CHUNK) {
         const int nc = total - c0 < BK_CHUNK ? total - c0 : BK_CHUNK;
-        for (int q = tid; q < nc; q += blockDim.x) {  // stage the chunk's ids and steps
+        for (int q = tid; q < nc; q += blockDim.x) {  // stage the chunk: id << 5 | step, or -1
             const int e = c0 + q;
             int k = 0;
             while (s_off[k + 1] <= e) k++;
-            s_id[q] = idx_b[(size_t)((lt0 + k - nd + M) % M) * a.N + (e - s_off[k])];
-            s_k[q] = (unsigned char)k;
+            const int id = idx_b[(size_t)((lt0 + k - nd + M) % M) * a.N + (e - s_off[k])];
+            s_id[q] = (unsigned)id < (unsigned)a.N ? id << 5 | k : -1;  // id < 2^22 (table < 2^31 entries)
         }
         __syncthreads();
         for (int q0 = wave * BK_UNROLL; q0 < nc; q0 += W * BK_UNROLL) {
-            // all BK_UNROLL table reads in flight at once: every lane loads from a clamped,
-            // valid row and tile, and invalid results are masked afterwards
-            int lo[BK_UNROLL], hi[BK_UNROLL];
-            bool ok[BK_UNROLL];
-            const int Tc = own ? T : 0;
+            // The BK_UNROLL spikes are the same for every lane of the wavefront (lane = tile):
+            // their ids are made scalar, so the row addresses are scalar arithmetic and each
+            // lane's table reads are one scalar base plus its fixed tile offset, all in flight
+            // together.  Past the chunk end a lane rereads a valid spike and masks the result.
+            int lo[BK_UNROLL], len[BK_UNROLL], kk[BK_UNROLL], id[BK_UNROLL];
+            bool use[BK_UNROLL];
 #pragma unroll
             for (int u = 0; u < BK_UNROLL; u++) {
                 const int q = q0 + u;
-                const int id = s_id[q < nc ? q : q0];
-                ok[u] = own && q < nc && (unsigned)id < (unsigned)a.N;
-                const int idc = (unsigned)id < (unsigned)a.N ? id : 0;
-                const unsigned at = __umul24((unsigned)idc, (unsigned)nt1) + (unsigned)Tc;  // < 2^31 (checked)
+                const int v = __builtin_amdgcn_readfirstlane(s_id[q < nc ? q : q0]);
+                use[u] = own && q < nc && v >= 0;
+                id[u] = v >= 0 ? v >> 5 : 0;
+                kk[u] = v & 31;
+            }
+            int l16[BK_UNROLL], h16[BK_UNROLL], r0[BK_UNROLL];
+#pragma unroll
+            for (int u = 0; u < BK_UNROLL; u++) {  // every load first ...
                 if constexpr (T16) {
-                    const int r0 = a.indptr[idc];
-                    lo[u] = r0 + a.tb16[at];
-                    hi[u] = r0 + a.tb16[at + 1];
+                    const uint16_t* row = a.tb16 + (unsigned)id[u] * (unsigned)nt1;  // < 2^31 (checked)
+                    r0[u] = a.indptr[id[u]];
+                    l16[u] = row[Tc];
+                    h16[u] = row[Tc + 1];
                 } else {
-                    lo[u] = a.tb[at];
-                    hi[u] = a.tb[at + 1];
+                    const int* row = a.tb + (unsigned)id[u] * (unsigned)nt1;
+                    r0[u] = 0;
+                    l16[u] = row[Tc];
+                    h16[u] = row[Tc + 1];
                 }
             }
 #pragma unroll
-            for (int u = 0; u < BK_UNROLL; u++)
-                if (!ok[u]) lo[u] = hi[u] = 0;
+            for (int u = 0; u < BK_UNROLL; u++) {  // ... then the arithmetic
+                lo[u] = r0[u] + l16[u];
+                len[u] = use[u] && h16[u] > l16[u] ? h16[u] - l16[u] : 0;
+            }
+            // pieces of at most BK_PIECE edges: one LDS reservation per lane for the batch, then
+            // plain stores (almost every segment is a single piece)
+            unsigned tot = 0;
 #pragma unroll
-            for (int u = 0; u < BK_UNROLL; u++)
-                if (hi[u] > lo[u]) {
-                    const int len = hi[u] - lo[u], np = (len + BK_PIECE - 1) / BK_PIECE;
-                    const int pos = atomicAdd(&s_cnt[lane], np);
-                    const int k = (int)s_k[q0 + u];
-                    for (int c = 0; c < np; c++) {
-                        const int l = len - c * BK_PIECE < BK_PIECE ? len - c * BK_PIECE : BK_PIECE;
-                        if (pos + c >= cap) break;
-                        a.bk[base + pos + c] = ((lo[u] + c * BK_PIECE) << 7) | ((l - 1) << 5) | k;
+            for (int u = 0; u < BK_UNROLL; u++) tot += ((unsigned)len[u] + (BK_PIECE - 1)) / BK_PIECE;
+            if (tot > 0) {
+                int pos = atomicAdd(&s_cnt[lane], (int)tot);
+#pragma unroll
+                for (int u = 0; u < BK_UNROLL; u++)
+                    if (len[u] > 0 && pos < capi) {  // past the capacity: dropped, counted, flagged below
+                        const int l = len[u] < BK_PIECE ? len[u] : BK_PIECE;
+                        dst[pos++] = (lo[u] << 7) | ((l - 1) << 5) | kk[u];
+                        for (int c = BK_PIECE; c < len[u]; c += BK_PIECE) {  // rare: longer segments
+                            const int lc = len[u] - c < BK_PIECE ? len[u] - c : BK_PIECE;
+                            if (pos < capi) dst[pos] = ((lo[u] + c) << 7) | ((lc - 1) << 5) | kk[u];
+                            pos++;
+                        }
                     }
-                }
+            }
         }
         __syncthreads();
     }
@@ -1105,7 +1149,8 @@ int resolve(const fz_engine* e, const fz_tune* t, Shape* s) {
         if (s->ut <= 0 || e->ts % s->ut || (npt != 1 && npt != 2 && npt != 4 && npt != 8))
             return fail(FZ_E_ARG, "tile engine: upd_threads %d must divide the tile (%d neurons) into 1, 2, 4 or 8",
                         s->ut, e->ts);
-        const size_t lds = (size_t)s->K * e->ts * 4 + (size_t)3 * s->ipt * s->ut * 4 + (size_t)e->ts * 4;
+        // K arrival columns and the hit-mask column per neuron, plus the segment-table staging
+        const size_t lds = (size_t)(s->K + 1) * e->ts * 4 + (s->src == 0 ? (size_t)3 * s->ipt * s->ut * 4 : 0);
         if (lds > e->lds_max - TILE_STATIC_LDS)
             return fail(FZ_E_ARG, "tile engine: %zu B of LDS for %d steps x %d neurons exceeds %zu", lds, s->K,
                         e->ts, e->lds_max);
@@ -1274,15 +1319,18 @@ void* tile_kernel(int fmt, int npt, int ipt, int src) {
 }
 int tile_fmt(const fz_engine* e) { return e->pk ? 2 : (e->w_bits == 16 ? 0 : 1); }
 
-// Row stride of the LDS accumulator: K | 1 (odd: no bank conflicts in the step loop) when the
-// extra column fits the LDS budget, else K.
+// LDS staging of the segment-table source (segment, prefix end, step per staged entry).
+size_t tile_staging(const Shape& s) { return s.src == 0 ? (size_t)3 * s.ipt * s.ut * 4 : 0; }
+
+// Row stride of the LDS accumulator: K arrival columns and the hit-mask column, rounded up to
+// odd (no bank conflicts in the step loop) when that fits the LDS budget.
 int tile_kp(const fz_engine* e, const Shape& s, int K) {
-    const size_t rest = (size_t)3 * s.ipt * s.ut * 4 + (size_t)e->ts * 4;
-    return (size_t)(K | 1) * e->ts * 4 + rest <= e->lds_max - TILE_STATIC_LDS ? (K | 1) : K;
+    const int odd = (K + 1) | 1;
+    return (size_t)odd * e->ts * 4 + tile_staging(s) <= e->lds_max - TILE_STATIC_LDS ? odd : K + 1;
 }
 
 size_t tile_lds(const fz_engine* e, const Shape& s, int K) {
-    return (size_t)tile_kp(e, s, K) * e->ts * 4 + (size_t)3 * s.ipt * s.ut * 4 + (size_t)e->ts * 4;  // acc, staging, hit masks
+    return (size_t)tile_kp(e, s, K) * e->ts * 4 + tile_staging(s);
 }
 
 __global__ void tile_seg_count(int n, int n_tiles, const int* tb, int* cnt) {
@@ -1718,7 +1766,8 @@ int fz_set_tiles(fz_engine* e, int tile, int n_tiles, const int* tb, const int* 
     if ((long long)e->n * (n_tiles + 1) >= (1ll << 31))
         return fail(FZ_E_ARG, "segment table of %d x %d entries is too large (limit 2^31)", e->n, n_tiles + 1);
     if (e->n_pl > 0 && (!ptg_ptr || !ptg_idx)) return fail(FZ_E_ARG, "plastic synapses need the per-tile target lists");
-    const size_t lds = (size_t)e->kmax * tile * 4 + (size_t)3 * 64 * 4 + (size_t)tile * 4;
+    // kmax arrival columns and the hit-mask column per neuron, plus the smallest staging
+    const size_t lds = (size_t)(e->kmax + 1) * tile * 4 + (size_t)3 * 64 * 4;
     if (lds > e->lds_max - TILE_STATIC_LDS)
         return fail(FZ_E_ARG, "tile %d needs %zu B of LDS for %d steps; the device allows %zu", tile, lds, e->kmax,
                     e->lds_max - TILE_STATIC_LDS);

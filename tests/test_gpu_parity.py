@@ -354,6 +354,36 @@ def test_tile_sources_give_identical_rasters_and_plastic_arrivals(small_fly, sou
     np.testing.assert_array_equal(_np(gb.g)[:2], ob.g[:2])
 
 
+@pytest.mark.parametrize("t_ref", [0.5, 1.8])
+@pytest.mark.parametrize("source", [0, 1])
+def test_tile_engine_refractory_shorter_than_or_equal_to_a_block(small_fly, t_ref, source):
+    # 18-step blocks: with t_ref = 0.5 ms (5 steps) a neuron can fire several times in one block
+    # (the general step loop); with t_ref = 1.8 ms (18 steps) at most once, at the limit of the
+    # once-per-block loop
+    fly = small_fly
+    p = LIFParams(t_ref=t_ref)
+    gb = BatchBrain(fly.weights, p, batch=2, seed=6, engine="tile", tile=1024)
+    ob = OracleBrain(fly.weights, p, batch=2, seed=6)
+    idx, rate = fly_inputs(fly)
+    for b in (gb, ob):
+        b.set_input(idx, rate)
+    gb.set_tune(upd_threads=512, block_steps=18, tile_source=source)
+    steps = 360
+    cg, rg = gb.run_steps(steps, raster=True)
+    co, ro = ob.run_steps(steps, raster=True)
+    rg = _np(rg).view(np.uint32)
+    first = np.flatnonzero((rg != ro).any(axis=(1, 2)))
+    assert len(first) == 0, f"first divergent step {first[:1]}"
+    np.testing.assert_array_equal(_np(cg), co)
+    _assert_state_equal(gb, ob)
+    bits = np.unpackbits(ro.view(np.uint8), axis=2, bitorder="little")   # steps x flies x neurons
+    per_block = bits.reshape(steps // 18, 18, 2, -1).sum(axis=1)
+    if t_ref == 1.8:
+        assert per_block.max() == 1
+    else:
+        assert per_block.max() >= 2
+
+
 def test_tile_engine_rejects_a_bad_segment_table(small_fly):
     from flyhip import _lib
 
