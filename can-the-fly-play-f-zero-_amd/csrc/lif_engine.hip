@@ -101,6 +101,18 @@ struct WorkList {
     long long ncs;            // chunk-start slots per fly
 };
 
+// t mod m, the slot of step t in a ring of m < 2^16 slots, in 32-bit arithmetic (the 64-bit
+// remainder is a long scalar sequence that every wavefront would run at its start).  Step
+// counts stay below 2^32 for 4.9 days of model time at dt = 0.1 ms; past that the high word is
+// folded in with 2^32 mod m.
+__device__ __forceinline__ int step_slot(long long t, int m) {
+    if (t < 0) return (int)(((t % m) + m) % m);
+    const unsigned mm = (unsigned)m, lo = (unsigned)t, hi = (unsigned)((unsigned long long)t >> 32);
+    if (hi == 0) return (int)(lo % mm);
+    const unsigned p32 = (0xffffffffu % mm + 1u) % mm;  // 2^32 mod m
+    return (int)(((hi % mm) * p32 + lo % mm) % mm);     // < m^2 + m < 2^32
+}
+
 struct UpdArgs {
     const long long* clock;
     int N, B, K, kb, blk, first, chunk;
@@ -143,7 +155,7 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_update(UpdArgs a) {
     const long long t0 = a.clock[0] + a.kb;
     const uint32_t step0 = (uint32_t)a.clock[1] + (uint32_t)a.kb;
     const int nd = a.c.n_delay;
-    const int slot0 = (int)(((t0 % nd) + nd) % nd);
+    const int slot0 = step_slot(t0, nd);
     int* ringb = a.ring + (size_t)b * nd * a.N;
 
     // Issue every ring-slot load of the block up front (K independent loads).
@@ -538,8 +550,8 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
     const long long t0 = a.clock[0] + a.kb;
     const uint32_t step0 = (uint32_t)a.clock[1] + (uint32_t)a.kb;
     const int nd = a.nd, M = a.M;
-    const int lt0 = (int)(((t0 % M) + M) % M);             // list slot of step t0
-    const int rs0 = (int)(((t0 % a.ring_slots) + a.ring_slots) % a.ring_slots);
+    const int lt0 = step_slot(t0, M);                      // list slot of step t0
+    const int rs0 = step_slot(t0, a.ring_slots);
     int* cnt_b = a.spk_cnt + (size_t)b * M;
     int* idx_b = a.spk_idx + (size_t)b * M * a.N;
     if (T == 0 && tid < nd) cnt_b[(lt0 + a.K + tid) % M] = 0;   // lists of the next n_delay steps
@@ -914,10 +926,10 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
 // into per-tile buckets of (first edge, length << 5 | step in block), segments longer than
 // BK_PIECE edges split into pieces, so that each (fly, tile)
 // workgroup reads only the segments that reach it, contiguously, instead of looking every
-// spike of the fly up in the segment table.  Grid (ceil(n_tiles / 64), B): lane t of every
-// wavefront owns tile 64 * blockIdx.x + t; the wavefronts split the spikes, staged through
-// LDS in chunks; positions come from LDS counters, so a bucket's order depends on timing while
-// its contents (and the integer sums made from them) do not.
+// spike of the fly up in the segment table.  Grid (ceil(n_tiles / 64), B) workgroups, one per
+// (tile group g, fly): lane t of every wavefront owns tile 64 g + t; the wavefronts split the
+// spikes, staged through LDS in chunks; positions come from LDS counters, so a bucket's order
+// depends on timing while its contents (and the integer sums made from them) do not.
 constexpr int BK_CHUNK = 2048;
 constexpr int BK_UNROLL = 16;  // table loads in flight per lane
 constexpr int BK_THREADS = 1024;
@@ -945,11 +957,15 @@ __global__ void __launch_bounds__(MAX_THREADS) tile_bucket(BucketArgs a) {
     __shared__ int s_cnt[64];
     __shared__ int s_id[BK_CHUNK];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, W = blockDim.x >> 6;
-    const int b = blockIdx.y, T = blockIdx.x * 64 + lane;
+    // the tile groups of one fly run on one XCD: they read the same spike lists, and the
+    // table-row spans of neighbouring groups share cache lines
+    int grp, b;
+    xcd_order(gridDim.x, gridDim.y, &grp, &b);
+    const int T = grp * 64 + lane;
     if (a.alive && !a.alive[b]) return;
     const long long t0 = a.clock[0] + a.kb;
     const int M = a.M, nd = a.nd;
-    const int lt0 = (int)(((t0 % M) + M) % M);
+    const int lt0 = step_slot(t0, M);
     const int* cnt_b = a.spk_cnt + (size_t)b * M;
     const int* idx_b = a.spk_idx + (size_t)b * M * a.N;
     if (tid < 64) s_cnt[tid] = 0;
