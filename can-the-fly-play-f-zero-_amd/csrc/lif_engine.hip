@@ -931,8 +931,10 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
 // spikes, staged through LDS in chunks; positions come from LDS counters, so a bucket's order
 // depends on timing while its contents (and the integer sums made from them) do not.
 constexpr int BK_CHUNK = 2048;
-constexpr int BK_UNROLL = 16;  // table loads in flight per lane
+constexpr int BK_UNROLL = 8;         // spikes per batch: table loads in flight per lane
 constexpr int BK_THREADS = 1024;
+constexpr int BK_STAGE = 256;        // entries staged in LDS per tile between flushes
+constexpr int BK_SP = BK_STAGE + 1;  // odd row stride: the 64 lanes' rows start in different banks
 
 struct BucketArgs {
     const long long* clock;
@@ -950,12 +952,24 @@ struct BucketArgs {
     int* err;
 };
 
-// T16: read the uint16 copy of the segment table (offsets from the row start).
+// T16: read the uint16 copy of the segment table (offsets from the row start).  At most 64
+// VGPRs (8 wavefronts per SIMD), so that two 1024-thread workgroups share a CU.
+//
+// Entries are staged per tile in LDS and written out in flushes, one tile per wavefront with
+// the lanes on consecutive entries: a tile gains only a few entries per batch, so writing
+// them straight to memory would cost one memory transaction per 4-byte entry.  A flush runs
+// after a round (every wavefront one batch) that left some stage more than half full; entries
+// past a full stage go straight to memory.
 template <bool T16>
-__global__ void __launch_bounds__(MAX_THREADS) tile_bucket(BucketArgs a) {
+__global__ void __launch_bounds__(BK_THREADS, 8) tile_bucket(BucketArgs a) {
     __shared__ int s_off[KMAX + 1];
-    __shared__ int s_cnt[64];
+    __shared__ int s_cnt[64];          // next position in each tile's bucket
+    __shared__ int s_fill[64];         // entries reserved in each tile's stage since the last flush
+    __shared__ int s_cap[64];          // each tile's bucket capacity
+    __shared__ long long s_base[64];   // each tile's bucket in the arena
+    __shared__ int s_flush;
     __shared__ int s_id[BK_CHUNK];
+    __shared__ int s_stage[64 * BK_SP];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, W = blockDim.x >> 6;
     // the tile groups of one fly run on one XCD: they read the same spike lists, and the
     // table-row spans of neighbouring groups share cache lines
@@ -968,8 +982,23 @@ __global__ void __launch_bounds__(MAX_THREADS) tile_bucket(BucketArgs a) {
     const int lt0 = step_slot(t0, M);
     const int* cnt_b = a.spk_cnt + (size_t)b * M;
     const int* idx_b = a.spk_idx + (size_t)b * M * a.N;
-    if (tid < 64) s_cnt[tid] = 0;
+    // buffers alternate with the block parity, so the bucket of block j can be written while
+    // the tile kernel of block j - 1 reads the other one
+    const int par = (int)((a.clock[2] + a.blk) & 1);
+    const bool own = T < a.n_tiles;
+    if (tid < 64) {
+        long long base = 0, cap = 0;
+        if (own) {
+            base = ((long long)par * a.B + b) * a.bk_stride + a.bk_off[T];
+            cap = a.bk_off[T + 1] - a.bk_off[T];
+        }
+        s_base[lane] = base;
+        s_cap[lane] = cap < 0x7fffffff ? (int)cap : 0x7fffffff;
+        s_cnt[lane] = 0;
+        s_fill[lane] = 0;
+    }
     if (tid == 0) {
+        s_flush = 0;
         int o = 0;
         for (int k = 0; k < a.K; k++) {
             s_off[k] = o;
@@ -980,19 +1009,27 @@ __global__ void __launch_bounds__(MAX_THREADS) tile_bucket(BucketArgs a) {
     }
     __syncthreads();
     const int total = s_off[a.K];
-    const bool own = T < a.n_tiles;
-    // buffers alternate with the block parity, so the bucket of block j can be written while
-    // the tile kernel of block j - 1 reads the other one
-    const int par = (int)((a.clock[2] + a.blk) & 1);
-    long long cap = 0;
-    int* dst = a.bk;  // this lane's bucket: entry p at dst[p]
-    if (own) {
-        dst = a.bk + ((long long)par * a.B + b) * a.bk_stride + a.bk_off[T];
-        cap = a.bk_off[T + 1] - a.bk_off[T];
-    }
-    const int capi = cap < 0x7fffffff ? (int)cap : 0x7fffffff;
+    int* const dst = a.bk + s_base[lane];  // this lane's bucket: entry p at dst[p]
+    const int capi = s_cap[lane];
     const int nt1 = a.n_tiles + 1;
     const int Tc = own ? T : 0;
+    // every thread, between barriers: stages to memory, then counters advanced and cleared
+    auto flush = [&]() {
+        for (int t = wave; t < 64; t += W) {
+            const int n = s_fill[t] < BK_STAGE ? s_fill[t] : BK_STAGE;
+            const int g0 = s_cnt[t], cp = s_cap[t];
+            int* d = a.bk + s_base[t];
+            for (int j = lane; j < n; j += 64)
+                if (g0 + j < cp) d[g0 + j] = s_stage[t * BK_SP + j];
+        }
+        __syncthreads();
+        if (tid < 64) {
+            s_cnt[tid] += s_fill[tid] < BK_STAGE ? s_fill[tid] : BK_STAGE;
+            s_fill[tid] = 0;
+        }
+        if (tid == 0) s_flush = 0;
+        __syncthreads();
+    };
     for (int c0 = 0; c0 < total; c0 += BK_CHUNK) {
         const int nc = total - c0 < BK_CHUNK ? total - c0 : BK_CHUNK;
         for (int q = tid; q < nc; q += blockDim.x) {  // stage the chunk: id << 5 | step, or -1
@@ -1003,68 +1040,81 @@ __global__ void __launch_bounds__(MAX_THREADS) tile_bucket(BucketArgs a) {
             s_id[q] = (unsigned)id < (unsigned)a.N ? id << 5 | k : -1;  // id < 2^22 (table < 2^31 entries)
         }
         __syncthreads();
-        for (int q0 = wave * BK_UNROLL; q0 < nc; q0 += W * BK_UNROLL) {
-            // The BK_UNROLL spikes are the same for every lane of the wavefront (lane = tile):
-            // their ids are made scalar, so the row addresses are scalar arithmetic and each
-            // lane's table reads are one scalar base plus its fixed tile offset, all in flight
-            // together.  Past the chunk end a lane rereads a valid spike and masks the result.
-            int lo[BK_UNROLL], len[BK_UNROLL], kk[BK_UNROLL], id[BK_UNROLL];
-            bool use[BK_UNROLL];
+        for (int r0 = 0; r0 < nc; r0 += W * BK_UNROLL) {
+            const int q0 = r0 + wave * BK_UNROLL;
+            if (q0 < nc) {
+                // The BK_UNROLL spikes are the same for every lane of the wavefront (lane = tile):
+                // their ids are made scalar, so the row addresses are scalar arithmetic and each
+                // lane's table reads are one scalar base plus its fixed tile offset, all in flight
+                // together.  Past the chunk end a lane rereads a valid spike and masks the result.
+                int lo[BK_UNROLL], len[BK_UNROLL], kk[BK_UNROLL], id[BK_UNROLL];
+                bool use[BK_UNROLL];
 #pragma unroll
-            for (int u = 0; u < BK_UNROLL; u++) {
-                const int q = q0 + u;
-                const int v = __builtin_amdgcn_readfirstlane(s_id[q < nc ? q : q0]);
-                use[u] = own && q < nc && v >= 0;
-                id[u] = v >= 0 ? v >> 5 : 0;
-                kk[u] = v & 31;
-            }
-            int l16[BK_UNROLL], h16[BK_UNROLL], r0[BK_UNROLL];
+                for (int u = 0; u < BK_UNROLL; u++) {
+                    const int q = q0 + u;
+                    const int v = __builtin_amdgcn_readfirstlane(s_id[q < nc ? q : q0]);
+                    use[u] = own && q < nc && v >= 0;
+                    id[u] = v >= 0 ? v >> 5 : 0;
+                    kk[u] = v & 31;
+                }
+                int l16[BK_UNROLL], h16[BK_UNROLL], r0_[BK_UNROLL];
 #pragma unroll
-            for (int u = 0; u < BK_UNROLL; u++) {  // every load first ...
-                if constexpr (T16) {
-                    const uint16_t* row = a.tb16 + (unsigned)id[u] * (unsigned)nt1;  // < 2^31 (checked)
-                    r0[u] = a.indptr[id[u]];
-                    l16[u] = row[Tc];
-                    h16[u] = row[Tc + 1];
-                } else {
-                    const int* row = a.tb + (unsigned)id[u] * (unsigned)nt1;
-                    r0[u] = 0;
-                    l16[u] = row[Tc];
-                    h16[u] = row[Tc + 1];
+                for (int u = 0; u < BK_UNROLL; u++) {  // every load first ...
+                    if constexpr (T16) {
+                        const uint16_t* row = a.tb16 + (unsigned)id[u] * (unsigned)nt1;  // < 2^31 (checked)
+                        r0_[u] = a.indptr[id[u]];
+                        l16[u] = row[Tc];
+                        h16[u] = row[Tc + 1];
+                    } else {
+                        const int* row = a.tb + (unsigned)id[u] * (unsigned)nt1;
+                        r0_[u] = 0;
+                        l16[u] = row[Tc];
+                        h16[u] = row[Tc + 1];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < BK_UNROLL; u++) {  // ... then the arithmetic
+                    lo[u] = r0_[u] + l16[u];
+                    len[u] = use[u] && h16[u] > l16[u] ? h16[u] - l16[u] : 0;
+                }
+                // pieces of at most BK_PIECE edges (almost every segment is one piece): one stage
+                // reservation per lane for the batch; what does not fit takes bucket positions now
+                int tot = 0;
+#pragma unroll
+                for (int u = 0; u < BK_UNROLL; u++) tot += (int)(((unsigned)len[u] + (BK_PIECE - 1)) / BK_PIECE);
+                if (tot > 0) {
+                    int slot = atomicAdd(&s_fill[lane], tot);
+                    const int over = slot + tot - BK_STAGE;
+                    int g = over > 0 ? atomicAdd(&s_cnt[lane], over < tot ? over : tot) : 0;
+                    if (slot + tot > BK_STAGE - W * BK_UNROLL) s_flush = 1;
+                    auto put = [&](int entry) {
+                        if (slot < BK_STAGE) {
+                            s_stage[lane * BK_SP + slot] = entry;
+                        } else {  // past the capacity: dropped, counted, flagged below
+                            if (g < capi) dst[g] = entry;
+                            g++;
+                        }
+                        slot++;
+                    };
+#pragma unroll
+                    for (int u = 0; u < BK_UNROLL; u++)
+                        if (len[u] > 0) {
+                            put((lo[u] << 7) | (((len[u] < BK_PIECE ? len[u] : BK_PIECE) - 1) << 5) | kk[u]);
+                            for (int c = BK_PIECE; c < len[u]; c += BK_PIECE)  // rare: longer segments
+                                put(((lo[u] + c) << 7) | (((len[u] - c < BK_PIECE ? len[u] - c : BK_PIECE) - 1) << 5) | kk[u]);
+                        }
                 }
             }
-#pragma unroll
-            for (int u = 0; u < BK_UNROLL; u++) {  // ... then the arithmetic
-                lo[u] = r0[u] + l16[u];
-                len[u] = use[u] && h16[u] > l16[u] ? h16[u] - l16[u] : 0;
-            }
-            // pieces of at most BK_PIECE edges: one LDS reservation per lane for the batch, then
-            // plain stores (almost every segment is a single piece)
-            unsigned tot = 0;
-#pragma unroll
-            for (int u = 0; u < BK_UNROLL; u++) tot += ((unsigned)len[u] + (BK_PIECE - 1)) / BK_PIECE;
-            if (tot > 0) {
-                int pos = atomicAdd(&s_cnt[lane], (int)tot);
-#pragma unroll
-                for (int u = 0; u < BK_UNROLL; u++)
-                    if (len[u] > 0 && pos < capi) {  // past the capacity: dropped, counted, flagged below
-                        const int l = len[u] < BK_PIECE ? len[u] : BK_PIECE;
-                        dst[pos++] = (lo[u] << 7) | ((l - 1) << 5) | kk[u];
-                        for (int c = BK_PIECE; c < len[u]; c += BK_PIECE) {  // rare: longer segments
-                            const int lc = len[u] - c < BK_PIECE ? len[u] - c : BK_PIECE;
-                            if (pos < capi) dst[pos] = ((lo[u] + c) << 7) | ((lc - 1) << 5) | kk[u];
-                            pos++;
-                        }
-                    }
-            }
+            __syncthreads();
+            if (s_flush) flush();
         }
-        __syncthreads();
     }
+    flush();
     if (tid < 64 && own) {
         int c = s_cnt[lane];
-        if (c > cap) {
+        if (c > capi) {
             atomicOr(a.err, 2);
-            c = (int)cap;
+            c = capi;
         }
         a.bk_cnt[((size_t)par * a.B + b) * a.n_tiles + T] = c;
     }
