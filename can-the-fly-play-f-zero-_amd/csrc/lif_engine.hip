@@ -479,6 +479,7 @@ struct TileArgs {
     const long long* bk_off;  // [n_tiles + 1] offset of each tile's bucket in a fly's arena
     long long bk_stride;
     const int* bk_cnt;     // [B][n_tiles] entries in each bucket
+    int b0, nb;            // this launch covers flies [b0, b0 + nb) (grid n_tiles x nb)
     Consts c;
 };
 
@@ -544,7 +545,8 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
     int* st_end = st_seg + TPB * IPT;    // [TPB * IPT] inclusive prefix of segment lengths
     int* st_k = st_end + TPB * IPT;      // [TPB * IPT]
     int b, T;
-    xcd_order(a.n_tiles, a.B, &T, &b);
+    xcd_order(a.n_tiles, a.nb, &T, &b);
+    b += a.b0;
     const int i0 = T * TS;
     const size_t bn = (size_t)b * a.N;
     const long long t0 = a.clock[0] + a.kb;
@@ -950,6 +952,7 @@ struct BucketArgs {
     int* bk_cnt;
     const unsigned char* alive;
     int* err;
+    int b0;  // this launch covers flies [b0, b0 + gridDim.y)
 };
 
 // T16: read the uint16 copy of the segment table (offsets from the row start).  At most 64
@@ -975,6 +978,7 @@ __global__ void __launch_bounds__(BK_THREADS, 8) tile_bucket(BucketArgs a) {
     // table-row spans of neighbouring groups share cache lines
     int grp, b;
     xcd_order(gridDim.x, gridDim.y, &grp, &b);
+    b += a.b0;
     const int T = grp * 64 + lane;
     if (a.alive && !a.alive[b]) return;
     const long long t0 = a.clock[0] + a.kb;
@@ -1311,6 +1315,8 @@ TileArgs make_tile(const fz_engine* e, const fz_inputs* in, int K, int KP, int k
     a.clock = e->st.clock;
     a.N = e->n;
     a.B = e->st.B;
+    a.b0 = 0;
+    a.nb = e->st.B;
     a.K = K;
     a.KP = KP;
     a.kb = kb;
@@ -1945,68 +1951,86 @@ int fz_graph_create(fz_engine* e, const fz_inputs* in, int steps, const fz_tune*
     const dim3 ugrid((e->n + s.ut - 1) / s.ut, e->st.B);
     const dim3 pgrid = push_grid(e, s);
     int kb = 0, blk = 0;
+    std::vector<hipGraphNode_t> lasts;  // tile engine: the last node of each chain
     if (e->mode == 2) {
-        const dim3 tgrid(e->n_tiles, e->st.B);
-        const dim3 bgrid((e->n_tiles + 63) / 64, e->st.B);
         void* fn = tile_kernel(tile_fmt(e), e->ts / s.ut, s.ipt, s.src);
         if (s.src == 1 && (rc = ensure_buckets(e))) {
             (void)hipGraphDestroy(gr->graph);
             delete gr;
             return rc;
         }
-        // The bucket node of block j needs the spikes of steps [kb - n_delay, kb - n_delay + K)
-        // and its parity buffer free (last read by block j - 2): with K <= n_delay / 2 both come
-        // from block j - 2 or earlier, so it runs beside the tile kernel of block j - 1.
-        std::vector<hipGraphNode_t> tiles;
-        std::vector<int> tile_end;  // window step after each tile block
+        // With the bucket source the flies are split into two halves with independent chains of
+        // nodes, so that one half's bucket prologue (latency-bound) can run beside the other
+        // half's tile kernel (ALU-bound).  Within a chain, the bucket node of block j needs the
+        // spikes of steps [kb - n_delay, kb - n_delay + K) and its parity buffer free (last read
+        // by block j - 2): with K <= n_delay / 2 both come from block j - 2 or earlier, so it
+        // also runs beside the chain's tile kernel of block j - 1.
+        struct Chain {
+            int b0, nb;
+            hipGraphNode_t prev = nullptr;
+            std::vector<hipGraphNode_t> tiles;
+            std::vector<int> tile_end;  // window step after each tile block
+        };
+        const int B = e->st.B, H = (s.src == 1 && B >= 2) ? 2 : 1;
+        std::vector<Chain> chains(H);
+        for (int h = 0; h < H; h++) {
+            chains[h].b0 = h * B / H;
+            chains[h].nb = (h + 1) * B / H - chains[h].b0;
+        }
         while (kb < steps) {
             const int K = (steps - kb) < s.K ? (steps - kb) : s.K;
-            hipGraphNode_t bnode = nullptr;
-            if (s.src == 1) {
-                BucketArgs ba = make_bucket(e, in, K, kb, blk);
-                void* bargs[] = {&ba};
-                const int need = kb - e->c.n_delay + K - 1;  // last spike step it reads
-                int dep = blk - 2;
-                for (int i = 0; i < (int)tiles.size(); i++)
-                    if (need >= 0 && tile_end[i] > need) {
-                        dep = i > dep ? i : dep;
-                        break;
-                    }
-                if (dep >= blk) dep = blk - 1;
+            for (Chain& c : chains) {
+                hipGraphNode_t bnode = nullptr;
+                if (s.src == 1) {
+                    BucketArgs ba = make_bucket(e, in, K, kb, blk);
+                    ba.b0 = c.b0;
+                    void* bargs[] = {&ba};
+                    const int need = kb - e->c.n_delay + K - 1;  // last spike step it reads
+                    int dep = blk - 2;
+                    for (int i = 0; i < (int)c.tiles.size(); i++)
+                        if (need >= 0 && c.tile_end[i] > need) {
+                            dep = i > dep ? i : dep;
+                            break;
+                        }
+                    if (dep >= blk) dep = blk - 1;
+                    hipKernelNodeParams kp{};
+                    kp.func = e->tb16 ? reinterpret_cast<void*>(&tile_bucket<true>)
+                                      : reinterpret_cast<void*>(&tile_bucket<false>);
+                    kp.gridDim = dim3((e->n_tiles + 63) / 64, c.nb);
+                    kp.blockDim = dim3(BK_THREADS);
+                    kp.sharedMemBytes = 0;
+                    kp.kernelParams = bargs;
+                    kp.extra = nullptr;
+                    hipGraphNode_t d = dep >= 0 ? c.tiles[dep] : nullptr;
+                    if ((err = hipGraphAddKernelNode(&bnode, gr->graph, d ? &d : nullptr, d ? 1 : 0, &kp)) != hipSuccess)
+                        return bail("hipGraphAddKernelNode(bucket)", err);
+                }
+                TileArgs ta = make_tile(e, in, K, tile_kp(e, s, K), kb, blk, kb == 0, nullptr, nullptr);
+                ta.b0 = c.b0;
+                ta.nb = c.nb;
+                void* targs[] = {&ta};
                 hipKernelNodeParams kp{};
-                kp.func = e->tb16 ? reinterpret_cast<void*>(&tile_bucket<true>)
-                                  : reinterpret_cast<void*>(&tile_bucket<false>);
-                kp.gridDim = bgrid;
-                kp.blockDim = dim3(BK_THREADS);
-                kp.sharedMemBytes = 0;
-                kp.kernelParams = bargs;
+                kp.func = fn;
+                kp.gridDim = dim3(e->n_tiles, c.nb);
+                kp.blockDim = dim3(s.ut);
+                kp.sharedMemBytes = (unsigned)tile_lds(e, s, K);
+                kp.kernelParams = targs;
                 kp.extra = nullptr;
-                hipGraphNode_t d = dep >= 0 ? tiles[dep] : nullptr;
-                if ((err = hipGraphAddKernelNode(&bnode, gr->graph, d ? &d : nullptr, d ? 1 : 0, &kp)) != hipSuccess)
-                    return bail("hipGraphAddKernelNode(bucket)", err);
+                hipGraphNode_t deps[2];
+                int nd = 0;
+                if (c.prev) deps[nd++] = c.prev;
+                if (bnode) deps[nd++] = bnode;
+                hipGraphNode_t tnode;
+                if ((err = hipGraphAddKernelNode(&tnode, gr->graph, nd ? deps : nullptr, nd, &kp)) != hipSuccess)
+                    return bail("hipGraphAddKernelNode(tile)", err);
+                c.prev = tnode;
+                c.tiles.push_back(tnode);
+                c.tile_end.push_back(kb + K);
             }
-            TileArgs ta = make_tile(e, in, K, tile_kp(e, s, K), kb, blk, kb == 0, nullptr, nullptr);
-            void* targs[] = {&ta};
-            hipKernelNodeParams kp{};
-            kp.func = fn;
-            kp.gridDim = tgrid;
-            kp.blockDim = dim3(s.ut);
-            kp.sharedMemBytes = (unsigned)tile_lds(e, s, K);
-            kp.kernelParams = targs;
-            kp.extra = nullptr;
-            hipGraphNode_t deps[2];
-            int nd = 0;
-            if (prev) deps[nd++] = prev;
-            if (bnode) deps[nd++] = bnode;
-            hipGraphNode_t tnode;
-            if ((err = hipGraphAddKernelNode(&tnode, gr->graph, nd ? deps : nullptr, nd, &kp)) != hipSuccess)
-                return bail("hipGraphAddKernelNode(tile)", err);
-            prev = tnode;
-            tiles.push_back(tnode);
             kb += K;
-            tile_end.push_back(kb);
             blk++;
         }
+        for (Chain& c : chains) lasts.push_back(c.prev);
     }
     while (kb < steps) {
         const int K = (steps - kb) < s.K ? (steps - kb) : s.K;
@@ -2030,8 +2054,20 @@ int fz_graph_create(fz_engine* e, const fz_inputs* in, int steps, const fz_tune*
     long long* clock = e->st.clock;
     int st = steps, nb = blk;
     void* cargs[] = {&clock, &st, &nb};
-    if ((err = add(reinterpret_cast<void*>(&lif_clock), dim3(1), dim3(64), cargs)) != hipSuccess)
+    if (!lasts.empty()) {  // the clock advances after every chain
+        hipKernelNodeParams kp{};
+        kp.func = reinterpret_cast<void*>(&lif_clock);
+        kp.gridDim = dim3(1);
+        kp.blockDim = dim3(64);
+        kp.sharedMemBytes = 0;
+        kp.kernelParams = cargs;
+        kp.extra = nullptr;
+        hipGraphNode_t node;
+        if ((err = hipGraphAddKernelNode(&node, gr->graph, lasts.data(), lasts.size(), &kp)) != hipSuccess)
+            return bail("hipGraphAddKernelNode(clock)", err);
+    } else if ((err = add(reinterpret_cast<void*>(&lif_clock), dim3(1), dim3(64), cargs)) != hipSuccess) {
         return bail("hipGraphAddKernelNode(clock)", err);
+    }
     err = hipGraphInstantiate(&gr->exec, gr->graph, nullptr, nullptr, 0);
     if (err != hipSuccess) return bail("hipGraphInstantiate", err);
     *out = gr;
