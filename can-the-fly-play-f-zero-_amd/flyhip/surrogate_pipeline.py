@@ -216,15 +216,15 @@ def inference_fps(ens, batch: int = 4096, frames: int = 20):
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--train-frames", type=int, default=7800, help="frames per fly (x --batch flies)")
+    ap.add_argument("--train-frames", type=int, default=23400, help="frames per fly (x --batch flies)")
     ap.add_argument("--batch", type=int, default=256)
-    ap.add_argument("--test-frames", type=int, default=200, help="frames per held-out sequence")
-    ap.add_argument("--test-sequences", type=int, default=250)
-    ap.add_argument("--replicas", type=int, default=32, help="simulated flies per held-out sequence")
+    ap.add_argument("--test-frames", type=int, default=150, help="frames per held-out sequence")
+    ap.add_argument("--test-sequences", type=int, default=128)
+    ap.add_argument("--replicas", type=int, default=128, help="simulated flies per held-out sequence")
     ap.add_argument("--train-replicas", type=int, default=8,
                     help="flies per training sequence; targets are their mean rate (same simulated frames)")
     ap.add_argument("--members", type=int, default=3)
-    ap.add_argument("--steps", type=int, default=10000)
+    ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--threshold", type=float, default=0.15)
     ap.add_argument("--rate-scale", type=float, default=2.0, help="Hz per unit of the network's output")
