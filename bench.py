@@ -78,8 +78,13 @@ def shape_spec(engine, tile, B, shape):
 
 
 def measure_line(fly, p, B, engine, tile, frames, warmup, torch, gain=None, drive=None, seed=0,
-                 kernel_frames=0):
-    """Build, autotune and time one engine configuration on ``frames`` game frames."""
+                 kernel_frames=0, episodes=None):
+    """Build, autotune and time one engine configuration on ``frames`` game frames.
+
+    ``episodes`` (B > 1): per-fly episode lengths in frames.  Each fly starts at a random point
+    of its episode; a fly whose episode ends is retired for one frame (alive = 0, its workgroups
+    return at once) and then respawned at rest with a new length drawn from the same range, as
+    the reference's fleet does with its emulators.  ``value`` counts alive fly-frames only."""
     from flyhip import BatchBrain, autotune
 
     bb = BatchBrain(fly.weights, p, batch=B, seed=seed, engine=engine, tile=tile)
@@ -91,17 +96,47 @@ def measure_line(fly, p, B, engine, tile, frames, warmup, torch, gain=None, driv
     for f in range(warmup):
         bb.p_ext[:, j] = probs[f]
         bb.run(FRAME_MS, copy=False)
+    # the episode schedule is drawn up front and staged on the device, so that retiring and
+    # respawning flies inside the timed region needs no host round trip
+    alive_frames = B * frames
+    sched = []
+    if episodes is not None:
+        erng = np.random.default_rng(seed + 100)
+        lo, hi = episodes
+        length = erng.integers(lo, hi + 1, B)
+        pos = erng.integers(0, length)
+        retired = np.zeros(B, bool)
+        alive_frames = 0
+        for f in range(frames):
+            alive_frames += int((~retired).sum())
+            back = np.flatnonzero(retired)         # retired this frame: respawn at rest
+            length[back] = erng.integers(lo, hi + 1, len(back))
+            pos[back] = 0
+            retired[back] = False
+            pos[~retired] += 1
+            retired[pos >= length] = True          # episode over: retire for the next frame
+            sched.append((torch.from_numpy(back).cuda() if len(back) else None,
+                          torch.from_numpy((~retired).astype(np.uint8)).cuda()))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for f in range(warmup, warmup + frames):
         bb.p_ext[:, j] = probs[f]
         bb.run(FRAME_MS, copy=False)
+        if sched:
+            back, alive = sched[f - warmup]
+            if back is not None:
+                bb.reset(back)
+            bb.alive.copy_(alive)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     steps = bb.steps_of(FRAME_MS)
     res = {"engine": engine if engine == "push" else f"tile{tile}", "B": B, "frames": frames,
-           "value": B * frames * steps * p.dt / 1000.0 / wall, "ms_per_frame": wall / frames * 1000.0,
+           "value": alive_frames * steps * p.dt / 1000.0 / wall, "ms_per_frame": wall / frames * 1000.0,
            "launch_shape": shape}
+    if episodes is not None:
+        res["alive_fly_frames"] = alive_frames
+        res["episodes"] = f"U{{{lo}..{hi}}} frames, random start phase, retire 1 frame then respawn"
+        bb.alive.fill_(1)
     if kernel_frames:
         deg = np.diff(bb.indptr).astype(np.float64)
         tim = {"update_ms": 0.0, "update_launches": 0, "push_ms": 0.0, "push_launches": 0}
@@ -138,6 +173,67 @@ def measure_line(fly, p, B, engine, tile, frames, warmup, torch, gain=None, driv
                            "kernel_ms_per_frame": round((tim["update_ms"] + tim["push_ms"]) / kernel_frames, 4),
                            "spikes_per_step_per_fly": round(spikes / (kernel_frames * steps * B), 1)}
     del bb, probs
+    torch.cuda.empty_cache()
+    return res
+
+
+def instruct_line(fly, p, torch, B=64, frames=100, warmup=3):
+    """Batched LIF plus instructed learning one layer deep (flyzero/batch.py:336-403): B slots
+    in groups of 4 per fly, each frame one window of the simulation and the rule's four kernels
+    (traces, retrograde error, DN update, deep update) on the synapses onto the DNs and onto
+    their presynaptic partners; targets from a slowly varying steer / gas intent."""
+    from flyhip import BatchBrain, autotune
+    from flyhip.instruct import BatchInstructDeep, deep_positions, plastic_positions, targets
+
+    p1 = plastic_positions(fly)
+    deep = deep_positions(fly, p1)
+    pos = np.union1d(p1, deep)
+    rng = np.random.default_rng(8)
+    drive = rng.uniform(20.0, 120.0, B).astype(np.float32)
+    gain = rng.uniform(0.5, 2.0, B).astype(np.float32)
+    bb = BatchBrain(fly.weights, p, batch=B, seed=4, plastic_pos=pos, engine="tile", tile=1024)
+    idx = np.r_[fly.input_idx, fly.drive_idx]
+    j = bb.slots(idx)
+    probs = torch.from_numpy(frame_rates(fly, 64, gain, drive)).cuda() * float(np.float32(p.dt / 1000.0))
+    bb.p_ext[:, j] = probs[0]
+    shape = autotune.tune(bb, use_cache=False, reps=1)
+    bi = BatchInstructDeep(bb, fly, fly_of_slot=np.arange(B) // 4, eta_deep=1e-6)
+    t = np.arange(warmup + frames)
+    tg = np.stack([targets([np.sin(k / 20.0), 0.0, 0.5 + 0.5 * np.cos(k / 30.0), 0.0, 0.0], bi.p) for k in t])
+    tg = np.broadcast_to(tg[:, None, :], (len(t), B, tg.shape[1])).astype(np.float32)
+    learn = np.ones(B, bool)
+    w0 = bb.pw.clone()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * frames)]
+
+    def frame(f, timed):
+        bb.p_ext[:, j] = probs[f % 64]
+        counts = bb.run(FRAME_MS, copy=False)
+        if timed:
+            ev[2 * (f - warmup)].record()
+        bi.step(counts, tg[f], learn, FRAME_MS, report=False)
+        if timed:
+            ev[2 * (f - warmup) + 1].record()
+
+    for f in range(warmup):
+        frame(f, False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for f in range(warmup, warmup + frames):
+        frame(f, True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    rule_ms = sum(ev[2 * i].elapsed_time(ev[2 * i + 1]) for i in range(frames)) / frames
+    changed = (bb.pw != w0)
+    res = {"B": B, "flies": int(bi.n_flies), "frames": frames, "dn_synapses": int(len(bi.dn_sel_np)),
+           "deep_synapses": int(bi.n_deep), "l1_neurons": int(len(bi.l1)),
+           "value": round(B * frames * bb.steps_of(FRAME_MS) * p.dt / 1000.0 / wall, 3),
+           "unit": "fly-sim-seconds/wall-second", "ms_per_frame": round(wall / frames * 1000.0, 3),
+           "instruct_step_ms": round(rule_ms, 4),
+           "weights_changed": {"dn": int(changed[:, torch.from_numpy(bi.dn_sel_np).cuda()].sum()),
+                               "deep": int(changed[:, torch.from_numpy(bi.deep_sel_np).cuda()].sum())},
+           "launch_shape": shape,
+           "workload": "tile engine + BatchInstructDeep every frame (4 slots per fly, eta_deep 1e-6)"}
+    del bb, bi, probs
     torch.cuda.empty_cache()
     return res
 
@@ -264,12 +360,14 @@ def main():
     ap.add_argument("--steps", type=int, default=60, help="timed game frames (166 LIF steps each)")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--batch", type=int, default=256, help="flies in the batched line (0: skip)")
-    ap.add_argument("--batch-steps", type=int, default=10)
+    ap.add_argument("--batch-steps", type=int, default=30)
     ap.add_argument("--engines1", default="push,tile512,tile1024")
     ap.add_argument("--enginesB", default="tile512,tile1024")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample length")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plastic-frames", type=int, default=300, help="frames of the B=64 plasticity line (0: skip)")
+    ap.add_argument("--instruct-frames", type=int, default=100,
+                    help="frames of the B=64 instructed-learning line (0: skip)")
     ap.add_argument("--profile", default="", help="counter-run mode: engine:B:upd_threads:block_steps:tile_stage[:tile_source]")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -321,7 +419,7 @@ def main():
         linesB = []
         for engine, tile in parse(args.enginesB):
             r = measure_line(fly, p, B, engine, tile, args.batch_steps, 2, torch, gain, drive, seed=1,
-                             kernel_frames=2)
+                             kernel_frames=2, episodes=(20, 600))
             log(f"B={B} {r['engine']}: {r['value']:.2f} fly-sim-s/s ({r['ms_per_frame']:.2f} ms/frame) "
                 f"{r['launch_shape']}; {r['roofline']['kernel']} {r['roofline']['avg_launch_us']} us/launch")
             linesB.append(r)
@@ -335,6 +433,14 @@ def main():
         plastic = plastic_line(fly, p, torch, B=64, frames=args.plastic_frames)
         log(f"B=64 plastic: {plastic['value']:.2f} fly-sim-s/s ({plastic['ms_per_frame']:.2f} ms/frame), "
             f"reward step {plastic['reward_step_ms']} ms, {plastic['synapse_counts_changed']} count edits")
+
+    # ---- B = 64 with instructed learning one layer deep every frame ------------------------
+    instruct = None
+    if args.instruct_frames > 0:
+        instruct = instruct_line(fly, p, torch, B=64, frames=args.instruct_frames)
+        log(f"B=64 instructed (deep): {instruct['value']:.2f} fly-sim-s/s ({instruct['ms_per_frame']:.2f} ms/frame), "
+            f"rule {instruct['instruct_step_ms']} ms, {instruct['dn_synapses']} DN + {instruct['deep_synapses']} "
+            f"deep synapses")
 
     # ---- CPU baseline (oracle, one thread) --------------------------------------------------
     cpu = None
@@ -382,6 +488,7 @@ def main():
                        "connections": int(fly.weights.nnz), "batch": 1, "steps_per_frame": steps_per_frame,
                        "engine": best["engine"], "parallelism": "single GPU"},
             "roofline": best["roofline"], "cpu_baseline": cpu, "batched": batched, "plastic": plastic,
+            "instruct": instruct,
             "b1_candidates": [{k: r[k] for k in ("engine", "value", "ms_per_frame")} for r in lines1]}
     print(json.dumps(line), flush=True)
 

@@ -81,7 +81,23 @@ SIGNATURES = {
     "fz_sg_cast": (c_int, [ctypes.c_longlong, c_void_p, c_void_p, c_void_p]),
     "fz_dn_rates": (c_int, [c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, ctypes.c_float, ctypes.c_float,
                             c_void_p, c_void_p]),
+    "fz_instruct_traces": (c_int, [c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, ctypes.c_float,
+                                   ctypes.c_float, c_int, c_void_p, c_void_p, ctypes.c_float, ctypes.c_float,
+                                   c_void_p]),
+    "fz_instruct_delta": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p,
+                                  c_int, c_void_p, c_int, ctypes.c_float, c_void_p, c_void_p]),
+    "fz_instruct_update": (c_int, [c_void_p, c_void_p]),
+    "fz_instruct_bias": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
+                                 c_void_p, c_int, ctypes.c_float, c_void_p, ctypes.c_float, c_void_p, c_void_p,
+                                 c_int, c_void_p, c_void_p]),
 }
+
+
+class FzInstructUpdateArgs(ctypes.Structure):
+    """fz_instruct_update_args of include/flyhip.h."""
+    _fields_ = [(f, c_int) for f in ("mode", "n_flies", "n_sel", "n_pl", "n_upre", "n_upost", "n_groups", "n_l")] + \
+               [(f, c_void_p) for f in ("sel", "pre_k", "group", "post_k", "mask", "post_l", "fly_ptr", "fly_slots",
+                                       "learn", "coef", "trace", "rate", "tgt", "delta", "sign", "wmax", "pw")]
 
 _lib = None
 

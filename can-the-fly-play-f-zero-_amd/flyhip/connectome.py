@@ -15,7 +15,8 @@ a download the reference performs in flyzero/connectome.py:52-65, which is out o
 * descending-neuron cell types with the names the reference's readout uses
   (flyzero/instruct.py:43-48: DNa02, DNg02*, DNa01 per side, DNp09, MDN), ten neurons each,
   picked among neurons whose in-degree is near 7,300 / 40 so that all inputs onto the four
-  motor types DNa02, DNa01, DNg02*, DNp09 number about 7,300.
+  motor types DNa02, DNa01, DNg02*, DNp09 number about 7,300; and the giant fiber DNp01 (one
+  per side) that the batched instructed rules also drive (flyzero/batch.py:20-22).
 
 The matrix holds signed synapse counts like ``Connectome.weights`` (flyzero/connectome.py:68-120);
 ``find(type, side)`` mirrors ``Connectome.find``.
@@ -159,6 +160,14 @@ def flywire_like(n: int = N_FLYWIRE, n_edges: int = E_FLYWIRE, seed: int = 0, si
     side = (r3.random(n) < 0.5).astype(np.int8)
     for t, ids in types.items():   # balanced sides within a type
         side[ids] = (np.arange(len(ids)) >= len(ids) // 2).astype(np.int8)
+    # the giant fiber (one per side), drawn afterwards from its own stream so that the types
+    # above do not depend on it
+    taken[pick] = True
+    r6 = np.random.default_rng(seed + 6)
+    cand = np.argsort(np.abs(indeg - want) + taken * 1e9)[:64]
+    gf = np.sort(r6.choice(cand, 2, replace=False)).astype(np.int64)
+    types["DNp01"] = gf
+    side[gf] = (0, 1)
     return SyntheticFly(w, input_idx, input_rate, input_xy, drive_idx, 60.0, types, side, seed)
 
 

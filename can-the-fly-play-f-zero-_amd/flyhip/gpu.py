@@ -215,7 +215,10 @@ class BatchBrain:
                 self.spk_cnt.zero_()
             self.t = 0
             return
-        f = torch.as_tensor(np.atleast_1d(flies), dtype=torch.long, device=self.device)
+        if isinstance(flies, torch.Tensor):  # a device index tensor: no host round trip
+            f = flies.to(self.device, torch.long)
+        else:
+            f = torch.as_tensor(np.atleast_1d(flies), dtype=torch.long, device=self.device)
         self.u[f] = 0.0
         self.g[f] = 0.0
         self.ref[f] = 0

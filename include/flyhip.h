@@ -182,6 +182,64 @@ int fz_reward_step(const fz_reward_args* a, void* stream);
 int fz_dn_rates(int B, int n, const int* counts, int n_groups, const int* gptr, const int* gidx, float hz_scale,
                 float alpha, float* rates, void* stream);
 
+/* ---- instructed learning (replaces the rule of flyzero/instruct.py:124-148 as batched by
+ * BatchInstruct / BatchInstructDeep / BatchInstructDeep2, flyzero/batch.py:226-451, and the
+ * deep_update kernel of flyzero/batch.py:27-52).  Float32 as written, fixed summation orders;
+ * slot -> fly grouping as a CSR (fly_ptr, fly_slots ascending).  All pointers are device
+ * pointers. ---- */
+
+/* trace[b][k] += a_pre * (counts[b][upre[k]] * hz * pre_gain - trace[b][k]) and
+ * rate[b][u] += a_rate * (counts[b][upost[u]] * hz - rate[b][u])
+ * (flyzero/batch.py:289-293; pre_gain = tau_pre / 1000). */
+int fz_instruct_traces(int B, int n, const int* counts, int n_upre, const int* upre, float* trace, float a_pre,
+                       float pre_gain, int n_upost, const int* upost, float* rate, float a_rate, float hz,
+                       void* stream);
+
+/* Retrograde error of the L1 neurons (flyzero/batch.py:395-399): delta[b][l] = sum over
+ * q in [lptr[l], lptr[l+1]) of (pw[b][lsyn[q]] / w_syn) * (tgt[b][lgroup[q]] - rate[b][lpostk[q]]),
+ * in ascending q.  Read pw before the DN synapses are updated. */
+int fz_instruct_delta(int B, int n_l1, const int* lptr, const int* lsyn, const int* lgroup, const int* lpostk,
+                      const float* tgt, int n_groups, const float* rate, int n_upost, const float* pw, int n_pl,
+                      float w_syn, float* delta, void* stream);
+
+typedef struct fz_instruct_update_args {
+    int mode;                   /* 0: delta rule onto the DNs (flyzero/batch.py:158-172,320-322,392-393);
+                                 * 1: a deeper layer (the deep_update kernel, flyzero/batch.py:27-52) */
+    int n_flies, n_sel, n_pl, n_upre, n_upost, n_groups, n_l;
+    const int* sel;             /* [n_sel] plastic index of each listed synapse, or NULL = identity */
+    const int* pre_k;           /* [n_sel] trace column of its pre neuron */
+    const int* group;           /* mode 0: [n_sel] instructed group of its post neuron */
+    const int* post_k;          /* mode 0: [n_sel] rate column of its post neuron */
+    const unsigned char* mask;  /* mode 0: [n_sel] 1 = learns, or NULL = all */
+    const int* post_l;          /* mode 1: [n_sel] delta column of its post neuron */
+    const int* fly_ptr;         /* [n_flies + 1] */
+    const int* fly_slots;       /* [B] */
+    const unsigned char* learn; /* mode 0: [B] */
+    const float* coef;          /* [B] mode 0: eta per slot; mode 1: eta_deep / learning slots of
+                                 * the fly, 0 for slots that do not learn */
+    const float* trace;         /* [B][n_upre] */
+    const float* rate;          /* mode 0: [B][n_upost] */
+    const float* tgt;           /* mode 0: [B][n_groups] target rates */
+    const float* delta;         /* mode 1: [B][n_l] */
+    const float* sign;          /* [n_pl] sign of the initial weight */
+    const float* wmax;          /* [n_pl] cap */
+    float* pw;                  /* [B][n_pl] plastic weights, mV, updated in place */
+} fz_instruct_update_args;
+
+/* w = sign * clip(sign * (w + d), 0, wmax) for every slot of each fly, d the fly's mean change
+ * (mode 0: sum over learning slots of (1 / n) * eta * trace * err * mask in ascending slot
+ * order, applied to every slot; mode 1: sum over slots of coef * trace * delta, applied where
+ * nonzero). */
+int fz_instruct_update(const fz_instruct_update_args* a, void* stream);
+
+/* Intrinsic plasticity of the instructed DNs (flyzero/batch.py:302-310): per (fly, DN) the
+ * error averaged over the fly's learning slots, then for every slot ib' = clip(ib + eta_bias *
+ * mean * ratio[b], -limit, limit) and bias_ext[b][dn_slot] += ib' - ib. */
+int fz_instruct_bias(int n_flies, int n_dn, const int* fly_ptr, const int* fly_slots, const unsigned char* learn,
+                     const int* dn_group, const int* dn_rate_k, const float* tgt, int n_groups, const float* rate,
+                     int n_upost, float eta_bias, const float* ratio, float limit, float* ib, float* bias_ext,
+                     int n_ext, const int* dn_slot, void* stream);
+
 /* ---- learned rate surrogate: BF16 MFMA GEMM and training / inference element-wise kernels ---- */
 
 #define FZ_SG_STORE32 0 /* C32 = A B (+ bias) */

@@ -270,9 +270,12 @@ def test_extreme_launch_shapes_are_exact(small_fly, shape):
                                         (1024, dict(upd_threads=256, tile_source=1)),
                                         (512, dict(upd_threads=64, block_steps=5, tile_source=1)),
                                         (2048, dict(upd_threads=512, block_steps=9, tile_stage=4, tile_source=1)),
-                                        (256, dict(upd_threads=256, block_steps=7, tile_source=1))])
+                                        (256, dict(upd_threads=256, block_steps=7, tile_source=1)),
+                                        (768, dict(upd_threads=384, tile_source=1)),
+                                        (640, dict(upd_threads=640, tile_stage=4))])
 def test_tile_engine_shapes_are_exact(small_fly, tile, shape):
-    # shapes that the LDS budget admits, incl. 8 neurons per thread and 5-step blocks
+    # shapes that the LDS budget admits, incl. 8 neurons per thread, 5-step blocks and tiles
+    # that are not powers of two
     fly = small_fly
     idx = np.r_[fly.input_idx, fly.drive_idx]
     gb, ob = _pair(fly, batch=2, seed=8, engine="tile", tile=tile)
@@ -287,7 +290,7 @@ def test_tile_engine_shapes_are_exact(small_fly, tile, shape):
 
 def test_tile_engine_rejects_shapes_over_the_lds_budget(small_fly):
     gb, _ = _pair(small_fly, batch=1, engine="tile", tile=2048)
-    gb.set_tune(upd_threads=1024, tile_stage=4)       # 18 x 2048 x 4 B + 3 x 4 x 1024 x 4 B > 160 KB
+    gb.set_tune(upd_threads=1024, tile_stage=4)       # 19 x 2048 x 4 B + 3 x 4 x 1024 x 4 B > 160 KB
     with pytest.raises(ValueError, match="LDS"):
         gb.run(FRAME_MS, graph=False)
 
