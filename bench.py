@@ -56,14 +56,27 @@ def frame_rates(fly, frames: int, gain=None, drive=None) -> np.ndarray:
 PROFILE = os.path.join(REPO, "profiles", "lif_tile_flywire138k.md")
 
 
+def _same_shape(a: str, b: str) -> bool:
+    """Launch specs engine:B:upd_threads:block_steps:tile_stage:tile_source; the staging depth
+    only changes the segment-table source, so it is ignored for the bucket source."""
+    x, y = a.split(":"), b.split(":")
+    if len(x) != 6 or len(y) != 6:
+        return a == b
+    if x[5] == y[5] == "1":
+        x[4] = y[4] = "*"
+    return x == y
+
+
 def measured_traffic(spec: str):
     """HBM bytes per launch of the dominant kernel for a launch shape, from the rocprofv3
     FETCH_SIZE / WRITE_SIZE summary in profiles/ (lines 'traffic-json: {...}'), or None."""
+    if not spec:
+        return None
     try:
         for line in open(PROFILE):
             if line.startswith("traffic-json:"):
                 rec = json.loads(line.split(":", 1)[1])
-                if rec.get("spec") == spec:
+                if _same_shape(rec.get("spec", ""), spec):
                     return rec
     except OSError:
         pass
