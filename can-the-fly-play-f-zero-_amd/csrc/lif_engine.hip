@@ -70,6 +70,7 @@ constexpr int MAX_WAVES = MAX_THREADS / 64;
 constexpr int QMIN = 64;          // smallest push chunk (sizes the chunk-start table)
 constexpr int BK_PIECE = 4;  // edges per tile-bucket entry at most (longer segments are split)
 constexpr size_t TILE_STATIC_LDS = 1024;  // room left for lif_tile's static shared arrays
+constexpr int PL_SERIAL = 2;  // plastic rows up to this length are pushed by their own thread
 constexpr int UNROLL = 4;         // tile engine: edge loads in flight per thread
 
 struct Consts {
@@ -895,7 +896,7 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
         }
     }
 
-    // append spikes to the per-step lists; push plastic rows n_delay steps ahead
+    // append spikes to the per-step lists, then push the spikers' plastic rows n_delay steps ahead
     if (tid < a.K) s_lcnt[tid] = 0;
     __syncthreads();
 #pragma unroll
@@ -911,14 +912,81 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
             const int k = __ffs(m) - 1;
             const int pos = atomicAdd(&s_lcnt[k], 1);
             if (pos < a.N) idx_b[(size_t)((lt0 + k) % M) * a.N + pos] = i;
-            if (a.pl_indptr) {
-                int* r = a.ring + ((size_t)b * a.ring_slots + (rs0 + k + nd) % a.ring_slots) * a.N;
-                const float* pwb = a.pw + (size_t)b * a.n_pl;
-                for (int q = a.pl_indptr[i]; q < a.pl_indptr[i + 1]; q++) {
+        }
+    }
+    if (a.pl_indptr) {
+        // Plastic rows of the block's spikes go n_delay steps ahead.  A short row is pushed by its
+        // own thread; longer rows are pushed by the whole workgroup: staged with their prefix
+        // sums (in acc, free after the step loop) and the edges split evenly over the threads by
+        // a binary search, so a long row costs no more than many short ones.  Rounds take at
+        // most one spike per neuron (one round when neurons fire once per block).
+        int* p_beg = acc;           // [TS] first plastic edge of the staged row (thread-major)
+        int* p_end = acc + TS;      // [TS] inclusive prefix of row lengths
+        int* p_k = acc + 2 * TS;    // [TS] block step of the spike
+        const float* pwb = a.pw + (size_t)b * a.n_pl;
+        uint32_t rem[NPT];
+        bool more = false;
+#pragma unroll
+        for (int p = 0; p < NPT; p++) {
+            rem[p] = 0;
+            const int i = i0 + p * TPB + tid;
+            if (masks[p] == 0) continue;
+            const int q0 = a.pl_indptr[i], q1 = a.pl_indptr[i + 1];
+            if (q1 - q0 > PL_SERIAL) {
+                rem[p] = masks[p];
+                more = true;
+                continue;
+            }
+            for (uint32_t m = masks[p]; m; m &= m - 1) {
+                int* r = a.ring + ((size_t)b * a.ring_slots + (rs0 + __ffs(m) - 1 + nd) % a.ring_slots) * a.N;
+                for (int q = q0; q < q1; q++) {
                     const int val = (int)rintf(pwb[q] * a.pw_qscale);
                     if (val != 0) atomicAdd(r + a.pl_post[q], val);
                 }
             }
+        }
+        while (__syncthreads_or(more)) {  // also: the previous round's staged rows are read
+            int beg[NPT], len[NPT], kk[NPT];
+            int sum = 0;
+#pragma unroll
+            for (int p = 0; p < NPT; p++) {
+                beg[p] = len[p] = kk[p] = 0;
+                if (rem[p]) {
+                    const int i = i0 + p * TPB + tid;
+                    kk[p] = __ffs(rem[p]) - 1;
+                    rem[p] &= rem[p] - 1;
+                    beg[p] = a.pl_indptr[i];
+                    len[p] = a.pl_indptr[i + 1] - beg[p];
+                }
+                sum += len[p];
+            }
+            int tot;
+            int run = block_incl_scan(sum, s_wsum, &tot) - sum;
+#pragma unroll
+            for (int p = 0; p < NPT; p++) {  // thread-major, the order of the prefix sum
+                run += len[p];
+                p_beg[tid * NPT + p] = beg[p];
+                p_end[tid * NPT + p] = run;
+                p_k[tid * NPT + p] = kk[p];
+            }
+            __syncthreads();
+            for (int q = tid; q < tot; q += TPB) {
+                int lo = 0, hi = TS - 1;  // first staged row with p_end > q
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (p_end[mid] > q) hi = mid;
+                    else lo = mid + 1;
+                }
+                const int j = p_beg[lo] + (q - (lo ? p_end[lo - 1] : 0));
+                const int val = (int)rintf(pwb[j] * a.pw_qscale);
+                if (val != 0) {
+                    int* r = a.ring + ((size_t)b * a.ring_slots + (rs0 + p_k[lo] + nd) % a.ring_slots) * a.N;
+                    atomicAdd(r + a.pl_post[j], val);
+                }
+            }
+            more = false;
+#pragma unroll
+            for (int p = 0; p < NPT; p++) more = more || rem[p] != 0;
         }
     }
 }
