@@ -251,6 +251,32 @@ def instruct_line(fly, p, torch, B=64, frames=100, warmup=3):
     return res
 
 
+def eye_line(fly, torch, B=256, frames=50, shape=(224, 256)):
+    """The batched motion-vision front end (flyzero/vision.py:137-191) on B game-sized RGB frames
+    per step (8 distinct random frames per fly staged in HBM), timed with HIP events."""
+    from flyhip.eye import BatchMotionEye
+
+    eye = BatchMotionEye.for_fly(fly, shape, B)
+    rng = np.random.default_rng(6)
+    movie = torch.from_numpy(rng.integers(0, 256, (8, B) + shape + (3,)).astype(np.uint8)).cuda()
+    for f in range(3):
+        eye.see(movie[f % 8])
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for f in range(frames):
+        eye.see(movie[f % 8])
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / frames
+    G = (shape[0] // eye.p.cell) * (shape[1] // eye.p.cell)
+    nbytes = B * (shape[0] * shape[1] * 3 + 6 * 4 * G + 4 * len(eye.idx))   # frame, state r+w, rates
+    res = {"B": B, "frame": list(shape), "t4t5_cells": int(len(eye.idx)), "ms_per_step": round(ms, 4),
+           "fly_frames_per_s": round(B / (ms / 1000.0)), "hbm_gbs_algorithmic": round(nbytes / (ms / 1000.0) / 1e9, 1)}
+    del eye, movie
+    torch.cuda.empty_cache()
+    return res
+
+
 def copy_bandwidth(torch, nbytes=1 << 30, reps=5):
     """A STREAM-like device-to-device copy (bytes read + written per second), for context."""
     x = torch.ones(nbytes // 4, dtype=torch.int32, device="cuda")
@@ -455,6 +481,10 @@ def main():
             f"rule {instruct['instruct_step_ms']} ms, {instruct['dn_synapses']} DN + {instruct['deep_synapses']} "
             f"deep synapses")
 
+    # ---- motion-vision front end at B = 256 ---------------------------------------------------
+    eye = eye_line(fly, torch)
+    log(f"eye B=256: {eye['ms_per_step']} ms per step, {eye['fly_frames_per_s']} fly-frames/s")
+
     # ---- CPU baseline (oracle, one thread) --------------------------------------------------
     cpu = None
     if not args.no_cpu:
@@ -501,7 +531,7 @@ def main():
                        "connections": int(fly.weights.nnz), "batch": 1, "steps_per_frame": steps_per_frame,
                        "engine": best["engine"], "parallelism": "single GPU"},
             "roofline": best["roofline"], "cpu_baseline": cpu, "batched": batched, "plastic": plastic,
-            "instruct": instruct,
+            "instruct": instruct, "eye": eye,
             "b1_candidates": [{k: r[k] for k in ("engine", "value", "ms_per_frame")} for r in lines1]}
     print(json.dumps(line), flush=True)
 

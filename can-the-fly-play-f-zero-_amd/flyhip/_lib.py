@@ -87,6 +87,7 @@ SIGNATURES = {
     "fz_instruct_delta": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p,
                                   c_int, c_void_p, c_int, ctypes.c_float, c_void_p, c_void_p]),
     "fz_instruct_update": (c_int, [c_void_p, c_void_p]),
+    "fz_eye_step": (c_int, [c_void_p, c_void_p]),
     "fz_instruct_bias": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
                                  c_void_p, c_int, ctypes.c_float, c_void_p, ctypes.c_float, c_void_p, c_void_p,
                                  c_int, c_void_p, c_void_p]),
@@ -140,3 +141,12 @@ def ptr(t) -> int | None:
 
 def stream_handle(device=None) -> int:
     return torch.cuda.current_stream(device).cuda_stream
+
+
+class FzEyeArgs(ctypes.Structure):
+    """fz_eye_args of include/flyhip.h."""
+    _fields_ = [(f, c_int) for f in ("B", "h", "w", "cell", "gh", "gw", "shift", "n_cells")] + \
+               [(f, c_void_p) for f in ("first", "frames", "lp", "d_on", "d_off", "key", "gy", "gx", "py")] + \
+               [(f, ctypes.c_float) for f in ("y_cut", "a_hp", "a_d", "gain", "max_rate")] + \
+               [("rates", c_void_p), ("p_ext", c_void_p), ("n_ext", c_int), ("slot", c_void_p),
+                ("p_scale", ctypes.c_float)]

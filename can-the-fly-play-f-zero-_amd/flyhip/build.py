@@ -16,7 +16,7 @@ CSRC = os.path.join(ROOT, "csrc")
 INCLUDE = os.path.join(REPO, "include")
 LIB_NAME = "libflyhip.so"
 LIB_PATH = os.path.join(PKG_DIR, LIB_NAME)
-SOURCES = ["lif_engine.hip", "plastic.hip", "surrogate.hip"]
+SOURCES = ["lif_engine.hip", "plastic.hip", "surrogate.hip", "eye.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("FLYHIP_ARCH", "gfx950")

@@ -240,6 +240,38 @@ int fz_instruct_bias(int n_flies, int n_dn, const int* fly_ptr, const int* fly_s
                      int n_upost, float eta_bias, const float* ratio, float limit, float* ib, float* bias_ext,
                      int n_ext, const int* dn_slot, void* stream);
 
+/* ---- batched motion-vision front end (replaces MotionEye.see, flyzero/vision.py:137-191, run per
+ * emulator worker on the CPU, flyzero/pool.py:41-44; default parameters: motion channels only) ---- */
+typedef struct fz_eye_args {
+    int B;                       /* flies (one frame each) */
+    int h, w;                    /* frame size, RGB uint8 [B][h][w][3] */
+    int cell, gh, gw;            /* pixels per grid point; gh = h / cell, gw = w / cell */
+    int shift;                   /* grid points between correlator arms */
+    int n_cells;                 /* T4/T5 neurons */
+    const unsigned char* first;  /* [B] 1: the fly's filters start from this frame (lp = lum,
+                                  * delayed = 0), or NULL */
+    const unsigned char* frames; /* [B][h][w][3] */
+    float* lp;                   /* [B][gh][gw] low-pass of the luminance (state) */
+    float* d_on;                 /* [B][gh][gw] delayed ON signal (state) */
+    float* d_off;                /* [B][gh][gw] delayed OFF signal (state) */
+    const int* key;              /* [n_cells] polarity * 4 + screen direction (0 right, 1 left, 2 down, 3 up) */
+    const int* gy;               /* [n_cells] grid row */
+    const int* gx;               /* [n_cells] grid column */
+    const int* py;               /* [n_cells] screen row, pixels */
+    float y_cut;                 /* rows above y_min * h give 0 */
+    float a_hp, a_d;             /* 1 / tau_hp, 1 / tau_delay (frames) */
+    float gain, max_rate;        /* rate = max_rate * clip(gain * motion, 0, 1) */
+    float* rates;                /* [B][n_cells] Hz */
+    float* p_ext;                /* optional [B][n_ext]: p_ext[b][slot[n]] = rate * p_scale */
+    int n_ext;
+    const int* slot;             /* [n_cells] ext slot of each neuron */
+    float p_scale;               /* dt / 1000 */
+} fz_eye_args;
+
+/* One frame per fly: grid luminance, ON / OFF split, Hassenstein-Reichardt correlators and the
+ * T4/T5 rates (and, with p_ext, the brain's per-step input probabilities, flyzero/fleet.py:76-80). */
+int fz_eye_step(const fz_eye_args* a, void* stream);
+
 /* ---- learned rate surrogate: BF16 MFMA GEMM and training / inference element-wise kernels ---- */
 
 #define FZ_SG_STORE32 0 /* C32 = A B (+ bias) */
