@@ -7,6 +7,8 @@
   ``rint(w)`` (``pw = count * w_syn``), so the counts the brain sees change in whole synapses.
 * ``DNReadout``: smoothed descending-neuron rates for the 8 groups of flyzero/instruct.py:43-48
   (the rate half of ``BatchMotor.update``, flyzero/batch.py:107-111), on the device.
+* ``BatchMotor``: the whole readout (flyzero/batch.py:77-138): the DN rates of the 9 READOUT
+  groups on the device, then button dicts (with the fleet's sigma-delta taps) on the host.
 """
 
 from __future__ import annotations
@@ -173,3 +175,88 @@ class DNReadout:
                                            _lib.ptr(self.d_idx), hz, alpha, self.rates.data_ptr(),
                                            _lib.stream_handle(self.rates.device)), "fz_dn_rates")
         return self.rates
+
+
+BUTTONS = ["B", "Y", "SELECT", "START", "UP", "DOWN", "LEFT", "RIGHT", "A", "X", "L", "R"]
+READOUT = ["steer_left", "steer_right", "lean_left", "lean_right", "wing_left", "wing_right", "accelerate", "brake",
+           "boost"]
+
+
+@dataclass
+class MotorParams:
+    """flyzero/motor.py:39-66 (same names and defaults)."""
+    tau_ms: float = 80.0
+    steer_threshold: float = 10.0
+    steer_bias: float = 0.0
+    lean_threshold: float = 10.0
+    accel_threshold: float = 5.0
+    brake_threshold: float = 15.0
+    boost_threshold: float = 1.0
+    taps: bool = False
+    steer_span: float = 150.0
+    lean_span: float = 70.0
+    groups: dict = None
+
+    def __post_init__(self):
+        if self.groups is None:
+            self.groups = {"steer_left": ("DNa02", "left"), "steer_right": ("DNa02", "right"),
+                           "lean_left": ("DNa01", "left"), "lean_right": ("DNa01", "right"),
+                           "wing_left": ("DNg02*", "left"), "wing_right": ("DNg02*", "right"),
+                           "accelerate": ("DNp09", None), "brake": ("MDN", None), "boost": ("DNp01", None)}
+
+
+class BatchMotor:
+    """``MotorReadout`` for every slot (flyzero/batch.py:77-138): smoothed DN rates on the device
+    (the ``dn_rates`` kernel over the READOUT groups), then button dicts on the host, with the
+    sigma-delta taps of the GPU fleet when ``params.taps``."""
+
+    def __init__(self, conn, batch: int, params: MotorParams | None = None, device=None):
+        self.p = p = params or MotorParams()
+        self.B = batch
+        self.readout = DNReadout(conn, batch, groups={k: p.groups[k] for k in READOUT}, tau_ms=p.tau_ms,
+                                 device=device)
+        self.size = np.array([len(conn.find(*p.groups[k])) for k in READOUT], np.float32)
+        self.rates = np.zeros((batch, len(READOUT)), np.float32)
+        self.acc = np.zeros((batch, 2))   # sigma-delta accumulators for taps (steer, lean)
+
+    def reset(self, slots=None):
+        self.readout.reset(slots)
+        if slots is None:
+            self.rates[:] = 0
+            self.acc[:] = 0
+        else:
+            self.rates[np.atleast_1d(slots)] = 0
+            self.acc[np.atleast_1d(slots)] = 0
+
+    def update(self, counts: torch.Tensor, window_ms: float) -> list:
+        """Rates from the window's (B, n) device counts, then one button dict per slot (a host copy)."""
+        self.rates = self.readout.update(counts, window_ms).cpu().numpy()
+        return [self.buttons(r, k) for k, r in enumerate(self.rates)]
+
+    def _tap(self, k, j, x, threshold, span):
+        """+1 (left), -1 (right) or 0 this frame; duty = (|x| - threshold) / span (flyzero/batch.py:114-120)."""
+        duty = np.sign(x) * np.clip((abs(x) - threshold) / span, 0.0, 1.0)
+        self.acc[k, j] += duty
+        t = 1 if self.acc[k, j] >= 0.5 else -1 if self.acc[k, j] <= -0.5 else 0
+        self.acc[k, j] -= t
+        return t
+
+    def buttons(self, r, k: int = 0) -> dict:
+        """flyzero/batch.py:122-138."""
+        p = self.p
+        steer = (r[0] - r[1]) + (r[5] - r[4]) + p.steer_bias
+        lean = r[2] - r[3]
+        b = {key: False for key in BUTTONS}
+        if p.taps:
+            s = self._tap(k, 0, steer, p.steer_threshold, p.steer_span)
+            le = self._tap(k, 1, lean, p.lean_threshold, p.lean_span)
+            b["LEFT"], b["RIGHT"], b["L"], b["R"] = s > 0, s < 0, le > 0, le < 0
+        else:
+            b["LEFT"] = bool(steer > p.steer_threshold)
+            b["RIGHT"] = bool(steer < -p.steer_threshold)
+            b["L"] = bool(lean > p.lean_threshold)
+            b["R"] = bool(lean < -p.lean_threshold)
+        b["B"] = bool(r[6] > p.accel_threshold)
+        b["Y"] = bool(r[7] > p.brake_threshold)
+        b["A"] = bool(r[8] > p.boost_threshold)
+        return b

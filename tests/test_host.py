@@ -106,3 +106,30 @@ def test_smooth_field_is_bounded_and_drifts():
     assert f0.dtype == np.float32 and ((f0 >= 0) & (f0 <= 200)).all()
     assert np.array_equal(f0, smooth_field(xy, 0))
     assert not np.array_equal(f0, f1)
+
+
+def test_motor_buttons_thresholds_and_taps():
+    # flyzero/batch.py:114-138 (a BatchMotor without a device: buttons() and _tap() are host logic)
+    from flyhip.batch import READOUT, BatchMotor, MotorParams
+
+    m = BatchMotor.__new__(BatchMotor)
+    m.p = MotorParams()
+    m.acc = np.zeros((1, 2))
+    r = np.zeros(len(READOUT), np.float32)
+    r[READOUT.index("steer_left")] = 30.0           # left turn
+    r[READOUT.index("wing_left")] = 5.0              # wings push the other way by 5 Hz
+    r[READOUT.index("accelerate")] = 6.0
+    r[READOUT.index("boost")] = 0.5
+    b = m.buttons(r)
+    assert b["LEFT"] and not b["RIGHT"] and b["B"] and not b["A"] and not b["Y"]
+    r[READOUT.index("steer_left")] = 14.0            # 14 - 5 = 9 Hz: below the 10 Hz threshold
+    assert not m.buttons(r)["LEFT"]
+    # taps: duty (|x| - 10) / 150 accumulates; a tap fires each time the accumulator reaches 0.5
+    m.p = MotorParams(taps=True)
+    r[:] = 0
+    r[READOUT.index("steer_left")] = 85.0            # duty 0.5: a tap every other frame
+    taps = [m.buttons(r)["LEFT"] for _ in range(6)]
+    assert taps == [True, False, True, False, True, False]
+    r[:] = 0
+    r[READOUT.index("lean_right")] = 80.0            # duty -1: lean right every frame
+    assert all(m.buttons(r)["R"] for _ in range(3))

@@ -286,3 +286,26 @@ def test_instructed_learning_drives_a_group_towards_its_target(fly):
     onto_gas = np.isin(gb.pl_post, fly.find("DNp09"))
     exc = onto_gas & (bi.sign_np > 0)
     assert (w[0, exc] >= w[1, exc]).all() and (w[0, exc] > w[1, exc]).any()
+
+
+@pytest.mark.gpu
+def test_batch_motor_rates_and_buttons(fly):
+    # flyzero/batch.py:77-138: the 9 readout groups (the giant fiber drives A) from device counts
+    import torch
+
+    from flyhip.batch import READOUT, BatchMotor, MotorParams
+
+    B = 3
+    motor = BatchMotor(fly, B, MotorParams(boost_threshold=0.0))
+    rng = np.random.default_rng(2)
+    ema = np.zeros((B, len(READOUT)), f32)
+    a = f32(1.0 - np.exp(-16.6 / 80.0))
+    for t in range(4):
+        c = rng.integers(0, 3, (B, fly.n)).astype(np.int32)
+        buttons = motor.update(torch.from_numpy(c).cuda(), 16.6)
+        for k, name in enumerate(READOUT):
+            ids = fly.find(*motor.p.groups[name])
+            inst = c[:, ids].sum(1).astype(f32) / f32(len(ids)) * f32(1000.0 / 16.6)
+            ema[:, k] = ema[:, k] + a * (inst - ema[:, k])
+        np.testing.assert_allclose(motor.rates, ema, rtol=1e-5)
+        assert len(buttons) == B and all(b["A"] == (motor.rates[i, 8] > 0.0) for i, b in enumerate(buttons))
