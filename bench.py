@@ -277,6 +277,56 @@ def eye_line(fly, torch, B=256, frames=50, shape=(224, 256)):
     return res
 
 
+def fleet_line(fly, torch, B=256, frames=60, warmup=3):
+    """The closed loop of flyzero/fleet.py:87-127 at B flies: each frame the eye kernel turns every
+    fly's 224 x 256 view into the brain's input probabilities, one window of brain time (dt =
+    0.25 ms, plastic synapses onto the motor DNs), the DN readout and the host's button logic,
+    then every game steps.  The games are the synthetic road game rendered on the device; the
+    copy an emulator pool's host frames would add is timed on its own."""
+    from flyhip.fleet import Fleet, SyntheticRace
+
+    race = SyntheticRace(B, device="cuda")
+    fleet = Fleet(race, B, seed=3, conn=fly, tile=1024, tune=True)
+    start = SyntheticRace.start_state(v=0.1)
+    fleet.exam(start, warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = fleet.exam(start, frames, retire=False)
+    torch.cuda.synchronize()
+    loop_s = time.perf_counter() - t0
+    # the brain alone on the same inputs (graph replays of one window)
+    fleet.brain.run(fleet.window)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        fleet.brain.run(fleet.window, copy=False)
+    torch.cuda.synchronize()
+    brain_s = time.perf_counter() - t0
+    # host frames of an emulator pool: pinned staging + one copy per frame
+    host = race.render().cpu().numpy()
+    fleet.frames_on_device(host)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(10):
+        fleet.frames_on_device(host)
+    torch.cuda.synchronize()
+    h2d_ms = (time.perf_counter() - t0) / 10 * 1000.0
+    from flyhip.fleet import summarize
+
+    summary = summarize(res)
+    out = {"B": B, "frames": frames, "neurons": fly.n, "eye_cells": int(len(fleet.eye.idx)),
+           "plastic_synapses": int(len(fleet.pos)), "steps_per_frame": fleet.brain.steps_of(fleet.window),
+           "ms_per_frame": round(loop_s / frames * 1000.0, 3),
+           "fly_frames_per_s": round(B * frames / loop_s, 1),
+           "fly_sim_s_per_s": round(B * frames / loop_s * fleet.window / 1000.0, 2),
+           "brain_ms_per_frame": round(brain_s / frames * 1000.0, 3),
+           "host_frames_copy_ms": round(h2d_ms, 3), "mean_progress": summary["mean_progress"]}
+    fleet.close()
+    del fleet, race
+    torch.cuda.empty_cache()
+    return out
+
+
 def copy_bandwidth(torch, nbytes=1 << 30, reps=5):
     """A STREAM-like device-to-device copy (bytes read + written per second), for context."""
     x = torch.ones(nbytes // 4, dtype=torch.int32, device="cuda")
@@ -405,6 +455,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample length")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plastic-frames", type=int, default=300, help="frames of the B=64 plasticity line (0: skip)")
+    ap.add_argument("--fleet-frames", type=int, default=60, help="frames of the B=256 fleet line (0: skip)")
     ap.add_argument("--instruct-frames", type=int, default=100,
                     help="frames of the B=64 instructed-learning line (0: skip)")
     ap.add_argument("--profile", default="", help="counter-run mode: engine:B:upd_threads:block_steps:tile_stage[:tile_source]")
@@ -485,6 +536,13 @@ def main():
     eye = eye_line(fly, torch)
     log(f"eye B=256: {eye['ms_per_step']} ms per step, {eye['fly_frames_per_s']} fly-frames/s")
 
+    # ---- the closed loop (eye -> brain -> readout -> buttons -> games) at B = 256 --------------
+    fleet = None
+    if args.fleet_frames > 0:
+        fleet = fleet_line(fly, torch, B=256, frames=args.fleet_frames)
+        log(f"fleet B=256: {fleet['fly_frames_per_s']} fly-frames/s ({fleet['ms_per_frame']} ms/frame; "
+            f"brain alone {fleet['brain_ms_per_frame']} ms; host-frame copy {fleet['host_frames_copy_ms']} ms)")
+
     # ---- CPU baseline (oracle, one thread) --------------------------------------------------
     cpu = None
     if not args.no_cpu:
@@ -531,7 +589,7 @@ def main():
                        "connections": int(fly.weights.nnz), "batch": 1, "steps_per_frame": steps_per_frame,
                        "engine": best["engine"], "parallelism": "single GPU"},
             "roofline": best["roofline"], "cpu_baseline": cpu, "batched": batched, "plastic": plastic,
-            "instruct": instruct, "eye": eye,
+            "instruct": instruct, "eye": eye, "fleet": fleet,
             "b1_candidates": [{k: r[k] for k in ("engine", "value", "ms_per_frame")} for r in lines1]}
     print(json.dumps(line), flush=True)
 

@@ -149,4 +149,5 @@ class FzEyeArgs(ctypes.Structure):
                [(f, c_void_p) for f in ("first", "frames", "lp", "d_on", "d_off", "key", "gy", "gx", "py")] + \
                [(f, ctypes.c_float) for f in ("y_cut", "a_hp", "a_d", "gain", "max_rate")] + \
                [("rates", c_void_p), ("p_ext", c_void_p), ("n_ext", c_int), ("slot", c_void_p),
-                ("p_scale", ctypes.c_float)]
+                ("p_scale", ctypes.c_float), ("tr_gain", ctypes.c_float), ("su_gain", ctypes.c_float),
+                ("ch_gain", ctypes.c_float), ("surround", c_int)]

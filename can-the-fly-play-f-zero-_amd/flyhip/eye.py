@@ -1,12 +1,13 @@
-"""Batched motion-vision front end on the GPU: a drop-in for ``MotionEye`` (flyzero/vision.py:64-191)
+"""Batched vision front end on the GPU: a drop-in for ``MotionEye`` (flyzero/vision.py:64-191)
 for many flies at once.
 
 The reference runs one ``MotionEye`` per emulator worker on the CPU (flyzero/pool.py:41-44) and
 copies the rates into the brain's input probabilities every frame (flyzero/fleet.py:76-80).
 Here every fly's frame goes through one kernel (``fz_eye_step``): grid luminance, the ON / OFF
-split against a running low-pass, Hassenstein-Reichardt correlators and the T4/T5 rates, which
-can be written straight into ``BatchBrain.p_ext``.  The default ``MotionParams`` (motion
-channels only) are supported; the optional medulla and colour channels are not.
+split against a running low-pass, Hassenstein-Reichardt correlators for the T4/T5 cells, the
+medulla transient / sustained-contrast channels and the R7/R8 colour channels, with the rates
+written straight into ``BatchBrain.p_ext``.  Cells are placed on the screen by the retinotopy of
+flyzero/eyes.py:37-88 (FlyWire) or by the synthetic fly's own layout.
 """
 
 from __future__ import annotations
@@ -41,6 +42,24 @@ class MotionParams:
 SUBTYPES = {"a": "front_to_back", "b": "back_to_front", "c": "up", "d": "down"}
 POLARITIES = {"T4": "on", "T5": "off"}
 DIRECTIONS = ("right", "left", "down", "up")   # kernel key = polarity * 4 + direction
+# medulla columnar inputs and what they encode (flyzero/vision.py:64-71), colour photoreceptors
+COLUMNAR = {"Mi1": "on_transient", "Tm3": "on_transient", "Tm1": "off_transient", "Tm2": "off_transient",
+            "Tm4": "off_transient", "Mi4": "on_sustained", "Mi9": "off_sustained", "Tm9": "off_sustained"}
+COLOUR = {"R7": "blue", "R8": "green"}
+# kernel keys of the feature channels (FZ_EYE_* of include/flyhip.h)
+FEATURE_KEYS = {"on_transient": 8, "off_transient": 9, "on_sustained": 10, "off_sustained": 11, "blue": 12,
+                "green": 13}
+
+
+def cell_key(kind: str, side: str) -> int:
+    """Kernel key of a cell: T4a .. T5d -> polarity * 4 + screen direction; medulla and colour
+    cell types -> their feature channel."""
+    if kind in COLUMNAR:
+        return FEATURE_KEYS[COLUMNAR[kind]]
+    if kind in COLOUR:
+        return FEATURE_KEYS[COLOUR[kind]]
+    pol = 0 if POLARITIES[kind[:2]] == "on" else 1
+    return pol * 4 + DIRECTIONS.index(screen_direction(SUBTYPES[kind[2]], side))
 
 
 def screen_direction(direction: str, side: str) -> str:
@@ -65,7 +84,80 @@ def field_to_screen(u, v, side, w: int, h: int, overlap: float):
     return px, py
 
 
-def synthetic_motion_cells(fly, frame_shape, overlap: float = 0.1, seed: int = 9):
+def _field(pos: np.ndarray, mid: float) -> np.ndarray:
+    """Visual-field coordinates of one eye's cells (flyzero/eyes.py:72-88): the two principal axes
+    of the point cloud; v is the axis best aligned with brain y (oriented so v grows with y,
+    which grows ventrally), u the other (oriented so u grows laterally); each scaled to 0..1."""
+    c = pos - pos.mean(0)
+    _, _, vt = np.linalg.svd(c, full_matrices=False)
+    uv = c @ vt[:2].T
+    cy = [abs(np.corrcoef(uv[:, k], pos[:, 1])[0, 1]) for k in range(2)]
+    v_k = int(np.argmax(cy))
+    u, v = uv[:, 1 - v_k], uv[:, v_k]
+    if np.corrcoef(v, pos[:, 1])[0, 1] < 0:
+        v = -v
+    if np.corrcoef(u, np.abs(pos[:, 0] - mid))[0, 1] < 0:
+        u = -u
+
+    def unit(a):
+        return (a - a.min()) / max(np.ptp(a), 1e-9)
+
+    return np.c_[unit(u), unit(v)]
+
+
+def retinotopy(conn, cells, w: int, h: int, overlap: float = 0.1):
+    """Screen pixel (px, py), eye and a placed-mask for each neuron in ``cells`` of a connectome
+    with positions (flyzero/eyes.py:37-69): per eye, the neurons' positions are mapped to the
+    visual field by ``_field``; a missing side label falls back to the side of the median x
+    (x above the median: left).  Eyes with fewer than 3 placed cells are not placed."""
+    cells = np.asarray(cells, np.int64)
+    nrn = conn.neurons.iloc[cells]
+    pos = nrn[["pos_x", "pos_y", "pos_z"]].to_numpy(float)
+    side = nrn["side"].fillna("").to_numpy().astype(object)
+    ok = np.isfinite(pos).all(1)
+    mid = np.nanmedian(conn.neurons["pos_x"].to_numpy(float))
+    unknown = ~np.isin(side, ["left", "right"])
+    side = np.where(unknown, np.where(pos[:, 0] > mid, "left", "right"), side).astype(str)
+    uv = np.full((len(cells), 2), np.nan)
+    for s in ("left", "right"):
+        m = (side == s) & ok
+        if m.sum() >= 3:
+            uv[m] = _field(pos[m], mid)
+    keep = np.isfinite(uv).all(1)
+    uv = np.nan_to_num(uv)
+    px, py = field_to_screen(uv[:, 0], uv[:, 1], side, w, h, overlap)
+    return px, py, side, keep
+
+
+def motion_cells(conn, frame_shape, params: "MotionParams | None" = None):
+    """The eye's cells of a connectome in ``MotionEye``'s order (flyzero/vision.py:67-103): T4a-d
+    then T5a-d, R7 and R8 with the colour channel, the medulla columnar types with the
+    transient or sustained channel.  Returns (idx, px, py, side, kind)."""
+    p = params or MotionParams()
+    h, w = frame_shape
+    kinds = [pol + sub for pol in POLARITIES for sub in SUBTYPES]
+    if p.chromatic_gain > 0:
+        kinds += list(COLOUR)
+    if p.transient_gain > 0 or p.sustained_gain > 0:
+        kinds += list(COLUMNAR)
+    out = [[], [], [], [], []]
+    for kind in kinds:
+        cells = conn.find(kind)
+        if len(cells) == 0:
+            continue
+        px, py, side, keep = retinotopy(conn, cells, w, h, p.overlap)
+        for lst, a in zip(out, (cells[keep], px[keep], py[keep], side[keep], np.full(keep.sum(), kind))):
+            lst.append(a)
+    if not out[0] or not any(k[:2] in POLARITIES for k in np.concatenate(out[4])):
+        raise ValueError("connectome has no T4/T5 neurons")
+    return tuple(np.concatenate(lst) for lst in out)
+
+
+# FlyWire cell counts standing in for the extra channels of the synthetic fly (both sides)
+SYNTHETIC_COUNTS = {"R7": 1500, "R8": 1500, **{k: 1600 for k in COLUMNAR}}
+
+
+def synthetic_motion_cells(fly, frame_shape, overlap: float = 0.1, seed: int = 9, params=None):
     """T4/T5 cells of the synthetic fly: its visual input neurons, each given a subtype (T4/T5 x
     a-d) from a seeded draw and an eye from its retinotopic x (x < 0.5: left); the visual-field
     coordinates are u = |2 x - 1| (0 medial .. 1 lateral) and v = y, standing in for the principal
@@ -79,17 +171,40 @@ def synthetic_motion_cells(fly, frame_shape, overlap: float = 0.1, seed: int = 9
     kind = kinds[rng.integers(0, len(kinds), len(idx))]
     side = np.where(xy[:, 0] < 0.5, "left", "right")
     px, py = field_to_screen(np.abs(2 * xy[:, 0] - 1), xy[:, 1], side, w, h, overlap)
+    extra = []
+    if params is not None and params.chromatic_gain > 0:
+        extra += list(COLOUR)
+    if params is not None and (params.transient_gain > 0 or params.sustained_gain > 0):
+        extra += list(COLUMNAR)
+    if extra:
+        # further cells, drawn from the neurons that are neither inputs nor descending neurons,
+        # at FlyWire's counts scaled to the fly's size, uniformly placed
+        from .connectome import DN_TYPES
+
+        taken = set(idx.tolist()) | {int(fly.drive_idx)}
+        for t in DN_TYPES:
+            taken |= set(np.asarray(fly.find(t)).tolist())
+        free = np.setdiff1d(np.arange(fly.weights.shape[0]), np.fromiter(taken, np.int64))
+        sizes = [max(2, round(SYNTHETIC_COUNTS[k] * fly.weights.shape[0] / 138_639)) for k in extra]
+        pick = rng.choice(free, size=min(sum(sizes), len(free)), replace=False)
+        ekind = np.repeat(extra, sizes)[:len(pick)]
+        exy = rng.random((len(pick), 2))
+        eside = np.where(exy[:, 0] < 0.5, "left", "right")
+        epx, epy = field_to_screen(np.abs(2 * exy[:, 0] - 1), exy[:, 1], eside, w, h, overlap)
+        idx, px, py = np.r_[idx, pick], np.r_[px, epx], np.r_[py, epy]
+        side, kind = np.r_[side, eside], np.r_[kind, ekind]
     return idx, px, py, side, kind
 
 
 class BatchMotionEye:
-    """``MotionEye`` for B flies: ``idx`` (the T4/T5 neurons), ``see(frames) -> rates``."""
+    """``MotionEye`` for B flies: ``idx`` (the eye's neurons), ``see(frames) -> rates``."""
 
     def __init__(self, cells, frame_shape, batch: int, params: MotionParams | None = None, device=None):
-        """``cells`` = (idx, px, py, side, kind) as ``synthetic_motion_cells`` returns (kind: T4a .. T5d)."""
+        """``cells`` = (idx, px, py, side, kind) as ``motion_cells`` / ``synthetic_motion_cells``
+        return (kind: T4a .. T5d, a ``COLUMNAR`` or ``COLOUR`` cell type)."""
         self.p = p = params or MotionParams()
-        if p.transient_gain or p.sustained_gain or p.chromatic_gain:
-            raise ValueError("the medulla and colour channels are not supported (MotionParams defaults only)")
+        if (p.transient_gain or p.sustained_gain or p.chromatic_gain) and not p.gain > 0:
+            raise ValueError("the feature channels need a positive gain")
         self.h, self.w = frame_shape
         self.gh, self.gw = self.h // p.cell, self.w // p.cell
         self.B = batch
@@ -98,11 +213,8 @@ class BatchMotionEye:
         self.idx = np.asarray(idx, np.int64)
         self.px, self.py = np.asarray(px, np.int64), np.asarray(py, np.int64)
         self.side = np.asarray(side)
-        key = np.empty(len(self.idx), np.int32)
-        for n, (k, s) in enumerate(zip(kind, self.side)):
-            pol = 0 if POLARITIES[k[:2]] == "on" else 1
-            key[n] = pol * 4 + DIRECTIONS.index(screen_direction(SUBTYPES[k[2]], s))
-        self.key = key
+        self.kind = np.asarray(kind).astype(str)
+        self.key = np.array([cell_key(k, s) for k, s in zip(self.kind, self.side)], np.int32)
         self.gx = np.clip(self.px // p.cell, 0, self.gw - 1).astype(np.int32)
         self.gy = np.clip(self.py // p.cell, 0, self.gh - 1).astype(np.int32)
         t = lambda a: torch.from_numpy(np.ascontiguousarray(a).astype(np.int32)).to(dev)  # noqa: E731
@@ -117,8 +229,14 @@ class BatchMotionEye:
 
     @classmethod
     def for_fly(cls, fly, frame_shape, batch: int, params: MotionParams | None = None, device=None):
+        """The eye of a synthetic fly (``flywire_like``) or of a connectome with positions and cell
+        types (``flywire.Connectome``)."""
         p = params or MotionParams()
-        return cls(synthetic_motion_cells(fly, frame_shape, p.overlap), frame_shape, batch, p, device)
+        if hasattr(fly, "input_xy"):
+            cells = synthetic_motion_cells(fly, frame_shape, p.overlap, params=p)
+        else:
+            cells = motion_cells(fly, frame_shape, p)
+        return cls(cells, frame_shape, batch, p, device)
 
     def reset(self, slots=None):
         """The given flies' filters restart from their next frame (a respawned emulator)."""
@@ -128,7 +246,7 @@ class BatchMotionEye:
             self.first[torch.as_tensor(np.atleast_1d(slots), device=self.dev)] = 1
 
     def see(self, frames: torch.Tensor, brain=None) -> torch.Tensor:
-        """One frame per fly: ``frames`` uint8 (B, h, w, 3) on the device.  Returns the (B, n) T4/T5
+        """One frame per fly: ``frames`` uint8 (B, h, w, 3) on the device.  Returns the (B, n) cell
         rates (Hz); with ``brain``, also writes them into ``brain.p_ext`` as per-step
         probabilities (rate * dt / 1000, flyzero/fleet.py:76-80)."""
         if frames.dtype != torch.uint8 or tuple(frames.shape) != (self.B, self.h, self.w, 3):
@@ -145,7 +263,8 @@ class BatchMotionEye:
                            frames.data_ptr(), P(self.lp), P(self.d_on), P(self.d_off), P(self.d_key), P(self.d_gy),
                            P(self.d_gx), P(self.d_py), np.float32(p.y_min * self.h), np.float32(1.0 / p.tau_hp),
                            np.float32(1.0 / p.tau_delay), np.float32(p.gain), np.float32(p.max_rate), P(self.rates),
-                           p_ext, n_ext, slot, scale)
+                           p_ext, n_ext, slot, scale, np.float32(p.transient_gain), np.float32(p.sustained_gain),
+                           np.float32(p.chromatic_gain), int(p.surround))
         _lib.check(_lib.load().fz_eye_step(ctypes.byref(a), _lib.stream_handle(self.dev)), "fz_eye_step")
         self.first.zero_()
         return self.rates

@@ -1,0 +1,345 @@
+"""A fleet: B flies on the GPU, each at the controls of its own game (flyzero/fleet.py:22-127).
+
+Glue between ``BatchBrain`` (brains), a game pool (games), ``BatchMotionEye`` (eyes),
+``BatchMotor`` (buttons) and the batched learning rules.  The fly's interface is the reference
+fleet's: the steady bias on the motor neurons, dt = 0.25 ms, one game frame per 16.64 ms of brain
+time, the eye's rates as the brain's Poisson input.
+
+Where the reference keeps one ``MotionEye`` per emulator subprocess and ships its rates back to
+the main process (flyzero/pool.py:41-44), here the pool hands over frames: every fly's frame
+goes through one eye kernel that writes the brain's input probabilities on the device.  A pool
+is any object with
+
+    load(states, flips=None, which=None) -> (frames, infos)
+    step(buttons) -> (frames, infos)
+
+where ``frames`` is uint8 (B, h, w, 3), host (numpy) or device (torch), and ``infos`` one dict per
+game with ``segment``, ``lap``, ``energy``, ``done`` and optionally ``finished`` and ``rank``
+(flyzero/pool.py:133-203, the F-Zero telemetry).  Host frames are staged through pinned memory
+into one device buffer.  ``SyntheticRace`` is such a pool: a deterministic road game with the
+same telemetry, for tests and measurement where no emulator exists.
+"""
+
+from __future__ import annotations
+
+import time
+
+import numpy as np
+import torch
+
+BIAS_TYPES = ("DNa02", "DNg02*", "DNa01", "DNp09")
+WINDOW_MS = 1000.0 / 60.0988
+FRAME_SHAPE = (224, 256)
+# the fleet's eye: motion, medulla and colour channels (flyzero/train.py:21, gain = 10 ** 2.7)
+VISION = dict(log_gain=2.7, y_min=0.2, transient_gain=4.0, sustained_gain=4.0, chromatic_gain=1.0)
+
+
+class Progress:
+    """Unwrapped track progress from the per-frame segment number (flyzero/tune.py:57-71): the
+    segment step is taken modulo the lap, in [-segments / 2, segments / 2)."""
+
+    def __init__(self, segments: int = 59):
+        self.segments = segments
+        self.last = None
+        self.total = 0
+
+    def update(self, seg: int, frame: int) -> int:
+        if self.last is not None:
+            n = self.segments
+            self.total += (seg - self.last + n // 2) % n - n // 2
+        self.last = seg
+        return self.total
+
+
+def summarize(results: list[dict]) -> dict:
+    """flyzero/fleet.py:129-136."""
+    prog = [r["progress"] for r in results]
+    return {"mean_progress": round(float(np.mean(prog)), 1), "best": int(max(prog)), "worst": int(min(prog)),
+            "laps_best": max(r["lap"] for r in results), "finished": sum(r["finished"] for r in results),
+            "laps5": sum(r.get("laps5", False) for r in results), "n": len(results),
+            "mean_frames": int(np.mean([r["frames"] for r in results]))}
+
+
+def buttons_to_mask(pressed: dict) -> np.ndarray:
+    """flyzero/record.py:87-88."""
+    from .batch import BUTTONS
+
+    return np.array([pressed.get(b, False) for b in BUTTONS], np.uint8)
+
+
+def vision_params(vision: dict | None = None):
+    """``MotionParams`` from a ``VISION``-style dict (``log_gain`` -> ``gain``)."""
+    from .eye import MotionParams
+
+    v = dict(VISION if vision is None else vision)
+    if "log_gain" in v:
+        v["gain"] = 10 ** v.pop("log_gain")
+    return MotionParams(**v)
+
+
+class Fleet:
+    def __init__(self, pool, batch: int, seed: int = 0, plastic_types=None, bias_mv: float = 7.8, conn=None,
+                 deep: bool | int = False, taps: bool = False, steer_span: float = 150.0, lean_span: float = 70.0,
+                 steer_threshold: float = 10.0, lean_threshold: float = 10.0, readout_tau: float = 80.0,
+                 vision: dict | None = None, segments: int = 59, engine: str = "tile", tile: int = 1024,
+                 tune: bool = True, device=None):
+        """flyzero/fleet.py:22-68 with ``pool`` (a game pool, see the module docstring) in place of
+        the ROM, ``conn`` (a connectome; default the synthetic FlyWire-sized fly) and ``segments``
+        (per lap, for ``Progress``).  ``deep``: also make the synapses onto the DNs' presynaptic
+        partners plastic (2: one layer further).  ``tune``: pick the brain's launch shapes by
+        timing them (cached per workload)."""
+        from . import BatchBrain, LIFParams
+        from .batch import BatchMotor, MotorParams
+        from .eye import BatchMotionEye
+        from .instruct import PLASTIC_TYPES, deep2_positions, deep_positions, plastic_positions
+
+        if conn is None:
+            from .connectome import flywire_like
+
+            conn = flywire_like()
+        self.conn = conn
+        self.B = batch
+        self.pos = plastic_positions(conn, plastic_types or PLASTIC_TYPES)
+        if deep:
+            dn = self.pos
+            self.pos = np.union1d(dn, deep_positions(conn, dn))
+            if int(deep) >= 2:
+                self.pos = np.union1d(self.pos, deep2_positions(conn, dn))
+        self.brain = BatchBrain(conn.weights, LIFParams(dt=0.25), batch=batch, seed=seed, plastic_pos=self.pos,
+                                engine=engine, tile=tile, device=device)
+        self.device = self.brain.device
+        self.eye = BatchMotionEye.for_fly(conn, FRAME_SHAPE, batch, vision_params(vision), device=self.device)
+        self.motor = BatchMotor(conn, batch, MotorParams(taps=taps, steer_span=steer_span, lean_span=lean_span,
+                                                         steer_threshold=steer_threshold,
+                                                         lean_threshold=lean_threshold, tau_ms=readout_tau),
+                                device=self.device)
+        self.bias_idx = np.concatenate([conn.find(t) for t in BIAS_TYPES])
+        self.bias_mv = bias_mv
+        self.brain.set_bias(self.bias_idx, bias_mv)
+        self.eye_slots = self.brain.slots(self.eye.idx)
+        self.segments = segments
+        self.pool = pool
+        self.window = WINDOW_MS
+        self.extra_idx = np.zeros(0, np.int64)    # optional extra Poisson inputs (exploration, heat)
+        self._stage = None                         # pinned host frames -> device frames
+        if tune:
+            from . import autotune
+
+            autotune.tune(self.brain)
+
+    def close(self):
+        close = getattr(self.pool, "close", None)
+        if close is not None:
+            close()
+
+    def frames_on_device(self, frames) -> torch.Tensor:
+        """uint8 (B, h, w, 3) frames on the brain's device; host frames go through one pinned
+        buffer and one asynchronous copy on the current stream (the buffer is refilled only once
+        the previous copy has finished)."""
+        if isinstance(frames, torch.Tensor) and frames.device == self.device:
+            return frames
+        shape = (self.B, *FRAME_SHAPE, 3)
+        if self._stage is None:
+            self._stage = (torch.empty(shape, dtype=torch.uint8, pin_memory=True),
+                           torch.empty(shape, dtype=torch.uint8, device=self.device), torch.cuda.Event())
+        else:
+            self._stage[2].synchronize()
+        host, dev, copied = self._stage
+        src = frames if isinstance(frames, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(frames))
+        if tuple(src.shape) != shape or src.dtype != torch.uint8:
+            raise ValueError(f"frames must be uint8 {shape}, got {src.dtype} {tuple(src.shape)}")
+        host.copy_(src)
+        dev.copy_(host, non_blocking=True)
+        copied.record()
+        return dev
+
+    def think(self, frames, extra: np.ndarray | None = None) -> torch.Tensor:
+        """One frame of brain time for every slot (flyzero/fleet.py:73-81): the eye turns
+        ``frames`` into the brain's input probabilities, ``extra`` (B, len(extra_idx)) Hz adds
+        to the extra inputs.  Returns the window's (B, n) spike counts on the device."""
+        brain = self.brain
+        if extra is not None and len(self.extra_idx):
+            j = brain.slots(self.extra_idx)
+        brain.p_ext.zero_()
+        self.eye.see(self.frames_on_device(frames), brain=brain)
+        if extra is not None and len(self.extra_idx):
+            brain.p_ext[:, j] += (torch.as_tensor(np.asarray(extra, np.float32), device=self.device) *
+                                  np.float32(brain.p.dt / 1000.0))
+        return brain.run(self.window)
+
+    def reset_slots(self, slots):
+        """Brain, readout and eye filters of the given slots back to their start."""
+        slots = np.atleast_1d(slots)
+        self.brain.reset(slots)
+        self.motor.reset(slots)
+        self.eye.reset(slots)
+
+    def exam(self, state, frames: int, on_frame=None, log_every: int = 0, record: bool = False,
+             retire: bool = True) -> list[dict]:
+        """flyzero/fleet.py:87-127: every slot drives alone from ``state`` (no learning), with its
+        own spiking noise.  ``record``: each result also holds the drive's button masks and DN
+        rates.  ``retire``: a slot whose race has ended stops costing brain time (its results
+        are final; the other slots' spikes are unchanged by it)."""
+        self.brain.reset()
+        self.motor.reset()
+        self.eye.reset()
+        frames_now, infos = self.pool.load([state] * self.B)
+        progs = [Progress(self.segments) for _ in range(self.B)]
+        done = np.zeros(self.B, bool)
+        res = [None] * self.B
+        masks, dn = [[] for _ in range(self.B)], [[] for _ in range(self.B)]
+        t0 = time.time()
+        try:
+            for i in range(frames):
+                counts = self.think(frames_now)
+                buttons = self.motor.update(counts, self.window)
+                if record:
+                    for k in np.flatnonzero(~done):
+                        masks[k].append(buttons_to_mask(buttons[k]))
+                        dn[k].append(self.motor.rates[k].astype(np.float16))
+                frames_now, infos = self.pool.step(buttons)
+                newly = []
+                for k, info in enumerate(infos):
+                    if done[k]:
+                        continue
+                    progs[k].update(info["segment"], i)
+                    if info["done"] or i == frames - 1:
+                        done[k] = True
+                        newly.append(k)
+                        res[k] = {"progress": progs[k].total, "lap": info["lap"], "frames": i + 1,
+                                  "energy": info["energy"], "finished": bool(info.get("finished")),
+                                  "laps5": info["lap"] >= 5, "rank": info.get("rank")}
+                        if record:
+                            res[k]["masks"], res[k]["rates"] = np.array(masks[k]), np.array(dn[k])
+                if retire and newly:
+                    self.brain.alive[torch.as_tensor(newly, device=self.device)] = 0
+                if on_frame:
+                    on_frame(i, counts, buttons, infos)
+                if log_every and (i + 1) % log_every == 0:
+                    print(f"  frame {i + 1}: {(i + 1) * self.B / (time.time() - t0):.0f} fly-frames/s, "
+                          f"progress {[p.total for p in progs]}", flush=True)
+                if done.all():
+                    break
+        finally:
+            self.brain.alive.fill_(1)
+        return res
+
+
+class SyntheticRace:
+    """A deterministic road game for ``n`` players with F-Zero's telemetry, standing in for the
+    emulator pool where no emulator exists.
+
+    The track is a closed loop of ``segments`` segments with curvature
+    kappa(s) = 0.8 sin(4 pi s / S) + 0.4 sin(10 pi s / S + 1).  Per frame a player's speed v
+    (segments per frame) rises by 0.004 with B (up to 0.25), falls by 0.01 with Y and by 0.001
+    otherwise; its lateral offset x (road half-widths, + right) drifts outwards with
+    0.6 v kappa(s) and is steered by 0.05 (0.2 + 4 v) (LEFT - RIGHT + 0.5 (L - R)); off the road
+    (|x| > 1) it loses 0.5 energy and 3 % of its speed; x stays in [-2, 2]; s advances by v.
+    The race ends with no energy, after 5 laps (finished) or at ``max_frames``.
+
+    The view is 224 x 256 RGB: sky above row 84, below it per row y (p = (y - 83) / 140) a
+    road of half-width 0.45 W p + 2 pixels centred at W / 2 - x hw + 60 kappa(s) (1 - p)^2,
+    white edges of max(1, hw / 20) pixels, and grass / road shades alternating with
+    floor(4 s + 8 / p) mod 2, so the road moves under the eye.  Row tables are float64 on the
+    host; the picture is integer selection, drawn with torch on ``device`` (identical bits on
+    every device).  States are bytes, like emulator save states; ``flips`` mirror a player's
+    view only, as the emulator pool does for its eye (flyzero/pool.py:37-41)."""
+
+    SKY = 84
+    PALETTE = np.array([[90, 140, 220], [30, 120, 30], [40, 160, 40], [100, 100, 100], [120, 120, 120],
+                        [250, 250, 250]], np.uint8)
+
+    def __init__(self, n: int, segments: int = 59, max_frames: int | None = None, device="cpu",
+                 frame_shape=FRAME_SHAPE):
+        self.n, self.S, self.max_frames = n, segments, max_frames
+        self.h, self.w = frame_shape
+        self.device = torch.device(device)
+        self.st = np.zeros((n, 5))      # s, x, v, energy, frame_no
+        self.flip = np.zeros(n, bool)
+        self._pal = torch.from_numpy(self.PALETTE).to(self.device)
+        self._cols = torch.arange(self.w, device=self.device, dtype=torch.int32)
+
+    @staticmethod
+    def start_state(s: float = 0.0, x: float = 0.0, v: float = 0.0, energy: float = 100.0) -> bytes:
+        return np.array([s, x, v, energy, 0.0]).tobytes()
+
+    def kappa(self, s):
+        a = 2 * np.pi * np.asarray(s, np.float64) / self.S
+        return 0.8 * np.sin(2 * a) + 0.4 * np.sin(5 * a + 1.0)
+
+    def _infos(self, which):
+        out = []
+        for k in which:
+            s, x, v, e, f = self.st[k]
+            lap = int(s // self.S)
+            finished = lap >= 5
+            done = e <= 0 or finished or (self.max_frames is not None and f >= self.max_frames)
+            out.append({"segment": int(s) % self.S, "lap": lap, "energy": float(e), "done": bool(done),
+                        "finished": bool(finished), "rank": None, "frame": int(f)})
+        return out
+
+    def rows(self, which=None):
+        """Per-row integer tables (left, right, edge, shade) of each player's view, [len(which), rows]."""
+        which = np.arange(self.n) if which is None else np.asarray(which)
+        s, x = self.st[which, 0:1], self.st[which, 1:2]
+        y = np.arange(self.SKY, self.h, dtype=np.float64)[None, :]
+        p = (y - (self.SKY - 1)) / (self.h - self.SKY)
+        hw = 0.45 * self.w * p + 2.0
+        cx = self.w / 2 - x * hw + 60.0 * self.kappa(s) * (1.0 - p) ** 2
+        left = np.floor(cx - hw).astype(np.int64)
+        right = np.floor(cx + hw).astype(np.int64)
+        edge = np.maximum(1, np.floor(hw / 20.0)).astype(np.int64)
+        shade = (np.floor(4.0 * s + 8.0 / p).astype(np.int64) % 2)
+        return left, right, edge, shade
+
+    def render(self, which=None) -> torch.Tensor:
+        left, right, edge, shade = (torch.from_numpy(a.astype(np.int32)).to(self.device)[..., None]
+                                    for a in self.rows(which))
+        c = self._cols
+        road = (c >= left) & (c < right)
+        rim = road & ((c < left + edge) | (c >= right - edge))
+        idx = torch.where(rim, 5, torch.where(road, 3 + shade, 1 + shade))
+        ground = self._pal[idx.long()]
+        m = ground.shape[0]
+        sky = self._pal[0].expand(m, self.SKY, self.w, 3)
+        img = torch.cat([sky, ground], 1)
+        flip = torch.from_numpy(self.flip[np.arange(self.n) if which is None else np.asarray(which)])
+        if flip.any():
+            f = flip.to(self.device)
+            img = torch.where(f[:, None, None, None], img.flip(2), img)
+        return img.contiguous()
+
+    def load(self, states, flips=None, which=None):
+        which = list(range(self.n)) if which is None else list(which)
+        flips = flips or [False] * len(which)
+        for k, st, f in zip(which, states, flips):
+            self.st[k] = np.frombuffer(st, np.float64, count=5)
+            self.st[k, 4] = 0.0
+            self.flip[k] = f
+        return self.render(which), self._infos(which)
+
+    def save(self, which=None):
+        which = range(self.n) if which is None else which
+        return [self.st[k].tobytes() for k in which]
+
+    def step(self, buttons):
+        if len(buttons) != self.n:
+            raise ValueError(f"one button dict per game ({self.n}), got {len(buttons)}")
+        press = lambda key: np.array([bool(b.get(key, False)) for b in buttons], np.float64)  # noqa: E731
+        steer = (press("LEFT") - press("RIGHT")) + 0.5 * (press("L") - press("R"))
+        s, x, v, e, f = (self.st[:, i].copy() for i in range(5))
+        live = ~np.array([i["done"] for i in self._infos(range(self.n))])
+        v = np.where(press("B") > 0, np.minimum(v + 0.004, 0.25), np.where(press("Y") > 0, v - 0.01, v - 0.001))
+        v = np.maximum(v, 0.0)
+        x = x + 0.6 * v * self.kappa(s) - 0.05 * (0.2 + 4.0 * v) * steer
+        off = np.abs(x) > 1.0
+        e = np.where(off, e - 0.5, e)
+        v = np.where(off, 0.97 * v, v)
+        x = np.clip(x, -2.0, 2.0)
+        s = s + v
+        new = np.stack([s, x, v, e, f + 1], 1)
+        self.st[live] = new[live]
+        return self.render(), self._infos(range(self.n))
+
+    def close(self):
+        pass

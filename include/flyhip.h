@@ -240,8 +240,15 @@ int fz_instruct_bias(int n_flies, int n_dn, const int* fly_ptr, const int* fly_s
                      int n_upost, float eta_bias, const float* ratio, float limit, float* ib, float* bias_ext,
                      int n_ext, const int* dn_slot, void* stream);
 
-/* ---- batched motion-vision front end (replaces MotionEye.see, flyzero/vision.py:137-191, run per
- * emulator worker on the CPU, flyzero/pool.py:41-44; default parameters: motion channels only) ---- */
+/* ---- batched vision front end (replaces MotionEye.see, flyzero/vision.py:137-191, run per emulator
+ * worker on the CPU, flyzero/pool.py:41-44): the T4/T5 motion channels, the medulla transient and
+ * sustained-contrast channels and the R7/R8 colour channels ---- */
+#define FZ_EYE_ON_TRANSIENT 8   /* tr_gain * max(hp, 0) / gain      (Mi1, Tm3) */
+#define FZ_EYE_OFF_TRANSIENT 9  /* tr_gain * max(-hp, 0) / gain     (Tm1, Tm2, Tm4) */
+#define FZ_EYE_ON_SUSTAINED 10  /* su_gain * max(con, 0) / gain     (Mi4) */
+#define FZ_EYE_OFF_SUSTAINED 11 /* su_gain * max(-con, 0) / gain    (Mi9, Tm9) */
+#define FZ_EYE_BLUE 12          /* ch_gain * blue cell mean / gain  (R7) */
+#define FZ_EYE_GREEN 13         /* ch_gain * green cell mean / gain (R8) */
 typedef struct fz_eye_args {
     int B;                       /* flies (one frame each) */
     int h, w;                    /* frame size, RGB uint8 [B][h][w][3] */
@@ -254,7 +261,8 @@ typedef struct fz_eye_args {
     float* lp;                   /* [B][gh][gw] low-pass of the luminance (state) */
     float* d_on;                 /* [B][gh][gw] delayed ON signal (state) */
     float* d_off;                /* [B][gh][gw] delayed OFF signal (state) */
-    const int* key;              /* [n_cells] polarity * 4 + screen direction (0 right, 1 left, 2 down, 3 up) */
+    const int* key;              /* [n_cells] 0-7: polarity * 4 + screen direction (0 right, 1 left, 2 down,
+                                  * 3 up); 8-13: the FZ_EYE_* feature channels */
     const int* gy;               /* [n_cells] grid row */
     const int* gx;               /* [n_cells] grid column */
     const int* py;               /* [n_cells] screen row, pixels */
@@ -266,10 +274,15 @@ typedef struct fz_eye_args {
     int n_ext;
     const int* slot;             /* [n_cells] ext slot of each neuron */
     float p_scale;               /* dt / 1000 */
+    float tr_gain, su_gain, ch_gain; /* transient, sustained and chromatic gains (0: channel off, its
+                                  * cells give 0) */
+    int surround;                /* grid points, radius of the contrast surround: con = lum - box mean of
+                                  * lum over the window clipped to the grid */
 } fz_eye_args;
 
-/* One frame per fly: grid luminance, ON / OFF split, Hassenstein-Reichardt correlators and the
- * T4/T5 rates (and, with p_ext, the brain's per-step input probabilities, flyzero/fleet.py:76-80). */
+/* One frame per fly: grid luminance, ON / OFF split, Hassenstein-Reichardt correlators, the
+ * feature channels and every cell's rate (and, with p_ext, the brain's per-step input
+ * probabilities, flyzero/fleet.py:76-80). */
 int fz_eye_step(const fz_eye_args* a, void* stream);
 
 /* ---- learned rate surrogate: BF16 MFMA GEMM and training / inference element-wise kernels ---- */
