@@ -285,15 +285,33 @@ def fleet_line(fly, torch, B=256, frames=60, warmup=3):
     copy an emulator pool's host frames would add is timed on its own."""
     from flyhip.fleet import Fleet, SyntheticRace
 
+    class Timed:
+        """The pool with the host time of its calls summed (the game's share of the loop)."""
+
+        def __init__(self, pool):
+            self.pool, self.s = pool, 0.0
+
+        def load(self, *a, **k):
+            return self.pool.load(*a, **k)
+
+        def step(self, buttons):
+            t0 = time.perf_counter()
+            out = self.pool.step(buttons)
+            self.s += time.perf_counter() - t0
+            return out
+
     race = SyntheticRace(B, device="cuda")
-    fleet = Fleet(race, B, seed=3, conn=fly, tile=1024, tune=True)
+    timed = Timed(race)
+    fleet = Fleet(timed, B, seed=3, conn=fly, tile=1024, tune=True)
     start = SyntheticRace.start_state(v=0.1)
     fleet.exam(start, warmup)
     torch.cuda.synchronize()
+    timed.s = 0.0
     t0 = time.perf_counter()
     res = fleet.exam(start, frames, retire=False)
     torch.cuda.synchronize()
     loop_s = time.perf_counter() - t0
+    game_s = timed.s
     # the brain alone on the same inputs (graph replays of one window)
     fleet.brain.run(fleet.window)
     torch.cuda.synchronize()
@@ -320,9 +338,9 @@ def fleet_line(fly, torch, B=256, frames=60, warmup=3):
            "fly_frames_per_s": round(B * frames / loop_s, 1),
            "fly_sim_s_per_s": round(B * frames / loop_s * fleet.window / 1000.0, 2),
            "brain_ms_per_frame": round(brain_s / frames * 1000.0, 3),
+           "game_ms_per_frame": round(game_s / frames * 1000.0, 3),
            "host_frames_copy_ms": round(h2d_ms, 3), "mean_progress": summary["mean_progress"]}
-    fleet.close()
-    del fleet, race
+    del fleet, race, timed
     torch.cuda.empty_cache()
     return out
 
@@ -541,7 +559,8 @@ def main():
     if args.fleet_frames > 0:
         fleet = fleet_line(fly, torch, B=256, frames=args.fleet_frames)
         log(f"fleet B=256: {fleet['fly_frames_per_s']} fly-frames/s ({fleet['ms_per_frame']} ms/frame; "
-            f"brain alone {fleet['brain_ms_per_frame']} ms; host-frame copy {fleet['host_frames_copy_ms']} ms)")
+            f"brain alone {fleet['brain_ms_per_frame']} ms; game {fleet['game_ms_per_frame']} ms; host-frame copy "
+            f"{fleet['host_frames_copy_ms']} ms)")
 
     # ---- CPU baseline (oracle, one thread) --------------------------------------------------
     cpu = None

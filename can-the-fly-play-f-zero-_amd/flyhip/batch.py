@@ -231,32 +231,41 @@ class BatchMotor:
     def update(self, counts: torch.Tensor, window_ms: float) -> list:
         """Rates from the window's (B, n) device counts, then one button dict per slot (a host copy)."""
         self.rates = self.readout.update(counts, window_ms).cpu().numpy()
-        return [self.buttons(r, k) for k, r in enumerate(self.rates)]
+        return [dict(zip(BUTTONS, row)) for row in self.masks(self.rates).tolist()]
 
-    def _tap(self, k, j, x, threshold, span):
-        """+1 (left), -1 (right) or 0 this frame; duty = (|x| - threshold) / span (flyzero/batch.py:114-120)."""
-        duty = np.sign(x) * np.clip((abs(x) - threshold) / span, 0.0, 1.0)
-        self.acc[k, j] += duty
-        t = 1 if self.acc[k, j] >= 0.5 else -1 if self.acc[k, j] <= -0.5 else 0
-        self.acc[k, j] -= t
+    def _taps(self, rows, j, x, threshold, span):
+        """+1 (left), -1 (right) or 0 this frame per slot; duty = (|x| - threshold) / span into a
+        sigma-delta accumulator (flyzero/batch.py:114-120)."""
+        duty = np.sign(x) * np.clip((np.abs(x) - threshold) / span, 0.0, 1.0)
+        acc = self.acc[rows, j] + duty
+        t = np.where(acc >= 0.5, 1, np.where(acc <= -0.5, -1, 0))
+        self.acc[rows, j] = acc - t
         return t
 
-    def buttons(self, r, k: int = 0) -> dict:
-        """flyzero/batch.py:122-138."""
+    def masks(self, r, rows=None) -> np.ndarray:
+        """Pressed buttons (m, len(BUTTONS)) for the readout rates ``r`` (m, 9) of slots ``rows``
+        (default 0 .. m-1), all slots at once (flyzero/batch.py:122-138)."""
         p = self.p
-        steer = (r[0] - r[1]) + (r[5] - r[4]) + p.steer_bias
-        lean = r[2] - r[3]
-        b = {key: False for key in BUTTONS}
+        r = np.asarray(r)
+        rows = np.arange(len(r)) if rows is None else np.atleast_1d(rows)
+        steer = (r[:, 0] - r[:, 1]) + (r[:, 5] - r[:, 4]) + p.steer_bias
+        lean = r[:, 2] - r[:, 3]
+        m = np.zeros((len(r), len(BUTTONS)), bool)
+        col = {k: i for i, k in enumerate(BUTTONS)}
         if p.taps:
-            s = self._tap(k, 0, steer, p.steer_threshold, p.steer_span)
-            le = self._tap(k, 1, lean, p.lean_threshold, p.lean_span)
-            b["LEFT"], b["RIGHT"], b["L"], b["R"] = s > 0, s < 0, le > 0, le < 0
+            s = self._taps(rows, 0, steer, p.steer_threshold, p.steer_span)
+            le = self._taps(rows, 1, lean, p.lean_threshold, p.lean_span)
+            m[:, col["LEFT"]], m[:, col["RIGHT"]], m[:, col["L"]], m[:, col["R"]] = s > 0, s < 0, le > 0, le < 0
         else:
-            b["LEFT"] = bool(steer > p.steer_threshold)
-            b["RIGHT"] = bool(steer < -p.steer_threshold)
-            b["L"] = bool(lean > p.lean_threshold)
-            b["R"] = bool(lean < -p.lean_threshold)
-        b["B"] = bool(r[6] > p.accel_threshold)
-        b["Y"] = bool(r[7] > p.brake_threshold)
-        b["A"] = bool(r[8] > p.boost_threshold)
-        return b
+            m[:, col["LEFT"]] = steer > p.steer_threshold
+            m[:, col["RIGHT"]] = steer < -p.steer_threshold
+            m[:, col["L"]] = lean > p.lean_threshold
+            m[:, col["R"]] = lean < -p.lean_threshold
+        m[:, col["B"]] = r[:, 6] > p.accel_threshold
+        m[:, col["Y"]] = r[:, 7] > p.brake_threshold
+        m[:, col["A"]] = r[:, 8] > p.boost_threshold
+        return m
+
+    def buttons(self, r, k: int = 0) -> dict:
+        """One slot's button dict (flyzero/batch.py:122-138)."""
+        return dict(zip(BUTTONS, self.masks(np.asarray(r)[None], [k])[0].tolist()))

@@ -17,12 +17,15 @@ where ``frames`` is uint8 (B, h, w, 3), host (numpy) or device (torch), and ``in
 game with ``segment``, ``lap``, ``energy``, ``done`` and optionally ``finished`` and ``rank``
 (flyzero/pool.py:133-203, the F-Zero telemetry).  Host frames are staged through pinned memory
 into one device buffer.  ``SyntheticRace`` is such a pool: a deterministic road game with the
-same telemetry, for tests and measurement where no emulator exists.
+same telemetry, for tests and measurement where no emulator exists.  ``GamePool`` runs one game
+per subprocess (an emulator allows one per process) and gathers their frames in shared memory.
 """
 
 from __future__ import annotations
 
+import multiprocessing as mp
 import time
+from multiprocessing import shared_memory
 
 import numpy as np
 import torch
@@ -153,10 +156,11 @@ class Fleet:
         copied.record()
         return dev
 
-    def think(self, frames, extra: np.ndarray | None = None) -> torch.Tensor:
+    def think(self, frames, extra: np.ndarray | None = None, copy: bool = True) -> torch.Tensor:
         """One frame of brain time for every slot (flyzero/fleet.py:73-81): the eye turns
         ``frames`` into the brain's input probabilities, ``extra`` (B, len(extra_idx)) Hz adds
-        to the extra inputs.  Returns the window's (B, n) spike counts on the device."""
+        to the extra inputs.  Returns the window's (B, n) spike counts on the device
+        (``copy=False``: the engine's own buffer, overwritten by the next window)."""
         brain = self.brain
         if extra is not None and len(self.extra_idx):
             j = brain.slots(self.extra_idx)
@@ -165,7 +169,7 @@ class Fleet:
         if extra is not None and len(self.extra_idx):
             brain.p_ext[:, j] += (torch.as_tensor(np.asarray(extra, np.float32), device=self.device) *
                                   np.float32(brain.p.dt / 1000.0))
-        return brain.run(self.window)
+        return brain.run(self.window, copy=copy)
 
     def reset_slots(self, slots):
         """Brain, readout and eye filters of the given slots back to their start."""
@@ -191,7 +195,7 @@ class Fleet:
         t0 = time.time()
         try:
             for i in range(frames):
-                counts = self.think(frames_now)
+                counts = self.think(frames_now, copy=on_frame is not None)
                 buttons = self.motor.update(counts, self.window)
                 if record:
                     for k in np.flatnonzero(~done):
@@ -258,6 +262,12 @@ class SyntheticRace:
         self.flip = np.zeros(n, bool)
         self._pal = torch.from_numpy(self.PALETTE).to(self.device)
         self._cols = torch.arange(self.w, device=self.device, dtype=torch.int32)
+        # per-row constants of the perspective: p, half-width, bend weight, shade phase
+        p = (np.arange(self.SKY, self.h, dtype=np.float64) - (self.SKY - 1)) / (self.h - self.SKY)
+        self._hw = (0.45 * self.w * p + 2.0)[None, :]
+        self._bend = (60.0 * (1.0 - p) ** 2)[None, :]
+        self._phase = (8.0 / p)[None, :]
+        self._edge = np.maximum(1, np.floor(self._hw / 20.0)).astype(np.int64)
 
     @staticmethod
     def start_state(s: float = 0.0, x: float = 0.0, v: float = 0.0, energy: float = 100.0) -> bytes:
@@ -267,29 +277,33 @@ class SyntheticRace:
         a = 2 * np.pi * np.asarray(s, np.float64) / self.S
         return 0.8 * np.sin(2 * a) + 0.4 * np.sin(5 * a + 1.0)
 
+    def _done(self, which):
+        s, e, f = self.st[which, 0], self.st[which, 3], self.st[which, 4]
+        finished = s // self.S >= 5
+        done = (e <= 0) | finished
+        if self.max_frames is not None:
+            done |= f >= self.max_frames
+        return done, finished
+
     def _infos(self, which):
-        out = []
-        for k in which:
-            s, x, v, e, f = self.st[k]
-            lap = int(s // self.S)
-            finished = lap >= 5
-            done = e <= 0 or finished or (self.max_frames is not None and f >= self.max_frames)
-            out.append({"segment": int(s) % self.S, "lap": lap, "energy": float(e), "done": bool(done),
-                        "finished": bool(finished), "rank": None, "frame": int(f)})
-        return out
+        which = np.asarray(list(which), np.int64)
+        done, finished = self._done(which)
+        s = self.st[which, 0]
+        cols = zip((s.astype(np.int64) % self.S).tolist(), (s // self.S).astype(np.int64).tolist(),
+                   self.st[which, 3].tolist(), done.tolist(), finished.tolist(), self.st[which, 4].astype(np.int64).tolist())
+        return [{"segment": seg, "lap": lap, "energy": e, "done": d, "finished": fin, "rank": None, "frame": f}
+                for seg, lap, e, d, fin, f in cols]
 
     def rows(self, which=None):
         """Per-row integer tables (left, right, edge, shade) of each player's view, [len(which), rows]."""
         which = np.arange(self.n) if which is None else np.asarray(which)
         s, x = self.st[which, 0:1], self.st[which, 1:2]
-        y = np.arange(self.SKY, self.h, dtype=np.float64)[None, :]
-        p = (y - (self.SKY - 1)) / (self.h - self.SKY)
-        hw = 0.45 * self.w * p + 2.0
-        cx = self.w / 2 - x * hw + 60.0 * self.kappa(s) * (1.0 - p) ** 2
+        hw = self._hw
+        cx = self.w / 2 - x * hw + self.kappa(s) * self._bend
         left = np.floor(cx - hw).astype(np.int64)
         right = np.floor(cx + hw).astype(np.int64)
-        edge = np.maximum(1, np.floor(hw / 20.0)).astype(np.int64)
-        shade = (np.floor(4.0 * s + 8.0 / p).astype(np.int64) % 2)
+        edge = np.broadcast_to(self._edge, left.shape)
+        shade = np.floor(4.0 * s + self._phase).astype(np.int64) % 2
         return left, right, edge, shade
 
     def render(self, which=None) -> torch.Tensor:
@@ -322,14 +336,16 @@ class SyntheticRace:
         which = range(self.n) if which is None else which
         return [self.st[k].tobytes() for k in which]
 
+    KEYS = ("LEFT", "RIGHT", "L", "R", "B", "Y")
+
     def step(self, buttons):
         if len(buttons) != self.n:
             raise ValueError(f"one button dict per game ({self.n}), got {len(buttons)}")
-        press = lambda key: np.array([bool(b.get(key, False)) for b in buttons], np.float64)  # noqa: E731
-        steer = (press("LEFT") - press("RIGHT")) + 0.5 * (press("L") - press("R"))
+        P = np.array([[b.get(k, False) for k in self.KEYS] for b in buttons], np.float64).reshape(self.n, 6)
+        steer = (P[:, 0] - P[:, 1]) + 0.5 * (P[:, 2] - P[:, 3])
         s, x, v, e, f = (self.st[:, i].copy() for i in range(5))
-        live = ~np.array([i["done"] for i in self._infos(range(self.n))])
-        v = np.where(press("B") > 0, np.minimum(v + 0.004, 0.25), np.where(press("Y") > 0, v - 0.01, v - 0.001))
+        live = ~self._done(np.arange(self.n))[0]
+        v = np.where(P[:, 4] > 0, np.minimum(v + 0.004, 0.25), np.where(P[:, 5] > 0, v - 0.01, v - 0.001))
         v = np.maximum(v, 0.0)
         x = x + 0.6 * v * self.kappa(s) - 0.05 * (0.2 + 4.0 * v) * steer
         off = np.abs(x) > 1.0
@@ -343,3 +359,102 @@ class SyntheticRace:
 
     def close(self):
         pass
+
+
+def _game_worker(conn, factory, shm_name, n, k, frame_shape):
+    """One game in its own process (flyzero/pool.py:22-130): ``factory()`` makes a one-player
+    pool; its frames go into row ``k`` of the shared (n, h, w, 3) block, its telemetry and saved
+    states through the pipe."""
+    torch.set_num_threads(1)
+    game = factory()
+    shm = shared_memory.SharedMemory(name=shm_name)
+    frames = np.ndarray((n, *frame_shape, 3), np.uint8, buffer=shm.buf)
+
+    def put(out):
+        fr, infos = out
+        frames[k] = fr[0].cpu().numpy() if isinstance(fr, torch.Tensor) else np.asarray(fr)[0]
+        return infos[0]
+
+    try:
+        while True:
+            msg = conn.recv()
+            cmd = msg[0]
+            if cmd == "load":
+                conn.send(put(game.load([msg[1]], flips=[msg[2]])))
+            elif cmd == "step":
+                conn.send(put(game.step([msg[1]])))
+            elif cmd == "save":
+                conn.send(game.save([0])[0])
+            elif cmd == "close":
+                return
+    finally:
+        del frames
+        shm.close()
+        conn.close()
+
+
+class GamePool:
+    """``EmulatorPool`` (flyzero/pool.py:133-203) for any game: ``factory()`` (picklable) makes a
+    one-player pool (``load`` / ``step`` / ``save`` of ``SyntheticRace``'s form, e.g. an emulator
+    wrapper); each of the ``n`` games runs in its own spawned process.  The frames come back in
+    one shared (n, h, w, 3) uint8 block, whose view ``load`` and ``step`` return (valid until the
+    next call), so the fleet copies them to the device in one piece."""
+
+    def __init__(self, factory, n: int, frame_shape=FRAME_SHAPE):
+        ctx = mp.get_context("spawn")
+        self.n, self.frame_shape = n, tuple(frame_shape)
+        size = n * self.frame_shape[0] * self.frame_shape[1] * 3
+        self._shm = shared_memory.SharedMemory(create=True, size=max(1, size))
+        self.frames = np.ndarray((n, *self.frame_shape, 3), np.uint8, buffer=self._shm.buf)
+        self.pipes, self.procs = [], []
+        for k in range(n):
+            a, b = ctx.Pipe()
+            p = ctx.Process(target=_game_worker, args=(b, factory, self._shm.name, n, k, self.frame_shape),
+                            daemon=True)
+            p.start()
+            b.close()
+            self.pipes.append(a)
+            self.procs.append(p)
+
+    def _all(self, msgs, which=None):
+        which = list(range(self.n)) if which is None else list(which)
+        msgs = list(msgs)
+        if len(msgs) != len(which):
+            raise ValueError("one message per worker")
+        for k, m in zip(which, msgs):
+            self.pipes[k].send(m)
+        return [self.pipes[k].recv() for k in which]
+
+    def load(self, states, flips=None, which=None):
+        which = list(range(self.n)) if which is None else list(which)
+        flips = flips or [False] * len(which)
+        infos = self._all([("load", s, f) for s, f in zip(states, flips)], which)
+        return (self.frames if len(which) == self.n and which == list(range(self.n)) else self.frames[which]), infos
+
+    def step(self, buttons):
+        infos = self._all([("step", b) for b in buttons])
+        return self.frames, infos
+
+    def save(self, which=None):
+        which = list(range(self.n)) if which is None else list(which)
+        return self._all([("save",)] * len(which), which)
+
+    def close(self):
+        for p in self.pipes:
+            try:
+                p.send(("close",))
+            except (BrokenPipeError, OSError):
+                pass
+        for p in self.procs:
+            p.join(timeout=10)
+            if p.is_alive():
+                p.terminate()
+                p.join(timeout=5)
+        for c in self.pipes:
+            c.close()
+        self.frames = None
+        try:
+            self._shm.close()
+        except BufferError:       # a caller still holds a frames view; the mapping goes with it
+            pass
+        self._shm.unlink()

@@ -103,6 +103,41 @@ def test_synthetic_race_view():
     assert (race.render([0]) != a).any()
 
 
+def test_game_pool_runs_one_game_per_process_like_the_batched_race():
+    # flyzero/pool.py:133-203: one game per spawned process; here the frames come back through
+    # shared memory and must equal the in-process batched game's, as must the telemetry
+    from functools import partial
+
+    from flyhip.fleet import GamePool
+
+    pool = GamePool(partial(SyntheticRace, 1, max_frames=30), 3)
+    ref = SyntheticRace(3, max_frames=30)
+    try:
+        states = [SyntheticRace.start_state(v=0.05 * k, x=0.3 * k) for k in range(3)]
+        f1, i1 = pool.load(states, flips=[False, True, False])
+        f2, i2 = ref.load(states, flips=[False, True, False])
+        np.testing.assert_array_equal(f1, f2.numpy())
+        assert i1 == i2
+        rng = np.random.default_rng(0)
+        for _ in range(8):
+            buttons = [{k: bool(rng.random() < 0.5) for k in ("B", "LEFT", "RIGHT", "Y", "L")} for _ in range(3)]
+            f1, i1 = pool.step(buttons)
+            f2, i2 = ref.step(buttons)
+            np.testing.assert_array_equal(f1, f2.numpy())
+            assert i1 == i2
+        assert pool.save([1]) == ref.save([1])
+        f1, i1 = pool.load([states[0]], which=[2])
+        f2, i2 = ref.load([states[0]], which=[2])
+        assert f1.shape == (1, H, W, 3)
+        np.testing.assert_array_equal(f1, f2.numpy())
+        assert i1 == i2
+        with pytest.raises(ValueError, match="one message"):
+            pool.load(states[:2])
+    finally:
+        pool.close()
+    assert all(not p.is_alive() for p in pool.procs)
+
+
 # ---------------------------------------------------------------- GPU
 
 def _small_fleet(B, seed, pool, **kw):
@@ -220,5 +255,26 @@ def test_retiring_finished_slots_changes_no_result():
     rb = fb.exam(start, 12, retire=False, record=True)
     for a, b in zip(ra, rb):
         assert a["frames"] == b["frames"] and a["progress"] == b["progress"] and a["energy"] == b["energy"]
+        np.testing.assert_array_equal(a["masks"], b["masks"])
+        np.testing.assert_array_equal(a["rates"], b["rates"])
+
+
+@pytest.mark.gpu
+def test_exam_with_games_in_subprocesses_equals_the_device_game():
+    from functools import partial
+
+    from flyhip.fleet import GamePool
+
+    start = SyntheticRace.start_state(x=0.3, v=0.12, energy=2.0)
+    pool = GamePool(partial(SyntheticRace, 1, max_frames=9), 2)
+    try:
+        fa, _ = _small_fleet(2, 7, pool)
+        ra = fa.exam(start, 10, record=True)
+    finally:
+        pool.close()
+    fb, _ = _small_fleet(2, 7, SyntheticRace(2, device="cuda", max_frames=9))
+    rb = fb.exam(start, 10, record=True)
+    for a, b in zip(ra, rb):
+        assert (a["frames"], a["progress"], a["energy"]) == (b["frames"], b["progress"], b["energy"])
         np.testing.assert_array_equal(a["masks"], b["masks"])
         np.testing.assert_array_equal(a["rates"], b["rates"])

@@ -133,3 +133,22 @@ def test_motor_buttons_thresholds_and_taps():
     r[:] = 0
     r[READOUT.index("lean_right")] = 80.0            # duty -1: lean right every frame
     assert all(m.buttons(r)["R"] for _ in range(3))
+
+
+def test_motor_masks_for_all_slots_equal_one_slot_at_a_time():
+    from flyhip.batch import BUTTONS, BatchMotor, MotorParams
+
+    rng = np.random.default_rng(2)
+    B = 40
+    for taps in (False, True):
+        a = BatchMotor.__new__(BatchMotor)
+        b = BatchMotor.__new__(BatchMotor)
+        for m in (a, b):
+            m.p = MotorParams(taps=taps, steer_bias=1.5)
+            m.acc = np.zeros((B, 2))
+        for _ in range(20):
+            r = (rng.random((B, 9)) * 120 - 20).astype(np.float32)
+            together = a.masks(r)
+            one = np.array([[d[key] for key in BUTTONS] for d in (b.buttons(r[k], k) for k in range(B))])
+            np.testing.assert_array_equal(together, one)
+        np.testing.assert_array_equal(a.acc, b.acc)
