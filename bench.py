@@ -277,12 +277,14 @@ def eye_line(fly, torch, B=256, frames=50, shape=(224, 256)):
     return res
 
 
-def fleet_line(fly, torch, B=256, frames=60, warmup=3):
+def fleet_line(fly, torch, B=256, frames=60, warmup=3, parts=1):
     """The closed loop of flyzero/fleet.py:87-127 at B flies: each frame the eye kernel turns every
     fly's 224 x 256 view into the brain's input probabilities, one window of brain time (dt =
     0.25 ms, plastic synapses onto the motor DNs), the DN readout and the host's button logic,
     then every game steps.  The games are the synthetic road game rendered on the device; the
-    copy an emulator pool's host frames would add is timed on its own."""
+    copy an emulator pool's host frames would add is timed on its own.  ``parts`` > 1 splits the
+    flies into that many brains served in turn (the host steps one part's games while the device
+    simulates the next)."""
     from flyhip.fleet import Fleet, SyntheticRace
 
     class Timed:
@@ -294,15 +296,15 @@ def fleet_line(fly, torch, B=256, frames=60, warmup=3):
         def load(self, *a, **k):
             return self.pool.load(*a, **k)
 
-        def step(self, buttons):
+        def step(self, buttons, **kw):
             t0 = time.perf_counter()
-            out = self.pool.step(buttons)
+            out = self.pool.step(buttons, **kw)
             self.s += time.perf_counter() - t0
             return out
 
     race = SyntheticRace(B, device="cuda")
     timed = Timed(race)
-    fleet = Fleet(timed, B, seed=3, conn=fly, tile=1024, tune=True)
+    fleet = Fleet(timed, B, seed=3, conn=fly, tile=1024, tune=True, parts=parts)
     start = SyntheticRace.start_state(v=0.1)
     fleet.exam(start, warmup)
     torch.cuda.synchronize()
@@ -312,34 +314,36 @@ def fleet_line(fly, torch, B=256, frames=60, warmup=3):
     torch.cuda.synchronize()
     loop_s = time.perf_counter() - t0
     game_s = timed.s
-    # the brain alone on the same inputs (graph replays of one window)
-    fleet.brain.run(fleet.window)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(frames):
-        fleet.brain.run(fleet.window, copy=False)
-    torch.cuda.synchronize()
-    brain_s = time.perf_counter() - t0
-    # host frames of an emulator pool: pinned staging + one copy per frame
-    host = race.render().cpu().numpy()
-    fleet.frames_on_device(host)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(10):
+    brain_s = h2d_ms = None
+    if parts == 1:
+        # the brain alone on the same inputs (graph replays of one window)
+        fleet.brain.run(fleet.window)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(frames):
+            fleet.brain.run(fleet.window, copy=False)
+        torch.cuda.synchronize()
+        brain_s = time.perf_counter() - t0
+        # host frames of an emulator pool: pinned staging + one copy per frame
+        host = race.render().cpu().numpy()
         fleet.frames_on_device(host)
-    torch.cuda.synchronize()
-    h2d_ms = (time.perf_counter() - t0) / 10 * 1000.0
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(10):
+            fleet.frames_on_device(host)
+        torch.cuda.synchronize()
+        h2d_ms = (time.perf_counter() - t0) / 10 * 1000.0
     from flyhip.fleet import summarize
 
     summary = summarize(res)
-    out = {"B": B, "frames": frames, "neurons": fly.n, "eye_cells": int(len(fleet.eye.idx)),
-           "plastic_synapses": int(len(fleet.pos)), "steps_per_frame": fleet.brain.steps_of(fleet.window),
+    out = {"B": B, "parts": parts, "frames": frames, "neurons": fly.n, "eye_cells": int(len(fleet.parts[0].eye.idx)),
+           "plastic_synapses": int(len(fleet.pos)), "steps_per_frame": fleet.parts[0].brain.steps_of(fleet.window),
            "ms_per_frame": round(loop_s / frames * 1000.0, 3),
            "fly_frames_per_s": round(B * frames / loop_s, 1),
            "fly_sim_s_per_s": round(B * frames / loop_s * fleet.window / 1000.0, 2),
-           "brain_ms_per_frame": round(brain_s / frames * 1000.0, 3),
+           "brain_ms_per_frame": round(brain_s / frames * 1000.0, 3) if brain_s else None,
            "game_ms_per_frame": round(game_s / frames * 1000.0, 3),
-           "host_frames_copy_ms": round(h2d_ms, 3), "mean_progress": summary["mean_progress"]}
+           "host_frames_copy_ms": round(h2d_ms, 3) if h2d_ms else None, "mean_progress": summary["mean_progress"]}
     del fleet, race, timed
     torch.cuda.empty_cache()
     return out
@@ -561,6 +565,9 @@ def main():
         log(f"fleet B=256: {fleet['fly_frames_per_s']} fly-frames/s ({fleet['ms_per_frame']} ms/frame; "
             f"brain alone {fleet['brain_ms_per_frame']} ms; game {fleet['game_ms_per_frame']} ms; host-frame copy "
             f"{fleet['host_frames_copy_ms']} ms)")
+        fleet["parts2"] = fleet_line(fly, torch, B=256, frames=args.fleet_frames, parts=2)
+        log(f"fleet B=256 in 2 parts: {fleet['parts2']['fly_frames_per_s']} fly-frames/s "
+            f"({fleet['parts2']['ms_per_frame']} ms/frame)")
 
     # ---- CPU baseline (oracle, one thread) --------------------------------------------------
     cpu = None

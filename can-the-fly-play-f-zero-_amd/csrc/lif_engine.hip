@@ -131,6 +131,7 @@ struct UpdArgs {
     unsigned* raster;      // [steps][B][rw] or null
     int rw;
     unsigned seed;
+    unsigned fly_base;     // RNG numbering of fly b is b + fly_base
     const unsigned char* alive;
     const int* indptr;
     const int* pl_indptr;  // null without plastic synapses
@@ -196,8 +197,9 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_update(UpdArgs a) {
     const float u_in = u, g_in = g;
     const int r_in = r;
 
-    const uint32_t hseed = mix32(a.seed + (uint32_t)b);
-    const uint32_t tid32 = (uint32_t)(bn + (size_t)i);
+    const uint32_t hb = (uint32_t)b + a.fly_base;               // the fly's number in the draw
+    const uint32_t hseed = mix32(a.seed + hb);
+    const uint32_t tid32 = hb * (uint32_t)a.N + (uint32_t)i;     // (b * N + i) mod 2^32
     uint32_t mask = 0;
 #pragma unroll
     for (int k = 0; k < KMAX; k++) {
@@ -457,6 +459,7 @@ struct TileArgs {
     unsigned* raster;
     int rw;
     unsigned seed;
+    unsigned fly_base;     // RNG numbering of fly b is b + fly_base
     const unsigned char* alive;
     const int* col;
     const void* w;
@@ -591,7 +594,8 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
     for (int q = tid; q < (KP * TS) >> 2; q += TPB) reinterpret_cast<int4*>(acc)[q] = make_int4(0, 0, 0, 0);
     // per-step keys of the Poisson draw (read after the next barrier)
     __shared__ uint32_t s_sk[KMAX];
-    const uint32_t hseed = mix32(a.seed + (uint32_t)b);
+    const uint32_t hb = (uint32_t)b + a.fly_base;  // the fly's number in the draw
+    const uint32_t hseed = mix32(a.seed + hb);
     if (tid < a.K) s_sk[tid] = mix32((step0 + (uint32_t)tid) * 0x9e3779b9U ^ hseed);
     const int* bkp4 = nullptr;  // this tile's bucket: entry e at bkp4[e]
     int total;
@@ -807,7 +811,7 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
                 if (a.hits) {
                     hit = (a.hits[((size_t)(a.kb + k) * a.B + b) * a.hw + (pl >> 5)] >> (pl & 31)) & 1u;
                 } else {
-                    const uint32_t h = mix32((uint32_t)(bn + (size_t)(i0 + lq)) ^ s_sk[k]);
+                    const uint32_t h = mix32((hb * (uint32_t)a.N + (uint32_t)(i0 + lq)) ^ s_sk[k]);
                     hit = (float)(h >> 8) * (1.0f / 16777216.0f) < __int_as_float(pl);
                 }
                 if (hit) atomicOr(&acc[__umul24(lq, KP) + a.K], 1 << k);
@@ -1340,6 +1344,7 @@ UpdArgs make_upd(const fz_engine* e, const fz_inputs* in, const Shape& s, int K,
     a.raster = raster;
     a.rw = (e->n + 31) / 32;
     a.seed = in->seed;
+    a.fly_base = in->fly_base;
     a.alive = in->alive;
     a.indptr = e->indptr;
     a.pl_indptr = e->n_pl > 0 ? e->pl_indptr : nullptr;
@@ -1408,6 +1413,7 @@ TileArgs make_tile(const fz_engine* e, const fz_inputs* in, int K, int KP, int k
     a.raster = raster;
     a.rw = (e->n + 31) / 32;
     a.seed = in->seed;
+    a.fly_base = in->fly_base;
     a.alive = in->alive;
     a.col = e->col;
     a.w = e->w;

@@ -36,7 +36,7 @@ class FzState(ctypes.Structure):
 
 class FzInputs(ctypes.Structure):
     _fields_ = [("n_ext", c_int), ("ext_idx", c_void_p), ("p_ext", c_void_p), ("bias_ext", c_void_p),
-                ("pw", c_void_p), ("alive", c_void_p), ("seed", ctypes.c_uint)]
+                ("pw", c_void_p), ("alive", c_void_p), ("seed", ctypes.c_uint), ("fly_base", ctypes.c_uint)]
 
 
 class FzTune(ctypes.Structure):

@@ -11,7 +11,7 @@ goes through one eye kernel that writes the brain's input probabilities on the d
 is any object with
 
     load(states, flips=None, which=None) -> (frames, infos)
-    step(buttons) -> (frames, infos)
+    step(buttons, which=None) -> (frames, infos)
 
 where ``frames`` is uint8 (B, h, w, 3), host (numpy) or device (torch), and ``infos`` one dict per
 game with ``segment``, ``lap``, ``energy``, ``done`` and optionally ``finished`` and ``rank``
@@ -80,68 +80,42 @@ def vision_params(vision: dict | None = None):
     return MotionParams(**v)
 
 
-class Fleet:
-    def __init__(self, pool, batch: int, seed: int = 0, plastic_types=None, bias_mv: float = 7.8, conn=None,
-                 deep: bool | int = False, taps: bool = False, steer_span: float = 150.0, lean_span: float = 70.0,
-                 steer_threshold: float = 10.0, lean_threshold: float = 10.0, readout_tau: float = 80.0,
-                 vision: dict | None = None, segments: int = 59, engine: str = "tile", tile: int = 1024,
-                 tune: bool = True, device=None):
-        """flyzero/fleet.py:22-68 with ``pool`` (a game pool, see the module docstring) in place of
-        the ROM, ``conn`` (a connectome; default the synthetic FlyWire-sized fly) and ``segments``
-        (per lap, for ``Progress``).  ``deep``: also make the synapses onto the DNs' presynaptic
-        partners plastic (2: one layer further).  ``tune``: pick the brain's launch shapes by
-        timing them (cached per workload)."""
+class _Part:
+    """The flies [b0, b0 + nb) of a fleet with their own brain, eye and readout.  A fleet of
+    several parts lets the host serve one part (buttons, games) while the device runs another.
+    The brain numbers its flies from b0 in the Poisson draw (``fly_base``), so every fly draws
+    the same spiking noise as in one brain of all the flies."""
+
+    def __init__(self, fleet, conn, b0: int, nb: int, seed: int, vision, motor_params, engine, tile, device):
         from . import BatchBrain, LIFParams
-        from .batch import BatchMotor, MotorParams
+        from .batch import BatchMotor
         from .eye import BatchMotionEye
-        from .instruct import PLASTIC_TYPES, deep2_positions, deep_positions, plastic_positions
 
-        if conn is None:
-            from .connectome import flywire_like
-
-            conn = flywire_like()
-        self.conn = conn
-        self.B = batch
-        self.pos = plastic_positions(conn, plastic_types or PLASTIC_TYPES)
-        if deep:
-            dn = self.pos
-            self.pos = np.union1d(dn, deep_positions(conn, dn))
-            if int(deep) >= 2:
-                self.pos = np.union1d(self.pos, deep2_positions(conn, dn))
-        self.brain = BatchBrain(conn.weights, LIFParams(dt=0.25), batch=batch, seed=seed, plastic_pos=self.pos,
+        self.b0, self.nb = b0, nb
+        self.brain = BatchBrain(conn.weights, LIFParams(dt=0.25), batch=nb, seed=seed, plastic_pos=fleet.pos,
                                 engine=engine, tile=tile, device=device)
-        self.device = self.brain.device
-        self.eye = BatchMotionEye.for_fly(conn, FRAME_SHAPE, batch, vision_params(vision), device=self.device)
-        self.motor = BatchMotor(conn, batch, MotorParams(taps=taps, steer_span=steer_span, lean_span=lean_span,
-                                                         steer_threshold=steer_threshold,
-                                                         lean_threshold=lean_threshold, tau_ms=readout_tau),
-                                device=self.device)
-        self.bias_idx = np.concatenate([conn.find(t) for t in BIAS_TYPES])
-        self.bias_mv = bias_mv
-        self.brain.set_bias(self.bias_idx, bias_mv)
+        self.brain.fly_base = b0
+        dev = self.device = self.brain.device
+        self.eye = BatchMotionEye.for_fly(conn, FRAME_SHAPE, nb, vision, device=dev)
+        self.motor = BatchMotor(conn, nb, motor_params, device=dev)
+        self.brain.set_bias(fleet.bias_idx, fleet.bias_mv)
         self.eye_slots = self.brain.slots(self.eye.idx)
-        self.segments = segments
-        self.pool = pool
-        self.window = WINDOW_MS
-        self.extra_idx = np.zeros(0, np.int64)    # optional extra Poisson inputs (exploration, heat)
-        self._stage = None                         # pinned host frames -> device frames
-        if tune:
-            from . import autotune
+        self.counts = None
+        self._stage = None
+        self._rates = torch.zeros((nb, self.motor.rates.shape[1]), dtype=torch.float32, pin_memory=True)
+        self._ready = torch.cuda.Event()
 
-            autotune.tune(self.brain)
-
-    def close(self):
-        close = getattr(self.pool, "close", None)
-        if close is not None:
-            close()
+    @property
+    def slots(self):
+        return range(self.b0, self.b0 + self.nb)
 
     def frames_on_device(self, frames) -> torch.Tensor:
-        """uint8 (B, h, w, 3) frames on the brain's device; host frames go through one pinned
-        buffer and one asynchronous copy on the current stream (the buffer is refilled only once
-        the previous copy has finished)."""
+        """uint8 (nb, h, w, 3) frames on the device; host frames go through one pinned buffer and
+        one asynchronous copy on the current stream (the buffer is refilled only once the
+        previous copy has finished)."""
         if isinstance(frames, torch.Tensor) and frames.device == self.device:
             return frames
-        shape = (self.B, *FRAME_SHAPE, 3)
+        shape = (self.nb, *FRAME_SHAPE, 3)
         if self._stage is None:
             self._stage = (torch.empty(shape, dtype=torch.uint8, pin_memory=True),
                            torch.empty(shape, dtype=torch.uint8, device=self.device), torch.cuda.Event())
@@ -156,16 +130,107 @@ class Fleet:
         copied.record()
         return dev
 
+    def see(self, frames):
+        """``p_ext`` of this frame: zero, then the eye cells' probabilities."""
+        self.brain.p_ext.zero_()
+        self.eye.see(self.frames_on_device(frames), brain=self.brain)
+
+    def launch(self, frames, window: float, copy: bool):
+        """Queue eye, one window of the brain, the readout and its copy to the host."""
+        self.see(frames)
+        self.counts = self.brain.run(window, copy=copy)
+        self.motor.readout.update(self.counts, window)
+        self._rates.copy_(self.motor.readout.rates, non_blocking=True)
+        self._ready.record()
+
+    def collect(self) -> list:
+        """Wait for this part's readout; one button dict per fly (``BatchMotor.update``'s)."""
+        from .batch import BUTTONS
+
+        self._ready.synchronize()
+        self.motor.rates = self._rates.numpy().copy()
+        return [dict(zip(BUTTONS, row)) for row in self.motor.masks(self.motor.rates).tolist()]
+
+
+class Fleet:
+    def __init__(self, pool, batch: int, seed: int = 0, plastic_types=None, bias_mv: float = 7.8, conn=None,
+                 deep: bool | int = False, taps: bool = False, steer_span: float = 150.0, lean_span: float = 70.0,
+                 steer_threshold: float = 10.0, lean_threshold: float = 10.0, readout_tau: float = 80.0,
+                 vision: dict | None = None, segments: int = 59, engine: str = "tile", tile: int = 1024,
+                 tune: bool = True, parts: int = 1, device=None):
+        """flyzero/fleet.py:22-68 with ``pool`` (a game pool, see the module docstring) in place of
+        the ROM, ``conn`` (a connectome; default the synthetic FlyWire-sized fly) and ``segments``
+        (per lap, for ``Progress``).  ``deep``: also make the synapses onto the DNs' presynaptic
+        partners plastic (2: one layer further).  ``tune``: pick the brains' launch shapes by
+        timing them (cached per workload).  ``parts``: split the flies into that many brains
+        that ``exam`` runs in turn, so that the host serves one part's games while the device
+        simulates another (``brain``, ``eye``, ``motor`` and ``think`` need ``parts=1``; the
+        parts are in ``self.parts``)."""
+        from .batch import MotorParams
+        from .instruct import PLASTIC_TYPES, deep2_positions, deep_positions, plastic_positions
+
+        if conn is None:
+            from .connectome import flywire_like
+
+            conn = flywire_like()
+        if not 1 <= parts <= batch:
+            raise ValueError(f"parts must be in [1, batch], got {parts}")
+        self.conn = conn
+        self.B = batch
+        self.pos = plastic_positions(conn, plastic_types or PLASTIC_TYPES)
+        if deep:
+            dn = self.pos
+            self.pos = np.union1d(dn, deep_positions(conn, dn))
+            if int(deep) >= 2:
+                self.pos = np.union1d(self.pos, deep2_positions(conn, dn))
+        self.bias_idx = np.concatenate([conn.find(t) for t in BIAS_TYPES])
+        self.bias_mv = bias_mv
+        mp = MotorParams(taps=taps, steer_span=steer_span, lean_span=lean_span, steer_threshold=steer_threshold,
+                         lean_threshold=lean_threshold, tau_ms=readout_tau)
+        vis = vision_params(vision)
+        edges = np.linspace(0, batch, parts + 1).round().astype(int)
+        self.parts = [_Part(self, conn, int(b0), int(b1 - b0), seed, vis, mp, engine, tile, device)
+                      for b0, b1 in zip(edges[:-1], edges[1:])]
+        one = self.parts[0] if parts == 1 else None
+        self.brain = one.brain if one else None
+        self.eye = one.eye if one else None
+        self.motor = one.motor if one else None
+        self.eye_slots = one.eye_slots if one else None
+        self.device = self.parts[0].device
+        self.segments = segments
+        self.pool = pool
+        self.window = WINDOW_MS
+        self.extra_idx = np.zeros(0, np.int64)    # optional extra Poisson inputs (exploration, heat)
+        if tune:
+            from . import autotune
+
+            for part in self.parts:
+                autotune.tune(part.brain)
+
+    def close(self):
+        close = getattr(self.pool, "close", None)
+        if close is not None:
+            close()
+
+    def _one(self) -> _Part:
+        if len(self.parts) != 1:
+            raise RuntimeError("this call needs a fleet of one part (Fleet(parts=1))")
+        return self.parts[0]
+
+    def frames_on_device(self, frames) -> torch.Tensor:
+        """uint8 (B, h, w, 3) frames on the device (one pinned buffer, one asynchronous copy)."""
+        return self._one().frames_on_device(frames)
+
     def think(self, frames, extra: np.ndarray | None = None, copy: bool = True) -> torch.Tensor:
         """One frame of brain time for every slot (flyzero/fleet.py:73-81): the eye turns
         ``frames`` into the brain's input probabilities, ``extra`` (B, len(extra_idx)) Hz adds
         to the extra inputs.  Returns the window's (B, n) spike counts on the device
         (``copy=False``: the engine's own buffer, overwritten by the next window)."""
-        brain = self.brain
+        part = self._one()
+        brain = part.brain
         if extra is not None and len(self.extra_idx):
             j = brain.slots(self.extra_idx)
-        brain.p_ext.zero_()
-        self.eye.see(self.frames_on_device(frames), brain=brain)
+        part.see(frames)
         if extra is not None and len(self.extra_idx):
             brain.p_ext[:, j] += (torch.as_tensor(np.asarray(extra, np.float32), device=self.device) *
                                   np.float32(brain.p.dt / 1000.0))
@@ -174,34 +239,57 @@ class Fleet:
     def reset_slots(self, slots):
         """Brain, readout and eye filters of the given slots back to their start."""
         slots = np.atleast_1d(slots)
-        self.brain.reset(slots)
-        self.motor.reset(slots)
-        self.eye.reset(slots)
+        for part in self.parts:
+            mine = slots[(slots >= part.b0) & (slots < part.b0 + part.nb)] - part.b0
+            if len(mine):
+                part.brain.reset(mine)
+                part.motor.reset(mine)
+                part.eye.reset(mine)
 
     def exam(self, state, frames: int, on_frame=None, log_every: int = 0, record: bool = False,
              retire: bool = True) -> list[dict]:
         """flyzero/fleet.py:87-127: every slot drives alone from ``state`` (no learning), with its
         own spiking noise.  ``record``: each result also holds the drive's button masks and DN
-        rates.  ``retire``: a slot whose race has ended stops costing brain time (its results
-        are final; the other slots' spikes are unchanged by it)."""
-        self.brain.reset()
-        self.motor.reset()
-        self.eye.reset()
-        frames_now, infos = self.pool.load([state] * self.B)
-        progs = [Progress(self.segments) for _ in range(self.B)]
-        done = np.zeros(self.B, bool)
-        res = [None] * self.B
-        masks, dn = [[] for _ in range(self.B)], [[] for _ in range(self.B)]
+        rates.  ``retire``: a slot whose race has ended stops costing brain time from the next
+        window but one (its results are final; the other slots' spikes are unchanged by it).
+        Each part's next window is queued as soon as its games have stepped, so with several
+        parts the device simulates one part while the host serves the next.  ``on_frame(i,
+        counts, buttons, infos)``: counts are the (B, n) device counts with one part, None with
+        several."""
+        B = self.B
+        for part in self.parts:
+            part.brain.reset()
+            part.motor.reset()
+            part.eye.reset()
+        frames_now, infos = self.pool.load([state] * B)
+        progs = [Progress(self.segments) for _ in range(B)]
+        done = np.zeros(B, bool)
+        res = [None] * B
+        masks, dn = [[] for _ in range(B)], [[] for _ in range(B)]
+        copy = on_frame is not None
+        for part in self.parts:
+            part.launch(frames_now[part.b0:part.b0 + part.nb], self.window, copy)
         t0 = time.time()
         try:
             for i in range(frames):
-                counts = self.think(frames_now, copy=on_frame is not None)
-                buttons = self.motor.update(counts, self.window)
-                if record:
-                    for k in np.flatnonzero(~done):
-                        masks[k].append(buttons_to_mask(buttons[k]))
-                        dn[k].append(self.motor.rates[k].astype(np.float16))
-                frames_now, infos = self.pool.step(buttons)
+                buttons, infos = [None] * B, [None] * B
+                counts = None
+                for part in self.parts:
+                    bt = part.collect()
+                    sl = slice(part.b0, part.b0 + part.nb)
+                    buttons[sl] = bt
+                    if record:
+                        for k in part.slots:
+                            if not done[k]:
+                                masks[k].append(buttons_to_mask(bt[k - part.b0]))
+                                dn[k].append(part.motor.rates[k - part.b0].astype(np.float16))
+                    counts = part.counts
+                    if len(self.parts) == 1:    # the reference's pool signature
+                        fr, infos[sl] = self.pool.step(bt)
+                    else:
+                        fr, infos[sl] = self.pool.step(bt, which=part.slots)
+                    if i + 1 < frames:
+                        part.launch(fr, self.window, copy)
                 newly = []
                 for k, info in enumerate(infos):
                     if done[k]:
@@ -216,16 +304,20 @@ class Fleet:
                         if record:
                             res[k]["masks"], res[k]["rates"] = np.array(masks[k]), np.array(dn[k])
                 if retire and newly:
-                    self.brain.alive[torch.as_tensor(newly, device=self.device)] = 0
+                    for part in self.parts:
+                        mine = [k - part.b0 for k in newly if part.b0 <= k < part.b0 + part.nb]
+                        if mine:
+                            part.brain.alive[torch.as_tensor(mine, device=part.device)] = 0
                 if on_frame:
-                    on_frame(i, counts, buttons, infos)
+                    on_frame(i, counts if len(self.parts) == 1 else None, buttons, infos)
                 if log_every and (i + 1) % log_every == 0:
-                    print(f"  frame {i + 1}: {(i + 1) * self.B / (time.time() - t0):.0f} fly-frames/s, "
+                    print(f"  frame {i + 1}: {(i + 1) * B / (time.time() - t0):.0f} fly-frames/s, "
                           f"progress {[p.total for p in progs]}", flush=True)
                 if done.all():
                     break
         finally:
-            self.brain.alive.fill_(1)
+            for part in self.parts:
+                part.brain.alive.fill_(1)
         return res
 
 
@@ -338,13 +430,15 @@ class SyntheticRace:
 
     KEYS = ("LEFT", "RIGHT", "L", "R", "B", "Y")
 
-    def step(self, buttons):
-        if len(buttons) != self.n:
-            raise ValueError(f"one button dict per game ({self.n}), got {len(buttons)}")
-        P = np.array([[b.get(k, False) for k in self.KEYS] for b in buttons], np.float64).reshape(self.n, 6)
+    def step(self, buttons, which=None):
+        """Advance the games ``which`` (default all) by one frame with one button dict each."""
+        which = np.arange(self.n) if which is None else np.asarray(list(which), np.int64)
+        if len(buttons) != len(which):
+            raise ValueError(f"one button dict per game ({len(which)}), got {len(buttons)}")
+        P = np.array([[b.get(k, False) for k in self.KEYS] for b in buttons], np.float64).reshape(len(which), 6)
         steer = (P[:, 0] - P[:, 1]) + 0.5 * (P[:, 2] - P[:, 3])
-        s, x, v, e, f = (self.st[:, i].copy() for i in range(5))
-        live = ~self._done(np.arange(self.n))[0]
+        s, x, v, e, f = (self.st[which, i].copy() for i in range(5))
+        live = ~self._done(which)[0]
         v = np.where(P[:, 4] > 0, np.minimum(v + 0.004, 0.25), np.where(P[:, 5] > 0, v - 0.01, v - 0.001))
         v = np.maximum(v, 0.0)
         x = x + 0.6 * v * self.kappa(s) - 0.05 * (0.2 + 4.0 * v) * steer
@@ -354,8 +448,8 @@ class SyntheticRace:
         x = np.clip(x, -2.0, 2.0)
         s = s + v
         new = np.stack([s, x, v, e, f + 1], 1)
-        self.st[live] = new[live]
-        return self.render(), self._infos(range(self.n))
+        self.st[which[live]] = new[live]
+        return self.render(which if len(which) < self.n else None), self._infos(which)
 
     def close(self):
         pass
@@ -429,11 +523,19 @@ class GamePool:
         which = list(range(self.n)) if which is None else list(which)
         flips = flips or [False] * len(which)
         infos = self._all([("load", s, f) for s, f in zip(states, flips)], which)
-        return (self.frames if len(which) == self.n and which == list(range(self.n)) else self.frames[which]), infos
+        return self._view(which), infos
 
-    def step(self, buttons):
-        infos = self._all([("step", b) for b in buttons])
-        return self.frames, infos
+    def step(self, buttons, which=None):
+        """One frame of the games ``which`` (default all); returns their frames (a view of the
+        shared block for a contiguous range) and telemetry."""
+        which = list(range(self.n)) if which is None else list(which)
+        infos = self._all([("step", b) for b in buttons], which)
+        return self._view(which), infos
+
+    def _view(self, which):
+        if which and which == list(range(which[0], which[-1] + 1)):
+            return self.frames[which[0]:which[-1] + 1]
+        return self.frames[which]
 
     def save(self, which=None):
         which = list(range(self.n)) if which is None else list(which)

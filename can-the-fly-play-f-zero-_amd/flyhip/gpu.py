@@ -120,6 +120,9 @@ class BatchBrain:
         self.pw = torch.from_numpy(np.tile(self.w0, (batch, 1)).astype(np.float32)).to(dev)
 
         self.seed = np.uint32(seed)
+        # the Poisson draw numbers fly b as b + fly_base: a brain holding flies [fly_base,
+        # fly_base + batch) of a larger batch draws the same noise as that batch does for them
+        self.fly_base = 0
         self._step_no = 0
         self._t = 0
         # neurons that take outside input (Poisson rates and/or a bias): "ext" slots
@@ -275,7 +278,7 @@ class BatchBrain:
         return _lib.FzInputs(n_ext, _lib.ptr(self.d_ext_idx) if n_ext else None,
                              _lib.ptr(self.p_ext) if n_ext else None, _lib.ptr(self.bias_ext) if n_ext else None,
                              _lib.ptr(self.pw) if len(self.plastic_pos) else None, _lib.ptr(self.alive),
-                             int(self.seed))
+                             int(self.seed), int(self.fly_base) & 0xFFFFFFFF)
 
     def _advance(self, steps: int):
         self._t += steps
@@ -340,7 +343,7 @@ class BatchBrain:
         self._drop_graphs()
 
     def _graph(self, steps: int, inp: _lib.FzInputs):
-        key = (steps, inp.n_ext, inp.ext_idx, inp.p_ext, inp.bias_ext, inp.pw, inp.alive, inp.seed,
+        key = (steps, inp.n_ext, inp.ext_idx, inp.p_ext, inp.bias_ext, inp.pw, inp.alive, inp.seed, inp.fly_base,
                tuple(getattr(self.tune, f) for f, _ in _lib.FzTune._fields_))
         gr = self._graphs.get(key)
         if gr is None:

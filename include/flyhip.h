@@ -67,6 +67,9 @@ typedef struct fz_inputs {
     const float* pw;            /* [B][n_pl] plastic weights, mV (NULL if n_pl == 0) */
     const unsigned char* alive; /* [B] 0 = retired fly (skipped), or NULL = all alive */
     unsigned seed;              /* RNG seed (flyzero/gpu.py:172) */
+    unsigned fly_base;          /* the draw numbers fly b as b + fly_base (0 in the reference): a
+                                 * brain holding flies [fly_base, fly_base + B) of a larger batch
+                                 * draws the same noise as that batch does for them */
 } fz_inputs;
 
 /* Launch shape (chosen by the autotuner; zero fields take defaults). */

@@ -278,3 +278,36 @@ def test_exam_with_games_in_subprocesses_equals_the_device_game():
         assert (a["frames"], a["progress"], a["energy"]) == (b["frames"], b["progress"], b["energy"])
         np.testing.assert_array_equal(a["masks"], b["masks"])
         np.testing.assert_array_equal(a["rates"], b["rates"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("pool", ["device", "processes"])
+def test_a_fleet_in_parts_drives_exactly_like_one_brain(pool):
+    # two brains of 2 flies each, run in turn with the games of one part stepping while the other
+    # part simulates, against one brain of 4 flies: same noise per fly, same results
+    from functools import partial
+
+    from flyhip.fleet import GamePool
+
+    start = SyntheticRace.start_state(x=0.5, v=0.12, energy=2.5)
+
+    def make():
+        if pool == "device":
+            return SyntheticRace(4, device="cuda", max_frames=11)
+        return GamePool(partial(SyntheticRace, 1, max_frames=11), 4)
+
+    pa, pb = make(), make()
+    try:
+        fa, _ = _small_fleet(4, 9, pa, parts=2)
+        fb, _ = _small_fleet(4, 9, pb)
+        assert fa.brain is None and [p.nb for p in fa.parts] == [2, 2]
+        ra = fa.exam(start, 12, record=True)
+        rb = fb.exam(start, 12, record=True)
+    finally:
+        for p in (pa, pb):
+            p.close()
+    for a, b in zip(ra, rb):
+        assert (a["frames"], a["progress"], a["energy"]) == (b["frames"], b["progress"], b["energy"])
+        np.testing.assert_array_equal(a["masks"], b["masks"])
+        np.testing.assert_array_equal(a["rates"], b["rates"])
+    assert sum(int(r["masks"].sum()) for r in ra) > 0

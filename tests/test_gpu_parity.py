@@ -178,6 +178,22 @@ def test_hash_rng_frames_graph_and_eager_match_oracle(small_fly, engine):
 
 
 @pytest.mark.parametrize("engine", ENGINES)
+def test_a_brain_of_a_slice_of_flies_draws_that_slice_s_noise(small_fly, engine):
+    # fly_base: flies [2, 4) of a 4-fly batch in a brain of their own get the same Poisson draws
+    # (hash key seed + fly, neuron key fly * N + i), hence the same spikes
+    whole, _ = _pair(small_fly, batch=4, seed=7, engine=engine)
+    lo, _ = _pair(small_fly, batch=2, seed=7, engine=engine)
+    hi, _ = _pair(small_fly, batch=2, seed=7, engine=engine)
+    hi.fly_base = 2
+    for frame in range(3):
+        cw = _np(whole.run(FRAME_MS))
+        np.testing.assert_array_equal(_np(lo.run(FRAME_MS)), cw[:2], err_msg=f"frame {frame}")
+        np.testing.assert_array_equal(_np(hi.run(FRAME_MS)), cw[2:], err_msg=f"frame {frame}")
+    assert cw.sum() > 0
+    np.testing.assert_array_equal(_np(hi.u), _np(whole.u[2:]))
+
+
+@pytest.mark.parametrize("engine", ENGINES)
 def test_retired_flies_are_skipped_and_others_unaffected(small_fly, engine):
     gb, ob = _pair(small_fly, batch=4, seed=2, engine=engine)
     gb.run(FRAME_MS)
