@@ -169,17 +169,25 @@ int fz_eye_step(const fz_eye_args* a, void* stream) {
         return fzi_fail(FZ_E_ARG, "the feature channels need a positive gain");
     const int planes = 4 + (a->su_gain != 0.0f ? 1 : 0) + (a->ch_gain != 0.0f ? 2 : 0);
     const size_t lds = (size_t)planes * a->gh * a->gw * sizeof(float);
-    int dev = 0, max_lds = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
-        return fzi_fail(FZ_E_HIP, "device query failed");
-    if (lds > (size_t)max_lds)
-        return fzi_fail(FZ_E_ARG, "grid %d x %d needs %zu B of LDS (device: %d)", a->gh, a->gw, lds, max_lds);
-    if (lds > 65536 && hipFuncSetAttribute(reinterpret_cast<const void*>(&eye_step),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) {
-        (void)hipGetLastError();
-        return fzi_fail(FZ_E_HIP, "hipFuncSetAttribute(eye LDS %zu) failed", lds);
+    // the device's LDS limit is queried, and the kernel's dynamic-LDS cap raised to it, once
+    // per device (the attribute call is slow; doing it every frame cost more than the kernel)
+    constexpr int MAX_DEV = 64;
+    static int max_lds_of[MAX_DEV] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEV) return fzi_fail(FZ_E_HIP, "device query failed");
+    if (max_lds_of[dev] == 0) {
+        int max_lds = 0;
+        if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+            return fzi_fail(FZ_E_HIP, "device query failed");
+        if (max_lds > 65536 && hipFuncSetAttribute(reinterpret_cast<const void*>(&eye_step),
+                                                   hipFuncAttributeMaxDynamicSharedMemorySize, max_lds) != hipSuccess) {
+            (void)hipGetLastError();
+            return fzi_fail(FZ_E_HIP, "hipFuncSetAttribute(eye LDS %d) failed", max_lds);
+        }
+        max_lds_of[dev] = max_lds;
     }
+    if (lds > (size_t)max_lds_of[dev])
+        return fzi_fail(FZ_E_ARG, "grid %d x %d needs %zu B of LDS (device: %d)", a->gh, a->gw, lds, max_lds_of[dev]);
     hipLaunchKernelGGL(eye_step, dim3(a->B), dim3(EYE_THREADS), lds, static_cast<hipStream_t>(stream), *a);
     if (hipGetLastError() != hipSuccess) return fzi_fail(FZ_E_HIP, "eye_step launch failed");
     return FZ_OK;

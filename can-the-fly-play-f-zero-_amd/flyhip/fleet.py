@@ -95,7 +95,10 @@ class _Part:
         self.brain = BatchBrain(conn.weights, LIFParams(dt=0.25), batch=nb, seed=seed, plastic_pos=fleet.pos,
                                 engine=engine, tile=tile, device=device)
         self.brain.fly_base = b0
-        dev = self.device = self.brain.device
+        dev = self.brain.device
+        if dev.type == "cuda" and dev.index is None:   # "cuda" != "cuda:0" in comparisons
+            dev = torch.device("cuda", torch.cuda.current_device())
+        self.device = dev
         self.eye = BatchMotionEye.for_fly(conn, FRAME_SHAPE, nb, vision, device=dev)
         self.motor = BatchMotor(conn, nb, motor_params, device=dev)
         self.brain.set_bias(fleet.bias_idx, fleet.bias_mv)
@@ -306,8 +309,9 @@ class Fleet:
                 if retire and newly:
                     for part in self.parts:
                         mine = [k - part.b0 for k in newly if part.b0 <= k < part.b0 + part.nb]
-                        if mine:
-                            part.brain.alive[torch.as_tensor(mine, device=part.device)] = 0
+                        if mine:   # a pinned, asynchronous index upload: no wait for queued work
+                            ix = torch.tensor(mine, dtype=torch.long).pin_memory().to(part.device, non_blocking=True)
+                            part.brain.alive[ix] = 0
                 if on_frame:
                     on_frame(i, counts if len(self.parts) == 1 else None, buttons, infos)
                 if log_every and (i + 1) % log_every == 0:
@@ -399,19 +403,26 @@ class SyntheticRace:
         return left, right, edge, shade
 
     def render(self, which=None) -> torch.Tensor:
-        left, right, edge, shade = (torch.from_numpy(a.astype(np.int32)).to(self.device)[..., None]
-                                    for a in self.rows(which))
+        """The views of players ``which`` (default all), uint8 (m, h, w, 3) on ``device``.  The
+        row tables and flip flags go up in one pinned, asynchronous copy, so drawing does not
+        wait for work already queued on the device."""
+        idx = np.arange(self.n) if which is None else np.asarray(which)
+        tabs = np.stack(self.rows(idx)).astype(np.int32)            # (4, m, rows)
+        flips = self.flip[idx]
+        packed = torch.from_numpy(np.concatenate([tabs.ravel(), flips.astype(np.int32)]))
+        if self.device.type == "cuda":
+            packed = packed.pin_memory().to(self.device, non_blocking=True)
+        m, R = tabs.shape[1], tabs.shape[2]
+        left, right, edge, shade = (packed[k * m * R:(k + 1) * m * R].view(m, R)[..., None] for k in range(4))
         c = self._cols
         road = (c >= left) & (c < right)
         rim = road & ((c < left + edge) | (c >= right - edge))
-        idx = torch.where(rim, 5, torch.where(road, 3 + shade, 1 + shade))
-        ground = self._pal[idx.long()]
-        m = ground.shape[0]
+        col = torch.where(rim, 5, torch.where(road, 3 + shade, 1 + shade))
+        ground = self._pal[col.long()]
         sky = self._pal[0].expand(m, self.SKY, self.w, 3)
         img = torch.cat([sky, ground], 1)
-        flip = torch.from_numpy(self.flip[np.arange(self.n) if which is None else np.asarray(which)])
-        if flip.any():
-            f = flip.to(self.device)
+        if flips.any():
+            f = packed[4 * m * R:].bool()
             img = torch.where(f[:, None, None, None], img.flip(2), img)
         return img.contiguous()
 

@@ -164,6 +164,7 @@ def test_think_is_eye_then_brain_and_host_frames_match_device_frames():
     eye = BatchMotionEye.for_fly(fly, FRAME_SHAPE, B, vision_params())
     frames, _ = race.load([SyntheticRace.start_state(v=0.1)] * B)
     host, _ = twin.pool.load([SyntheticRace.start_state(v=0.1)] * B)
+    assert fleet.frames_on_device(frames) is frames          # device frames are used in place
     total = 0
     for t in range(6):
         c1 = fleet.think(frames).cpu().numpy()
