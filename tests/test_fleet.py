@@ -140,11 +140,42 @@ def test_game_pool_runs_one_game_per_process_like_the_batched_race():
 
 # ---------------------------------------------------------------- GPU
 
-def _small_fleet(B, seed, pool, **kw):
+def _schema_connectome(n=5000, seed=0):
+    """A small connectome in the FlyWire schema (flywire.Connectome): cell types, sides and
+    positions for the eye's T4/T5, medulla and R7/R8 cells and the descending neurons, random
+    Dale-signed whole-synapse counts."""
+    import pandas as pd
+    import scipy.sparse as sp
+
+    from flyhip.flywire import Connectome
+
+    rng = np.random.default_rng(seed)
+    types = ([f"T{p}{d}" for p in (4, 5) for d in "abcd"] * 60 + ["Mi1", "Tm3", "Tm1", "Tm2", "Tm4", "Mi4", "Mi9",
+                                                                   "Tm9", "R7", "R8"] * 30)
+    dns = ["DNa02", "DNa01", "DNg02_a", "DNp09", "MDN", "DNp01"] * 4
+    ct = np.array(types + dns + [None] * (n - len(types) - len(dns)), dtype=object)
+    side = np.where(rng.random(n) < 0.5, "left", "right").astype(object)
+    pos = rng.random((n, 3)) * 1000
+    pos[:, 0] = np.where(side == "left", 600 + 300 * rng.random(n), 100 + 300 * rng.random(n))
+    neurons = pd.DataFrame({"root_id": np.arange(n), "cell_type": pd.array(ct, dtype="string"),
+                            "hemibrain_type": pd.array([None] * n, dtype="string"),
+                            "side": pd.array(side, dtype="string"), "pos_x": pos[:, 0], "pos_y": pos[:, 1],
+                            "pos_z": pos[:, 2]})
+    w = sp.random(n, n, density=0.015, format="csr", random_state=seed, dtype=np.float32)
+    w.data = np.rint(1 + 5 * w.data).astype(np.float32)
+    sign = np.where(rng.random(n) < 0.6, 1.0, -1.0).astype(np.float32)
+    w = sp.csr_matrix(sp.diags(sign) @ w)
+    w.setdiag(0)
+    w.eliminate_zeros()
+    w.sort_indices()
+    return Connectome(w, neurons, name="schema test")
+
+
+def _small_fleet(B, seed, pool, conn=None, **kw):
     from flyhip.connectome import flywire_like
     from flyhip.fleet import Fleet
 
-    fly = flywire_like(n=6000, n_edges=6000 * 50, seed=4)
+    fly = conn if conn is not None else flywire_like(n=6000, n_edges=6000 * 50, seed=4)
     return Fleet(pool, B, seed=seed, conn=fly, tile=512, tune=False, **kw), fly
 
 
@@ -187,8 +218,10 @@ def test_think_is_eye_then_brain_and_host_frames_match_device_frames():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("taps", [False, True])
-def test_exam_matches_the_cpu_chain_of_oracles(taps):
+@pytest.mark.parametrize("taps,schema", [(False, False), (True, False), (False, True)])
+def test_exam_matches_the_cpu_chain_of_oracles(taps, schema):
+    # schema: a connectome in the FlyWire schema, whose eye comes from cell types and the
+    # retinotopy of neuron positions
     from flyhip.fleet import Fleet  # noqa: F401
     from oracle.eye_oracle import OracleEye
     from oracle.oracle import OracleBrain
@@ -197,7 +230,9 @@ def test_exam_matches_the_cpu_chain_of_oracles(taps):
     B, frames_n = 3, 14
     start = SyntheticRace.start_state(x=0.6, v=0.15, energy=3.0)
     fleet, fly = _small_fleet(B, 11, SyntheticRace(B, device="cuda", max_frames=12), taps=taps,
-                              steer_threshold=4.0, lean_threshold=4.0)
+                              steer_threshold=4.0, lean_threshold=4.0, conn=_schema_connectome() if schema else None)
+    if schema:
+        assert len(fleet.eye.idx) == 8 * 60 + 10 * 30 and set(fleet.eye.key) == set(range(14))
     got_buttons = []
     res = fleet.exam(start, frames_n, on_frame=lambda i, c, b, inf: got_buttons.append(b), record=True)
 
@@ -242,7 +277,7 @@ def test_exam_matches_the_cpu_chain_of_oracles(taps):
         assert (res[k]["progress"], res[k]["lap"], res[k]["frames"], res[k]["energy"]) == want[k], k
         assert res[k]["masks"].shape == (res[k]["frames"], 12)
     pressed = sum(int(m.sum()) for r in res for m in r["masks"])
-    assert pressed > 0 and fleet.brain.alive.all()
+    assert (pressed > 0 or schema) and fleet.brain.alive.all()
 
 
 @pytest.mark.gpu
