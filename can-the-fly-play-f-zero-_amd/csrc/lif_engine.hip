@@ -528,6 +528,37 @@ __device__ __forceinline__ int block_incl_scan(int v, int* wsum, int* total) {
     return x + wsum[wave];
 }
 
+// The step loop of a block for the case n_ref >= K (at most one spike per block, the refractory
+// counter carried as the block step rend at which it runs out; see the kernel).  KC > 0 is the
+// block length known at compile time: the loop unrolls, the LDS reads take immediate offsets
+// and the step index is a constant in every compare.  The float operations and their order are
+// those of the runtime loop (KC = 0), so the two give the same bits.
+template <int KC>
+__device__ __forceinline__ int once_steps(const int* row, int K, float& u_io, float& g_io, int rend, float bias,
+                                          uint32_t hits, const Consts& c) {
+    float u = u_io, g = g_io;
+    auto step = [&](int k) {
+        const float inj = (float)row[k] * c.in_scale;
+        g = g * c.decay_syn + inj;
+        const float u_int = u + (g + bias - u) * c.decay_m;
+        u = k < rend ? u : u_int;
+        const float u_kick = u + c.kick;
+        u = ((hits >> k) & 1u) ? u_kick : u;
+        const bool sp = u >= c.u_th && k + 1 >= rend;
+        u = sp ? c.u_reset : u;
+        rend = sp ? k + 1 + c.n_ref : rend;
+    };
+    if constexpr (KC > 0) {
+#pragma unroll
+        for (int k = 0; k < KC; k++) step(k);
+    } else {
+        for (int k = 0; k < K; k++) step(k);
+    }
+    u_io = u;
+    g_io = g;
+    return rend;
+}
+
 // IPT: spike-list entries staged per thread (more loads in flight, more LDS).
 // BK: segments come from the tile's bucket (filled by tile_bucket) instead of being looked up
 // in the segment table for every spike of the fly.
@@ -847,18 +878,11 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
             int r = r_[p];
             const float bias = bias_[p];
             if (once) {
-                int rend = r;
-                for (int k = 0; k < a.K; k++) {
-                    const float inj = (float)row[k] * a.c.in_scale;
-                    g = g * a.c.decay_syn + inj;
-                    const float u_int = u + (g + bias - u) * a.c.decay_m;
-                    u = k < rend ? u : u_int;
-                    const float u_kick = u + a.c.kick;
-                    u = ((hits >> k) & 1u) ? u_kick : u;
-                    const bool sp = u >= a.c.u_th && k + 1 >= rend;
-                    u = sp ? a.c.u_reset : u;
-                    rend = sp ? k + 1 + a.c.n_ref : rend;
-                }
+                int rend;
+                // the block lengths of dt = 0.1 ms and 0.25 ms (n_delay 18 and 7) unrolled
+                if (a.K == 18) rend = once_steps<18>(row, a.K, u, g, r, bias, hits, a.c);
+                else if (a.K == 7) rend = once_steps<7>(row, a.K, u, g, r, bias, hits, a.c);
+                else rend = once_steps<0>(row, a.K, u, g, r, bias, hits, a.c);
                 if (rend != r) mask = 1u << (rend - 1 - a.c.n_ref);
                 r = rend - min(max(rend, 0), a.K);  // the counter after the last step
             } else {
