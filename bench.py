@@ -285,6 +285,7 @@ def fleet_line(fly, torch, B=256, frames=60, warmup=3, parts=1):
     copy an emulator pool's host frames would add is timed on its own.  ``parts`` > 1 splits the
     flies into that many brains served in turn (the host steps one part's games while the device
     simulates the next)."""
+    from flyhip import autotune
     from flyhip.fleet import Fleet, SyntheticRace
 
     class Timed:
@@ -343,7 +344,8 @@ def fleet_line(fly, torch, B=256, frames=60, warmup=3, parts=1):
            "fly_sim_s_per_s": round(B * frames / loop_s * fleet.window / 1000.0, 2),
            "brain_ms_per_frame": round(brain_s / frames * 1000.0, 3) if brain_s else None,
            "game_ms_per_frame": round(game_s / frames * 1000.0, 3),
-           "host_frames_copy_ms": round(h2d_ms, 3) if h2d_ms else None, "mean_progress": summary["mean_progress"]}
+           "host_frames_copy_ms": round(h2d_ms, 3) if h2d_ms else None, "mean_progress": summary["mean_progress"],
+           "launch_shapes": [autotune.current(part.brain) for part in fleet.parts]}
     del fleet, race, timed
     torch.cuda.empty_cache()
     return out

@@ -549,8 +549,29 @@ __device__ __forceinline__ int once_steps(const int* row, int K, float& u_io, fl
         rend = sp ? k + 1 + c.n_ref : rend;
     };
     if constexpr (KC > 0) {
+        // The same steps with two VALU operations fewer each: "held" (k < rend) is carried from
+        // the previous step, where it was computed for the spike test: k < rend_new is
+        // (k < rend_old) unless a spike just moved rend_new to k + n_ref > k (n_ref >= K >= 1);
+        // and rend is kept as rel = rend - n_ref, so the spike sets it to the inline
+        // constant k + 1 (no constant-bus move into a VGPR).
+        int rel = rend - c.n_ref;
+        bool held = 0 < rend;
 #pragma unroll
-        for (int k = 0; k < KC; k++) step(k);
+        for (int k = 0; k < KC; k++) {
+            const float inj = (float)row[k] * c.in_scale;
+            g = g * c.decay_syn + inj;
+            const float u_int = u + (g + bias - u) * c.decay_m;
+            u = held ? u : u_int;
+            const float u_kick = u + c.kick;
+            u = ((hits >> k) & 1u) ? u_kick : u;
+            const bool held_next = k + 1 - c.n_ref < rel;  // k + 1 < rend
+            const bool ge = u >= c.u_th;
+            const bool sp = ge & !held_next;
+            u = sp ? c.u_reset : u;
+            rel = sp ? k + 1 : rel;
+            held = held_next | ge;                          // = held_next || sp
+        }
+        rend = rel + c.n_ref;
     } else {
         for (int k = 0; k < K; k++) step(k);
     }

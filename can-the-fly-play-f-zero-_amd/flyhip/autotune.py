@@ -1,11 +1,13 @@
 """Launch-shape autotuner for the LIF engine.
 
 Times candidate ``fz_tune`` shapes (update workgroup size, push workgroup size, push edge
-chunk, push grid) with HIP events on real windows of the caller's own workload, then restores
-the brain's state, so tuning never advances the simulation.  Search is coordinate descent:
+chunk, push grid) with HIP events on real windows of the caller's own workload, replayed as
+the same HIP graphs ``run`` uses, then restores the brain's state, so tuning never advances
+the simulation.  Search is coordinate descent:
 the update kernel's shape first, then the push kernel's, each over a small grid of shapes that
 respect the kernels' limits (multiples of 64 threads, chunk >= 64 edges).  Results are cached
-per (device, neurons, connections, flies, plastic synapses) in memory and in a JSON file.
+per (device, neurons, connections, flies, plastic synapses, delay steps) in memory and in a
+JSON file.
 """
 
 from __future__ import annotations
@@ -29,7 +31,8 @@ PUSH_GRID_PER_CU = (2, 4, 8, 16)
 def _key(bb) -> str:
     name = torch.cuda.get_device_name(bb.device)
     eng = bb.engine + (str(bb.tile) if bb.engine == "tile" else "")
-    return f"{name}|{eng}|n={bb.n}|nnz={int(bb.d_col.numel())}|B={bb.B}|npl={len(bb.plastic_pos)}"
+    return (f"{name}|{eng}|n={bb.n}|nnz={int(bb.d_col.numel())}|B={bb.B}|npl={len(bb.plastic_pos)}"
+            f"|nd={bb.n_delay}")
 
 
 def _state(bb):
@@ -52,16 +55,26 @@ def _restore(bb, snap):
 
 
 def _time(bb, shape: dict, steps: int, reps: int, snap) -> float:
-    """Best of ``reps`` windows; inf for a shape the engine rejects (e.g. LDS over the limit)."""
+    """Best of ``reps`` windows replayed as the HIP graph that ``run`` uses (the graph's
+    concurrent fly-half chains overlap kernels that eager launches serialise, so eager timings
+    can rank the shapes wrongly), timed with HIP events on the engine's stream; inf for a shape
+    the engine rejects (e.g. LDS over the limit)."""
     bb.set_tune(**shape)
+    ms = steps * bb.p.dt
+    try:
+        _restore(bb, snap)
+        bb.run(ms, copy=False)  # graph capture and first-launch costs stay out of the timing
+    except ValueError:
+        return float("inf")
     best = float("inf")
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     for _ in range(reps):
         _restore(bb, snap)
-        try:
-            r = bb.run_timed(steps)
-        except ValueError:
-            return float("inf")
-        best = min(best, r["update_ms"] + r["push_ms"])
+        e0.record()
+        bb.run(ms, copy=False)
+        e1.record()
+        e1.synchronize()
+        best = min(best, e0.elapsed_time(e1))
     return best
 
 
