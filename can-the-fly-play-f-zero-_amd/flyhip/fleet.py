@@ -552,7 +552,16 @@ class GamePool:
         which = list(range(self.n)) if which is None else list(which)
         return self._all([("save",)] * len(which), which)
 
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
     def close(self):
+        """Stop the workers and release the shared frame block (idempotent)."""
+        if self._shm is None:
+            return
         for p in self.pipes:
             try:
                 p.send(("close",))
@@ -571,3 +580,4 @@ class GamePool:
         except BufferError:       # a caller still holds a frames view; the mapping goes with it
             pass
         self._shm.unlink()
+        self._shm = None

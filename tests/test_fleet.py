@@ -136,6 +136,10 @@ def test_game_pool_runs_one_game_per_process_like_the_batched_race():
     finally:
         pool.close()
     assert all(not p.is_alive() for p in pool.procs)
+    pool.close()                                   # idempotent
+    with GamePool(partial(SyntheticRace, 1), 1) as one:
+        f, info = one.load([SyntheticRace.start_state()])
+        assert f.shape == (1, H, W, 3) and info[0]["segment"] == 0
 
 
 # ---------------------------------------------------------------- GPU
