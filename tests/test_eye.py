@@ -282,3 +282,25 @@ def test_batched_eye_matches_oracle_and_feeds_the_brain(channels):
     np.testing.assert_array_equal(gb.p_ext.cpu().numpy()[:, slots], got * f32(gb.p.dt / 1000.0))
     counts = gb.run(16.6).cpu().numpy()
     assert counts.sum() > 0
+
+
+@pytest.mark.gpu
+def test_batched_eye_on_a_frame_that_is_not_a_multiple_of_the_cell():
+    # 101 x 131 pixels, cell 4: a 25 x 32 grid, the last rows and columns of pixels unused
+    import torch
+
+    from flyhip.connectome import flywire_like
+    from flyhip.eye import BatchMotionEye
+
+    fly = flywire_like(n=3000, n_edges=3000 * 40, seed=8)
+    shape = (101, 131)
+    p = MotionParams(gain=300.0, surround=2, **VISION)
+    eye = BatchMotionEye.for_fly(fly, shape, 2, p)
+    assert (eye.gh, eye.gw) == (25, 32)
+    oracles = [OracleEye(eye.key, eye.gy, eye.gx, eye.py, shape, gain=300.0, surround=2, **VISION) for _ in range(2)]
+    rng = np.random.default_rng(2)
+    for t in range(4):
+        frames = rng.integers(0, 256, (2,) + shape + (3,)).astype(np.uint8)
+        got = eye.see(torch.from_numpy(frames).cuda()).cpu().numpy()
+        for b in range(2):
+            np.testing.assert_array_equal(got[b], oracles[b].see(frames[b]), err_msg=f"frame {t} fly {b}")
