@@ -7,7 +7,8 @@
 //   e_k     = ae * e_k + c_pre * dev_k            (eligibility, stored float32)
 //   <post> += ap * (c_post - <post>)
 //   adv     = R - <R>,  <R> += ar * adv
-//   w_k    += mean over the fly's learning slots of eta * adv * e_k
+//   w_k    += mean over the fly's learning slots of eta_b * adv * e_k   (eta_b: one eta for every
+//             slot, flyzero/batch.py:494, or a per-slot rate, flyzero/school.py:289-304)
 //   w_k     = sign * clip(sign * w_k, 0, wmax_k)  (Dale's law and cap)
 //
 // Weights are kept in synapse units in float64 and the simulation uses the integer count
@@ -47,7 +48,8 @@ __global__ void reward_synapses(fz_reward_args a) {
         a.elig[bk] = ef;
         if (a.learn[b]) {
             const double adv = a.reward[b] - a.r_mean[b];
-            acc += a.eta * adv * (double)ef;
+            const double eta = a.eta_slot ? a.eta_slot[b] : a.eta;  // school.py:300-302 scales per slot
+            acc += eta * adv * (double)ef;
             nact++;
         }
     }

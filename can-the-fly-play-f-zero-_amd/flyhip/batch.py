@@ -44,7 +44,7 @@ class FzRewardArgs(ctypes.Structure):
                                                "w", "count", "pw", "post_mean", "r_mean", "reward", "learn",
                                                "fly_ptr", "fly_slots")] + \
                [("ae", ctypes.c_double), ("ap", ctypes.c_double), ("ar", ctypes.c_double), ("eta", ctypes.c_double),
-                ("w_syn", ctypes.c_float), ("adv_out", ctypes.c_void_p)]
+                ("w_syn", ctypes.c_float), ("adv_out", ctypes.c_void_p), ("eta_slot", ctypes.c_void_p)]
 
 
 def rule_constants(p: LearnParams, window_ms: float, w_syn: float) -> dict:
@@ -101,6 +101,7 @@ class BatchReward:
         self._reward = torch.zeros(self.B, dtype=torch.float64, device=dev)
         self._learn = torch.zeros(self.B, dtype=torch.uint8, device=dev)
         self._adv = torch.zeros(self.B, dtype=torch.float64, device=dev)
+        self._eta_slot = torch.zeros(self.B, dtype=torch.float64, device=dev)
 
     def reset(self, slots):
         s = np.atleast_1d(slots)
@@ -113,10 +114,13 @@ class BatchReward:
         self.noise += -a * self.noise + np.sqrt(2 * a) * self.rng.standard_normal(self.noise.shape)
         return np.maximum(self.p.noise_hz * self.noise, 0.0)
 
-    def step(self, counts: torch.Tensor, reward, learn, window_ms: float, return_adv: bool = True):
+    def step(self, counts: torch.Tensor, reward, learn, window_ms: float, return_adv: bool = True,
+             eta_slot=None):
         """One frame: traces, means and (if any slot learns) weights.  ``counts``: the window's
-        (B, n) int32 device counts; ``reward``, ``learn``: per slot.  Returns the advantage (a
-        host copy, which waits for the device; ``return_adv=False`` returns None instead)."""
+        (B, n) int32 device counts; ``reward``, ``learn``: per slot; ``eta_slot``: None (every slot
+        learns at ``p.eta``) or a per-slot learning rate in mV per unit, like ``p.eta``
+        (flyzero/school.py:289-304).  Returns the advantage (a host copy, which waits for the
+        device; ``return_adv=False`` returns None instead)."""
         brain = self.brain
         if counts.dtype != torch.int32 or not counts.is_contiguous() or tuple(counts.shape) != (self.B, brain.n):
             raise ValueError("counts must be the window's contiguous int32 (B, n) device counts")
@@ -127,6 +131,9 @@ class BatchReward:
             self._reward.copy_(torch.from_numpy(np.broadcast_to(np.asarray(reward, np.float64), (self.B,)).copy()))
         self._learn.copy_(torch.from_numpy(learn.astype(np.uint8)))
         c = rule_constants(self.p, window_ms, brain.p.w_syn)
+        if eta_slot is not None:   # mV -> synapses per unit, as rule_constants does for p.eta
+            e = np.broadcast_to(np.asarray(eta_slot, np.float64), (self.B,)) / float(np.float32(brain.p.w_syn))
+            self._eta_slot.copy_(torch.from_numpy(np.ascontiguousarray(e)))
         if not brain.pw.is_contiguous() or brain.pw.dtype != torch.float32:
             raise ValueError("brain.pw must be contiguous float32")
         P = _lib.ptr
@@ -135,7 +142,7 @@ class BatchReward:
                          P(self.d_sign), P(self.d_wmax), P(self.elig), P(self.w), P(self.count), P(brain.pw),
                          P(self.post_mean), P(self.r_mean), P(self._reward), P(self._learn), P(self.d_fly_ptr),
                          P(self.d_fly_slots), c["ae"], c["ap"], c["ar"], c["eta"], np.float32(brain.p.w_syn),
-                         P(self._adv))
+                         P(self._adv), P(self._eta_slot) if eta_slot is not None else None)
         _lib.check(_lib.load().fz_reward_step(ctypes.byref(a), _lib.stream_handle(brain.device)), "fz_reward_step")
         return self._adv.cpu().numpy() if return_adv else None
 
@@ -269,3 +276,10 @@ class BatchMotor:
     def buttons(self, r, k: int = 0) -> dict:
         """One slot's button dict (flyzero/batch.py:122-138)."""
         return dict(zip(BUTTONS, self.masks(np.asarray(r)[None], [k])[0].tolist()))
+
+
+def step_scaled(rp: BatchReward, counts: torch.Tensor, reward, learn, window_ms: float, eta_slot):
+    """``BatchReward.step`` with a per-slot learning rate (flyzero/school.py:289-304,
+    ``_step_scaled``: one rule shared by several flies that learn at different rates).
+    ``eta_slot``: (B,) in mV per unit of (reward x eligibility), like ``LearnParams.eta``."""
+    return rp.step(counts, reward, np.asarray(learn, bool), window_ms, eta_slot=eta_slot)

@@ -175,6 +175,9 @@ typedef struct fz_reward_args {
     double eta;                 /* synapses per unit of (reward x eligibility) */
     float w_syn;                /* mV per synapse */
     double* adv_out;            /* [B] advantage R - <R> before the update */
+    const double* eta_slot;     /* [B] per-slot eta in synapses per unit (the per-slot learning rate of
+                                 * flyzero/school.py:289-304, `_step_scaled`), or NULL = `eta` for
+                                 * every slot */
 } fz_reward_args;
 
 int fz_reward_step(const fz_reward_args* a, void* stream);

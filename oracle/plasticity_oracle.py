@@ -35,10 +35,12 @@ class RewardOracle:
         return self.count().astype(np.float32) * np.float32(w_syn)
 
     def step(self, counts, reward, learn, ae, ap, ar, eta):
-        """``eta`` in synapses per unit; ``counts`` (B, n) int."""
+        """``eta`` in synapses per unit, one value or one per slot (the per-slot rate of
+        flyzero/school.py:289-304); ``counts`` (B, n) int."""
         counts = np.asarray(counts)
         reward = np.broadcast_to(np.asarray(reward, np.float64), (self.B,))
         learn = np.broadcast_to(np.asarray(learn, bool), (self.B,))
+        eta = np.broadcast_to(np.asarray(eta, np.float64), (self.B,))
         adv = reward - self.r_mean
         for b in range(self.B):
             c_pre = counts[b, self.pre].astype(np.float64)
@@ -54,7 +56,7 @@ class RewardOracle:
                     continue
                 acc = np.zeros(self.w.shape[1])
                 for b in act:
-                    acc = acc + (eta * adv[b]) * self.elig[b].astype(np.float64)
+                    acc = acc + (eta[b] * adv[b]) * self.elig[b].astype(np.float64)
                 d = acc / float(len(act))
                 for b in slots:
                     w = self.w[b] + d

@@ -18,3 +18,11 @@ def _name_the_case(request):
     # flushed before every test so a GPU failure names its case
     print(f"\n[case] {request.node.nodeid}", flush=True)
     yield
+
+
+@pytest.fixture(scope="session")
+def full_fly():
+    """The full-size FlyWire-shaped connectome: 138,639 neurons, 15.1 M connections (seed 0)."""
+    from flyhip.connectome import flywire_like
+
+    return flywire_like()

@@ -39,3 +39,16 @@ def fly_inputs(fly):
     idx = np.r_[fly.input_idx, fly.drive_idx].astype(np.int64)
     rate = np.r_[fly.input_rate, np.float32(fly.drive_hz)].astype(np.float32)
     return idx, rate
+
+
+def frame_rates(fly, frames: int, gain=None, drive=None) -> np.ndarray:
+    """[frames, B, n_inputs + 1] Hz: the smooth retinotopic field of each frame times each fly's
+    visual gain, and each fly's drive-neuron rate in the last column."""
+    from flyhip.connectome import smooth_field
+
+    vis = np.stack([smooth_field(fly.input_xy, f) for f in range(frames)])
+    gain = np.ones(1, np.float32) if gain is None else np.asarray(gain, np.float32)
+    drive = np.full(len(gain), fly.drive_hz, np.float32) if drive is None else np.asarray(drive, np.float32)
+    out = np.concatenate([vis[:, None, :] * gain[None, :, None],
+                          np.broadcast_to(drive[None, :, None], (frames, len(gain), 1))], -1)
+    return np.minimum(out, 1e4).astype(np.float32)

@@ -311,11 +311,6 @@ def test_tile_engine_rejects_shapes_over_the_lds_budget(small_fly):
         gb.run(FRAME_MS, graph=False)
 
 
-@pytest.fixture(scope="module")
-def full_fly():
-    return flywire_like()          # 138,639 neurons, 15.1 M connections
-
-
 @pytest.mark.parametrize("engine", ENGINES)
 def test_full_connectome_raster_1000_steps_and_one_second_of_frames(full_fly, engine):
     # the full-size workload: identical rasters over the first 1,000 steps on the same

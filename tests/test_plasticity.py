@@ -78,6 +78,21 @@ def test_slots_of_one_fly_share_the_averaged_change():
     assert np.allclose(o.w[:, 0], 11.3, atol=1e-12)
 
 
+def test_per_slot_learning_rate_by_hand():
+    # flyzero/school.py:289-304: slot b's change is eta_b * adv_b * e_b before the fly's average
+    o = RewardOracle([0], [1], [10.0], B=3, cap=1e9, fly_of_slot=[0, 0, 1])
+    counts = np.array([[1, 2], [3, 4], [2, 5]])
+    o.step(counts, reward=[1.0, 2.0, -1.0], learn=True, ae=0.0, ap=0.5, ar=0.5, eta=[0.1, 0.3, 0.2])
+    # fly 0: e = 2 and 12, (0.1*1*2 + 0.3*2*12) / 2 = 3.7; fly 1: e = 10, 0.2 * -1 * 10 = -2
+    assert np.allclose(o.w[:, 0], [13.7, 13.7, 8.0], atol=1e-12)
+    # one eta for every slot is the scalar rule
+    a = RewardOracle([0], [1], [10.0], B=2, cap=1e9)
+    b = RewardOracle([0], [1], [10.0], B=2, cap=1e9)
+    a.step(counts[:2], reward=[1.0, 2.0], learn=True, ae=0.0, ap=0.5, ar=0.5, eta=0.25)
+    b.step(counts[:2], reward=[1.0, 2.0], learn=True, ae=0.0, ap=0.5, ar=0.5, eta=[0.25, 0.25])
+    assert np.array_equal(a.w, b.w)
+
+
 def test_dn_readout_closed_form():
     counts = np.array([[2, 4, 0, 6]])
     r = dn_rates_step(np.zeros((1, 2), np.float32), counts, [np.array([0, 1]), np.array([3])], window_ms=10.0,
@@ -131,6 +146,37 @@ def test_reward_kernel_equals_fp64_oracle(small_fly):
         np.testing.assert_array_equal(_np(br.post_mean), o.post_mean)
         np.testing.assert_array_equal(_np(br.r_mean), o.r_mean)
     assert np.abs(o.w - gb.w0_syn[None]).max() >= 1.0          # integer counts moved
+
+
+@pytest.mark.gpu
+def test_reward_kernel_per_slot_eta_equals_fp64_oracle(small_fly):
+    # step_scaled (flyzero/school.py:289-304): slots of one fly learn at their own rates
+    import torch
+
+    from flyhip import BatchBrain, LIFParams
+    from flyhip.batch import BatchReward, LearnParams, rule_constants, step_scaled
+    from flyhip.connectome import plastic_positions
+
+    pos = plastic_positions(small_fly)
+    B = 5
+    gb = BatchBrain(small_fly.weights, LIFParams(), batch=B, plastic_pos=pos)
+    p = LearnParams(tau_elig_ms=200.0, eta=2e-2, w_abs_max=50.0)
+    fos = [0, 0, 1, 1, 1]
+    br = BatchReward(gb, small_fly, p, fly_of_slot=fos)
+    o = RewardOracle(gb.pl_pre, gb.pl_post, gb.w0_syn, B, cap=p.cap, w_abs_max=p.w_abs_max, fly_of_slot=fos)
+    eta_mv = np.array([2e-2, 6e-2, 0.0, 1e-2, 4e-2])
+    eta_syn = eta_mv / float(np.float32(gb.p.w_syn))
+    rng = np.random.default_rng(9)
+    c = rule_constants(p, 16.64, gb.p.w_syn)
+    for frame in range(15):
+        counts = rng.integers(0, 4, (B, gb.n)).astype(np.int32)
+        reward = rng.normal(0, 1, B)
+        learn = rng.random(B) < 0.8
+        adv = step_scaled(br, torch.from_numpy(counts).cuda(), reward, learn, 16.64, eta_mv)
+        np.testing.assert_array_equal(adv, o.step(counts, reward, learn, c["ae"], c["ap"], c["ar"], eta_syn))
+        np.testing.assert_array_equal(_np(br.w), o.w, err_msg=f"frame {frame}")
+        np.testing.assert_array_equal(_np(br.count), o.count())
+    assert np.abs(o.w - gb.w0_syn[None]).max() >= 1.0
 
 
 @pytest.mark.gpu
