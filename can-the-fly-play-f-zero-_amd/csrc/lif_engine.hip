@@ -33,6 +33,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -1265,7 +1266,18 @@ __global__ void lif_scatter_rows(int n_list, const int* list, const int* indptr,
     }
 }
 
+__global__ void zero_u64(unsigned long long* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
 }  // namespace
+
+// Device scratch of fz_run_static: the push engine's spike work list, carved per call out of
+// this fixed array instead of memory allocated by fz_bind.  Each GPU the library runs on holds
+// its own copy for the life of the process.  In host code the name is only a stand-in for the
+// symbol: its device address (per device) comes from hipGetSymbolAddress at fz_create.
+static __device__ __align__(256) unsigned char scratch[FZ_STATIC_SCRATCH_BYTES];
 
 struct fz_engine {
     int n = 0;
@@ -1303,6 +1315,9 @@ struct fz_engine {
     const int* tb = nullptr;
     const int* ptg_ptr = nullptr;
     const int* ptg_idx = nullptr;
+    // fz_run_static: the device the engine was created on and that device's scratch address
+    int dev = 0;
+    unsigned char* scr = nullptr;
 };
 
 struct fz_graph {
@@ -1365,7 +1380,7 @@ int check_inputs(const fz_engine* e, const fz_inputs* in) {
 }
 
 UpdArgs make_upd(const fz_engine* e, const fz_inputs* in, const Shape& s, int K, int kb, int blk, int first,
-                 const unsigned* hits, unsigned* raster) {
+                 const unsigned* hits, unsigned* raster, const WorkList& wl) {
     UpdArgs a{};
     a.clock = e->st.clock;
     a.N = e->n;
@@ -1393,14 +1408,14 @@ UpdArgs make_upd(const fz_engine* e, const fz_inputs* in, const Shape& s, int K,
     a.alive = in->alive;
     a.indptr = e->indptr;
     a.pl_indptr = e->n_pl > 0 ? e->pl_indptr : nullptr;
-    a.wl = e->wl;
+    a.wl = wl;
     a.err = e->err;
     a.c = e->c;
     return a;
 }
 
 template <typename WT>
-PushArgs<WT> make_push(const fz_engine* e, const fz_inputs* in, const Shape& s, int blk) {
+PushArgs<WT> make_push(const fz_engine* e, const fz_inputs* in, const Shape& s, int blk, const WorkList& wl) {
     PushArgs<WT> a{};
     a.clock = e->st.clock;
     a.N = e->n;
@@ -1416,7 +1431,7 @@ PushArgs<WT> make_push(const fz_engine* e, const fz_inputs* in, const Shape& s, 
     a.n_pl = e->n_pl;
     a.pw_qscale = e->c.pw_qscale;
     a.ring = e->st.ring;
-    a.wl = e->wl;
+    a.wl = wl;
     return a;
 }
 
@@ -1708,7 +1723,8 @@ BucketArgs make_bucket(const fz_engine* e, const fz_inputs* in, int K, int kb, i
 // One window as a sequence of kernel launches (the graph builder mirrors it).
 // With `ev_*` non-null, events bracket every kernel.
 int enqueue(fz_engine* e, const fz_inputs* in, int steps, const unsigned* hits, unsigned* raster,
-            const Shape& s, hipStream_t stream, std::vector<hipEvent_t>* ev_upd, std::vector<hipEvent_t>* ev_push) {
+            const Shape& s, hipStream_t stream, std::vector<hipEvent_t>* ev_upd, std::vector<hipEvent_t>* ev_push,
+            const WorkList& wl) {
     const dim3 ugrid((e->n + s.ut - 1) / s.ut, e->st.B);
     const dim3 pgrid = push_grid(e, s);
     int kb = 0, blk = 0;
@@ -1752,16 +1768,16 @@ int enqueue(fz_engine* e, const fz_inputs* in, int steps, const unsigned* hits, 
     }
     while (kb < steps) {
         const int K = (steps - kb) < s.K ? (steps - kb) : s.K;
-        UpdArgs ua = make_upd(e, in, s, K, kb, blk, kb == 0, hits, raster);
+        UpdArgs ua = make_upd(e, in, s, K, kb, blk, kb == 0, hits, raster, wl);
         if ((rc = mark(ev_upd))) return rc;
         hipLaunchKernelGGL(lif_update, ugrid, dim3(s.ut), 0, stream, ua);
         HIPCHK(hipGetLastError());
         if ((rc = mark(ev_upd))) return rc;
         if ((rc = mark(ev_push))) return rc;
         if (e->w_bits == 16) {
-            hipLaunchKernelGGL(lif_push<int16_t>, pgrid, dim3(s.pt), 0, stream, make_push<int16_t>(e, in, s, blk));
+            hipLaunchKernelGGL(lif_push<int16_t>, pgrid, dim3(s.pt), 0, stream, make_push<int16_t>(e, in, s, blk, wl));
         } else {
-            hipLaunchKernelGGL(lif_push<int32_t>, pgrid, dim3(s.pt), 0, stream, make_push<int32_t>(e, in, s, blk));
+            hipLaunchKernelGGL(lif_push<int32_t>, pgrid, dim3(s.pt), 0, stream, make_push<int32_t>(e, in, s, blk, wl));
         }
         HIPCHK(hipGetLastError());
         if ((rc = mark(ev_push))) return rc;
@@ -1783,6 +1799,60 @@ void free_worklist(fz_engine* e) {
     (void)hipFree(e->wl.cs);
     e->wl = WorkList{};
     e->wl_B = 0;
+}
+
+// Work-list capacity of the push engine: entries per fly (a neuron fires at most
+// ceil(K / n_ref) times in a block of K <= kmax steps) and chunk-start slots per fly.
+void worklist_dims(const fz_engine* e, long long* cap, long long* ncs) {
+    const int per = e->c.n_ref > 1 ? e->c.n_ref : 1;
+    const long long maxspk = (e->kmax + per - 1) / per;
+    *cap = (long long)e->n * maxspk;
+    *ncs = (e->nnz + e->n_pl) * maxspk / QMIN + 2;
+}
+
+// The work list of B flies laid out from `base` (256-byte aligned arrays); returns the bytes it
+// spans.  base = nullptr only measures.
+size_t static_worklist(const fz_engine* e, int B, unsigned char* base, WorkList* wl) {
+    long long cap, ncs;
+    worklist_dims(e, &cap, &ncs);
+    size_t off = 0;
+    auto take = [&](size_t bytes) -> void* {
+        void* q = base ? static_cast<void*>(base + off) : nullptr;
+        off += (bytes + 255) & ~(size_t)255;
+        return q;
+    };
+    const size_t ne = (size_t)B * (size_t)cap;
+    wl->ctr = static_cast<unsigned long long*>(take(sizeof(unsigned long long) * 2 * (size_t)B));
+    wl->off = static_cast<unsigned*>(take(sizeof(unsigned) * ne));
+    wl->fbeg = static_cast<int*>(take(sizeof(int) * ne));
+    wl->fdeg = static_cast<int*>(take(sizeof(int) * ne));
+    wl->pbeg = static_cast<int*>(take(sizeof(int) * ne));
+    wl->slot = static_cast<int*>(take(sizeof(int) * ne));
+    wl->cs = static_cast<unsigned*>(take(sizeof(unsigned) * (size_t)B * (size_t)ncs));
+    wl->cap = cap;
+    wl->ncs = ncs;
+    return off;
+}
+
+// Device address of `scratch` on device `dev` (current), looked up once per device.
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scr_mu;
+unsigned char* g_scr[MAX_DEVICES] = {};
+
+int scratch_address(int dev, unsigned char** out) {
+    if (dev < 0 || dev >= MAX_DEVICES) return fail(FZ_E_ARG, "device %d outside [0, %d)", dev, MAX_DEVICES);
+    std::lock_guard<std::mutex> lock(g_scr_mu);
+    if (!g_scr[dev]) {
+        void* q = nullptr;
+        const hipError_t r = hipGetSymbolAddress(&q, HIP_SYMBOL(scratch));
+        if (r != hipSuccess || !q) {
+            (void)hipGetLastError();
+            return fail(FZ_E_HIP, "hipGetSymbolAddress(scratch) on device %d failed: %s", dev, hipGetErrorString(r));
+        }
+        g_scr[dev] = static_cast<unsigned char*>(q);
+    }
+    *out = g_scr[dev];
+    return FZ_OK;
 }
 
 }  // namespace
@@ -1817,14 +1887,25 @@ int fz_create(int n, long long nnz, const int* indptr, const int* col, const voi
                   p->n_ref, p->n_delay, frac_bits};
     e->kmax = p->n_delay < KMAX ? p->n_delay : KMAX;
     int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        (void)hipGetLastError();
+        delete e;
+        return fail(FZ_E_HIP, "hipGetDevice failed");
+    }
+    e->dev = dev;
     hipDeviceProp_t prop;
-    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess) {
+    if (hipGetDeviceProperties(&prop, dev) == hipSuccess) {
         e->n_cu = prop.multiProcessorCount;
         e->lds_max = prop.sharedMemPerBlock;
     }
     if (hipMalloc(&e->err, sizeof(int)) != hipSuccess || hipMemset(e->err, 0, sizeof(int)) != hipSuccess) {
         fz_destroy(e);
         return fail(FZ_E_HIP, "allocating the engine's error flag failed");
+    }
+    int rc = scratch_address(dev, &e->scr);  // set-up of fz_run_static, with the device current
+    if (rc) {
+        fz_destroy(e);
+        return rc;
     }
     *out = e;
     return FZ_OK;
@@ -1872,11 +1953,11 @@ int fz_bind(fz_engine* e, const fz_state* st) {
         return fail(FZ_E_ARG, "the tile engine needs the spike lists spk_idx and spk_cnt");
     const int per = e->c.n_ref > 1 ? e->c.n_ref : 1;
     const long long maxspk = (e->kmax + per - 1) / per;  // spikes per neuron per block
-    const long long cap = (long long)e->n * maxspk;       // entries per fly
     const long long edges = (e->nnz + e->n_pl) * maxspk;  // edges per fly and block, at most
+    long long cap, ncs;                                   // entries and chunk-start slots per fly
+    worklist_dims(e, &cap, &ncs);
     if (cap >= (1ll << 31)) return fail(FZ_E_ARG, "n too large for the per-fly work list");
     if (edges >= (1ll << 31)) return fail(FZ_E_ARG, "nnz too large for the per-fly work list");
-    const long long ncs = edges / QMIN + 2;
     if (st->B != e->wl_B || cap != e->wl.cap || ncs != e->wl.ncs) {
         free_worklist(e);
         const size_t ne = (size_t)st->B * (size_t)cap;
@@ -2001,7 +2082,40 @@ int fz_run(fz_engine* e, const fz_inputs* in, int steps, const unsigned* hits, u
     if (steps == 0) return FZ_OK;
     Shape s;
     if ((rc = resolve(e, tune, &s))) return rc;
-    return enqueue(e, in, steps, hits, raster, s, static_cast<hipStream_t>(stream), nullptr, nullptr);
+    return enqueue(e, in, steps, hits, raster, s, static_cast<hipStream_t>(stream), nullptr, nullptr, e->wl);
+}
+
+int fz_run_static(fz_engine* e, const fz_inputs* in, int steps, const unsigned* hits, unsigned* raster,
+                  const fz_tune* tune, void* stream) {
+    int rc = check_inputs(e, in);
+    if (rc) return rc;
+    if (steps < 0) return fail(FZ_E_ARG, "steps %d < 0", steps);
+    if (steps == 0) return FZ_OK;
+    Shape s;
+    if ((rc = resolve(e, tune, &s))) return rc;
+    hipStream_t hs = static_cast<hipStream_t>(stream);
+    int dev = -1;
+    HIPCHK(hipGetDevice(&dev));
+    if (hs) HIPCHK(hipStreamGetDevice(hs, &dev));
+    if (dev != e->dev)
+        return fail(FZ_E_STATE, "fz_run_static: the stream's device %d is not the engine's device %d", dev, e->dev);
+    if (e->mode == 2) {  // the segment-table source reads only caller state: no engine scratch
+        if (s.src != 0)
+            return fail(FZ_E_ARG, "fz_run_static: the tile engine's bucket source (tile_source 1) needs the bucket "
+                        "arena; use tile_source 0 or fz_run");
+        return enqueue(e, in, steps, hits, raster, s, hs, nullptr, nullptr, e->wl);
+    }
+    WorkList wl{};
+    const size_t need = static_worklist(e, e->st.B, nullptr, &wl);
+    if (need > (size_t)FZ_STATIC_SCRATCH_BYTES)
+        return fail(FZ_E_ARG, "fz_run_static: %d flies of %d neurons need %zu bytes of work list; the static scratch "
+                    "holds %lld (use fz_run)", e->st.B, e->n, need, (long long)FZ_STATIC_SCRATCH_BYTES);
+    static_worklist(e, e->st.B, e->scr, &wl);
+    // the work-list counters start at zero (a fill kernel, so that a captured call has no memset node);
+    // the other arrays are written before they are read in every block
+    hipLaunchKernelGGL(zero_u64, dim3((2 * e->st.B + 255) / 256), dim3(256), 0, hs, wl.ctr, 2 * e->st.B);
+    HIPCHK(hipGetLastError());
+    return enqueue(e, in, steps, hits, raster, s, hs, nullptr, nullptr, wl);
 }
 
 int fz_run_timed(fz_engine* e, const fz_inputs* in, int steps, const fz_tune* tune, void* stream, float* upd_ms,
@@ -2013,7 +2127,7 @@ int fz_run_timed(fz_engine* e, const fz_inputs* in, int steps, const fz_tune* tu
     if ((rc = resolve(e, tune, &s))) return rc;
     hipStream_t hs = static_cast<hipStream_t>(stream);
     std::vector<hipEvent_t> eu, ep;
-    rc = enqueue(e, in, steps, nullptr, nullptr, s, hs, &eu, &ep);
+    rc = enqueue(e, in, steps, nullptr, nullptr, s, hs, &eu, &ep, e->wl);
     float tu = 0.f, tp = 0.f;
     if (rc == FZ_OK) {
         if (hipStreamSynchronize(hs) != hipSuccess) rc = fail(FZ_E_HIP, "stream synchronize failed");
@@ -2153,16 +2267,16 @@ int fz_graph_create(fz_engine* e, const fz_inputs* in, int steps, const fz_tune*
     }
     while (kb < steps) {
         const int K = (steps - kb) < s.K ? (steps - kb) : s.K;
-        UpdArgs ua = make_upd(e, in, s, K, kb, blk, kb == 0, nullptr, nullptr);
+        UpdArgs ua = make_upd(e, in, s, K, kb, blk, kb == 0, nullptr, nullptr, e->wl);
         void* uargs[] = {&ua};
         if ((err = add(reinterpret_cast<void*>(&lif_update), ugrid, dim3(s.ut), uargs)) != hipSuccess)
             return bail("hipGraphAddKernelNode(update)", err);
         if (e->w_bits == 16) {
-            PushArgs<int16_t> pa = make_push<int16_t>(e, in, s, blk);
+            PushArgs<int16_t> pa = make_push<int16_t>(e, in, s, blk, e->wl);
             void* pargs[] = {&pa};
             err = add(reinterpret_cast<void*>(&lif_push<int16_t>), pgrid, dim3(s.pt), pargs);
         } else {
-            PushArgs<int32_t> pa = make_push<int32_t>(e, in, s, blk);
+            PushArgs<int32_t> pa = make_push<int32_t>(e, in, s, blk, e->wl);
             void* pargs[] = {&pa};
             err = add(reinterpret_cast<void*>(&lif_push<int32_t>), pgrid, dim3(s.pt), pargs);
         }

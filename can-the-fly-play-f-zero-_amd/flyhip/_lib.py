@@ -57,6 +57,8 @@ SIGNATURES = {
     "fz_set_tiles": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
     "fz_run": (c_int, [c_void_p, ctypes.POINTER(FzInputs), c_int, c_void_p, c_void_p,
                        ctypes.POINTER(FzTune), c_void_p]),
+    "fz_run_static": (c_int, [c_void_p, ctypes.POINTER(FzInputs), c_int, c_void_p, c_void_p,
+                              ctypes.POINTER(FzTune), c_void_p]),
     "fz_run_timed": (c_int, [c_void_p, ctypes.POINTER(FzInputs), c_int, ctypes.POINTER(FzTune), c_void_p,
                              ctypes.POINTER(ctypes.c_float), ctypes.POINTER(c_int),
                              ctypes.POINTER(ctypes.c_float), ctypes.POINTER(c_int)]),

@@ -123,6 +123,23 @@ int fz_set_tiles(fz_engine* e, int tile, int n_tiles, const int* tb, const int* 
 int fz_run(fz_engine* e, const fz_inputs* in, int steps, const unsigned* hits, unsigned* raster,
            const fz_tune* tune, void* stream);
 
+/* fz_run without device memory of the engine's own, for callers that cannot allocate device
+ * memory or take memory the library allocated for them (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid allocation).  It has no counterpart in the reference.
+ * Push engine: the per-fly spike work list that fz_bind otherwise allocates is carved out of a
+ * fixed device array of FZ_STATIC_SCRATCH_BYTES that the library holds on each GPU it runs on
+ * until the process ends (about 3.7 MB per fly of the 138,639-neuron, 15.1 M-connection fly, so
+ * up to 8 such flies); a batch that needs more is refused with FZ_E_ARG.  Tile engine: only the
+ * segment-table source (tile_source 0), which uses caller state only; the bucket source is
+ * refused.  The call allocates nothing and calls no allocating routine; the caller still passes
+ * (and binds) the state, inputs and outputs.  Every call on one GPU shares that GPU's array and
+ * leaves the last call's contents in it (its counters are cleared at the start of each call):
+ * calls that overlap on one GPU race, so serialise them (one stream, or events).  The stream
+ * must belong to the device the engine was created on. */
+#define FZ_STATIC_SCRATCH_BYTES (32ll << 20)
+int fz_run_static(fz_engine* e, const fz_inputs* in, int steps, const unsigned* hits, unsigned* raster,
+                  const fz_tune* tune, void* stream);
+
 /* As fz_run, timing every kernel with HIP events on `stream`; waits for the
  * stream.  Out: summed milliseconds and launch counts of the update and push
  * kernels. */
