@@ -47,10 +47,17 @@ typedef struct {
     int n;
 } spk_list;
 
+/* Shadow statistics of one thread (merged after each sweep). */
+typedef struct {
+    double dv, dg, dv_sub, vabs;  /* FP64 trajectory vs FP32 */
+    long long flips;
+    double sdv, sdg;              /* one FP64 step from the FP32 state vs the FP32 step */
+    long long sflips;
+} shadow_acc;
+
 /* One fly, one step, neurons [i0, i1): phase 1 of brain.py:39-50. */
 static void sweep(const orc_net* net, orc_state* st, const orc_input* in, int b, int s, int slot,
-                  int i0, int i1, int* counts, unsigned* raster, spk_list* out, orc_shadow_report* rep,
-                  double* dv_max, double* dg_max, long long* flips, double* dv_sub, double* vabs) {
+                  int i0, int i1, int* counts, unsigned* raster, spk_list* out, int one_step, shadow_acc* sa) {
     const int n = net->n;
     const size_t bn = (size_t)b * n;
     int* ring = st->ring + ((size_t)b * net->n_delay + slot) * n;
@@ -62,10 +69,10 @@ static void sweep(const orc_net* net, orc_state* st, const orc_input* in, int b,
     const int hw = (net->n_ext + 31) / 32;
     const int rw = (n + 31) / 32;
     const unsigned step = st->step_no + (unsigned)s;
-    (void)rep;
     for (int i = i0; i < i1; i++) {
         int acc = ring[i];
         ring[i] = 0;
+        const float g_prev = g[i], u_prev = v[i];
         float inj = (float)acc * net->in_scale;
         float gi = g[i] * net->decay_syn + inj;
         g[i] = gi;
@@ -93,6 +100,19 @@ static void sweep(const orc_net* net, orc_state* st, const orc_input* in, int b,
         }
         if (hit) vi += net->kick;
         int spike = (vi >= net->u_th) && (r <= 0);
+        if (one_step) {
+            /* The same step in FP64 from this step's FP32 state (g, u before it; same integer
+             * arrivals, kick and refractory state; the model's float32 constants), against the
+             * FP32 result before the spike test. */
+            double gd = (double)g_prev * (double)net->decay_syn + (double)acc * (double)net->in_scale;
+            double vd = (double)u_prev;
+            if (ref[i] <= 0) vd += (gd + (double)bias - vd) * (double)net->decay_m;
+            if (hit) vd += (double)net->kick;
+            const double dv = fabs(vd - (double)vi), dg = fabs(gd - (double)gi);
+            if (dv > sa->sdv) sa->sdv = dv;
+            if (dg > sa->sdg) sa->sdg = dg;
+            if (((vd >= (double)net->u_th) && (r <= 0)) != spike) sa->sflips++;
+        }
         if (v64) {
             /* FP64 shadow: same integer arrivals, same kicks, spike decisions
              * forced to the FP32 ones so the two trajectories stay comparable. */
@@ -102,12 +122,12 @@ static void sweep(const orc_net* net, orc_state* st, const orc_input* in, int b,
             if (ref[i] <= 0) vd += (gd + (double)bias - vd) * (double)net->decay_m;
             if (hit) vd += (double)net->kick;
             int spike64 = (vd >= (double)net->u_th) && (r <= 0);
-            if (spike64 != spike) (*flips)++;
+            if (spike64 != spike) sa->flips++;
             double dv = fabs(vd - (double)vi), dg = fabs(gd - (double)gi);
-            if (dv > *dv_max) *dv_max = dv;
-            if (vd < (double)net->u_th && vd >= -12.0 && dv > *dv_sub) *dv_sub = dv;
-            if (fabs(vd) > *vabs) *vabs = fabs(vd);
-            if (dg > *dg_max) *dg_max = dg;
+            if (dv > sa->dv) sa->dv = dv;
+            if (vd < (double)net->u_th && vd >= -12.0 && dv > sa->dv_sub) sa->dv_sub = dv;
+            if (fabs(vd) > sa->vabs) sa->vabs = fabs(vd);
+            if (dg > sa->dg) sa->dg = dg;
             v64[i] = spike ? (double)net->u_reset : vd;
         }
         if (spike) {
@@ -151,17 +171,18 @@ long long orc_lif_run(const orc_net* net, orc_state* st, const orc_input* in, in
 #endif
     spk_list* lists = (spk_list*)calloc((size_t)nthreads, sizeof(spk_list));
     for (int t = 0; t < nthreads; t++) lists[t].idx = (int*)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
-    double dv_run = 0.0, dg_run = 0.0, dvs_run = 0.0, va_run = 0.0;
-    long long flips = 0, total = 0;
+    const int one_step = rep && rep->one_step;
+    shadow_acc run = {0};
+    long long total = 0;
     for (int s = 0; s < steps; s++) {
         const int slot = (int)((st->t + s) % net->n_delay);
-        double dv_step = 0.0;
+        double dv_step = 0.0, sdv_step = 0.0;
         for (int b = 0; b < st->B; b++) {
             if (in->alive && !in->alive[b]) continue;
             /* neuron chunks are multiples of 32 so raster words are not shared */
             int chunk = ((n + nthreads - 1) / nthreads + 31) / 32 * 32;
-            double dvt[64] = {0}, dgt[64] = {0}, dst[64] = {0}, vat[64] = {0};
-            long long flt[64] = {0};
+            shadow_acc part[64];
+            memset(part, 0, sizeof(part));
 #ifdef _OPENMP
 #pragma omp parallel for num_threads(nthreads) schedule(static, 1) if (nthreads > 1)
 #endif
@@ -169,36 +190,38 @@ long long orc_lif_run(const orc_net* net, orc_state* st, const orc_input* in, in
                 int i0 = t * chunk, i1 = i0 + chunk;
                 if (i1 > n) i1 = n;
                 lists[t].n = 0;
-                if (i0 < i1)
-                    sweep(net, st, in, b, s, slot, i0, i1, counts, raster, &lists[t], rep, &dvt[t & 63],
-                          &dgt[t & 63], &flt[t & 63], &dst[t & 63], &vat[t & 63]);
+                if (i0 < i1) sweep(net, st, in, b, s, slot, i0, i1, counts, raster, &lists[t], one_step, &part[t]);
             }
             for (int t = 0; t < nthreads; t++) {
-                if (dvt[t & 63] > dv_step) dv_step = dvt[t & 63];
-                if (dgt[t & 63] > dg_run) dg_run = dgt[t & 63];
-                if (dst[t & 63] > dvs_run) dvs_run = dst[t & 63];
-                if (vat[t & 63] > va_run) va_run = vat[t & 63];
-                dst[t & 63] = 0;
-                vat[t & 63] = 0;
-                flips += flt[t & 63];
-                dvt[t & 63] = 0;
-                dgt[t & 63] = 0;
-                flt[t & 63] = 0;
+                const shadow_acc* a = &part[t];
+                if (a->dv > dv_step) dv_step = a->dv;
+                if (a->sdv > sdv_step) sdv_step = a->sdv;
+                if (a->dg > run.dg) run.dg = a->dg;
+                if (a->dv_sub > run.dv_sub) run.dv_sub = a->dv_sub;
+                if (a->vabs > run.vabs) run.vabs = a->vabs;
+                if (a->sdg > run.sdg) run.sdg = a->sdg;
+                run.flips += a->flips;
+                run.sflips += a->sflips;
                 push(net, st, in, b, slot, &lists[t]);
                 total += lists[t].n;
             }
         }
-        if (dv_step > dv_run) dv_run = dv_step;
+        if (dv_step > run.dv) run.dv = dv_step;
+        if (sdv_step > run.sdv) run.sdv = sdv_step;
         if (rep && rep->dv_step) rep->dv_step[s] = dv_step;
+        if (rep && rep->step_dv_step) rep->step_dv_step[s] = sdv_step;
     }
     st->t += steps;
     st->step_no += (unsigned)steps;
     if (rep) {
-        rep->max_dv = dv_run;
-        rep->max_dg = dg_run;
-        rep->max_dv_sub = dvs_run;
-        rep->max_abs_v = va_run;
-        rep->flips = flips;
+        rep->max_dv = run.dv;
+        rep->max_dg = run.dg;
+        rep->max_dv_sub = run.dv_sub;
+        rep->max_abs_v = run.vabs;
+        rep->flips = run.flips;
+        rep->step_max_dv = run.sdv;
+        rep->step_max_dg = run.sdg;
+        rep->step_flips = run.sflips;
     }
     for (int t = 0; t < nthreads; t++) free(lists[t].idx);
     free(lists);

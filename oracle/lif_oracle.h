@@ -79,6 +79,12 @@ typedef struct {
     double max_dg;          /* max |g32 - g64| over the run */
     long long flips;        /* neuron-steps where the FP64 spike test disagrees */
     double* dv_step;        /* per-step max |v32 - v64| ([steps]) or NULL */
+    /* per-step reading: every step advanced once more in FP64 from that step's FP32 state */
+    int one_step;           /* in: 1 = measure it */
+    double step_max_dv;     /* max |u32 - u64| after one step, before the spike test (mV) */
+    double step_max_dg;     /* max |g32 - g64| after one step (mV) */
+    long long step_flips;   /* neuron-steps whose one-step FP64 spike test disagrees */
+    double* step_dv_step;   /* per-step max of the one-step |u32 - u64| ([steps]) or NULL */
 } orc_shadow_report;
 
 unsigned orc_mix(unsigned x);

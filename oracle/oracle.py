@@ -107,7 +107,9 @@ class _Input(ctypes.Structure):
 class _Report(ctypes.Structure):
     _fields_ = [("max_dv", ctypes.c_double), ("max_dv_sub", ctypes.c_double), ("max_abs_v", ctypes.c_double),
                 ("max_dg", ctypes.c_double), ("flips", ctypes.c_longlong),
-                ("dv_step", ctypes.c_void_p)]
+                ("dv_step", ctypes.c_void_p), ("one_step", ctypes.c_int), ("step_max_dv", ctypes.c_double),
+                ("step_max_dg", ctypes.c_double), ("step_flips", ctypes.c_longlong),
+                ("step_dv_step", ctypes.c_void_p)]
 
 
 _lib = None
@@ -248,8 +250,10 @@ class OracleBrain:
         self.bias_ext[:] = 0
         self.bias_ext[:, j] = np.asarray(mv, np.float32)
 
-    def run_steps(self, steps: int, hits=None, raster=False, nthreads=1, dv_per_step=False):
-        """Advance ``steps`` steps; returns counts [B, n] (and raster words, shadow report)."""
+    def run_steps(self, steps: int, hits=None, raster=False, nthreads=1, dv_per_step=False, one_step=False):
+        """Advance ``steps`` steps; returns counts [B, n] (and raster words).  ``one_step``: also
+        advance every step once in FP64 from the FP32 state and record the difference
+        (``last_report['step_max_dv']`` etc.)."""
         net, c = self.net, self.c
         counts = np.zeros((self.B, self.n), np.int32)
         rw = (self.n + 31) // 32
@@ -270,7 +274,8 @@ class OracleBrain:
         alive = None if self.alive is None else np.ascontiguousarray(self.alive, np.uint8)
         inp = _Input(_ptr(p_ext), _ptr(bias), _ptr(pw), _ptr(hits), _ptr(alive), self.seed, nthreads)
         dv = np.zeros(steps, np.float64) if dv_per_step else None
-        rep = _Report(0.0, 0.0, 0.0, 0.0, 0, _ptr(dv))
+        sdv = np.zeros(steps, np.float64) if (dv_per_step and one_step) else None
+        rep = _Report(0.0, 0.0, 0.0, 0.0, 0, _ptr(dv), int(bool(one_step)), 0.0, 0.0, 0, _ptr(sdv))
         n_sp = lib().orc_lif_run(ctypes.byref(nn), ctypes.byref(st), ctypes.byref(inp), steps,
                                  counts.ctypes.data, _ptr(ras), ctypes.byref(rep))
         if n_sp < 0:
@@ -278,7 +283,9 @@ class OracleBrain:
         self.t += steps
         self.step_no = (self.step_no + steps) & 0xFFFFFFFF
         self.last_report = {"max_dv": rep.max_dv, "max_dv_sub": rep.max_dv_sub, "max_abs_v": rep.max_abs_v,
-                            "max_dg": rep.max_dg, "flips": rep.flips, "dv_step": dv}
+                            "max_dg": rep.max_dg, "flips": rep.flips, "dv_step": dv,
+                            "step_max_dv": rep.step_max_dv, "step_max_dg": rep.step_max_dg,
+                            "step_flips": rep.step_flips, "step_dv_step": sdv}
         if raster:
             return counts, ras
         return counts
