@@ -84,49 +84,60 @@ def field_to_screen(u, v, side, w: int, h: int, overlap: float):
     return px, py
 
 
-def _field(pos: np.ndarray, mid: float) -> np.ndarray:
-    """Visual-field coordinates of one eye's cells (flyzero/eyes.py:72-88): the two principal axes
-    of the point cloud; v is the axis best aligned with brain y (oriented so v grows with y,
-    which grows ventrally), u the other (oriented so u grows laterally); each scaled to 0..1."""
-    c = pos - pos.mean(0)
-    _, _, vt = np.linalg.svd(c, full_matrices=False)
-    uv = c @ vt[:2].T
-    cy = [abs(np.corrcoef(uv[:, k], pos[:, 1])[0, 1]) for k in range(2)]
-    v_k = int(np.argmax(cy))
-    u, v = uv[:, 1 - v_k], uv[:, v_k]
-    if np.corrcoef(v, pos[:, 1])[0, 1] < 0:
+def _corr(a: np.ndarray, b: np.ndarray) -> float:
+    """Pearson correlation; 0 when either side is constant."""
+    da, db = a - a.mean(), b - b.mean()
+    den = float(np.sqrt((da * da).sum() * (db * db).sum()))
+    return float((da * db).sum()) / den if den > 0 else 0.0
+
+
+def _unit(a: np.ndarray) -> np.ndarray:
+    lo = a.min()
+    return (a - lo) / max(float(a.max() - lo), 1e-9)
+
+
+def eye_plane(xyz: np.ndarray, mid_x: float) -> np.ndarray:
+    """Visual-field coordinates (u, v) in [0, 1]^2 of one eye's cells from their 3-D positions,
+    the PCA retinotopy of flyzero/eyes.py:72-88: the eye's cells lie on a curved sheet whose two
+    widest directions (the top two eigenvectors of the position covariance) span the visual
+    field.  The direction whose coordinate follows brain y most closely is vertical (v grows with
+    y, which grows ventrally, so v = 0 is dorsal); the other is horizontal, oriented so that u
+    grows with the distance from the midline x = ``mid_x`` (u = 0 medial, 1 lateral)."""
+    centred = xyz - xyz.mean(axis=0)
+    _, vecs = np.linalg.eigh(centred.T @ centred)          # eigenvalues ascending
+    coords = centred @ vecs[:, [-1, -2]]                   # along the two widest directions
+    vert = 1 if abs(_corr(coords[:, 1], xyz[:, 1])) > abs(_corr(coords[:, 0], xyz[:, 1])) else 0
+    v, u = coords[:, vert], coords[:, 1 - vert]
+    if _corr(v, xyz[:, 1]) < 0:
         v = -v
-    if np.corrcoef(u, np.abs(pos[:, 0] - mid))[0, 1] < 0:
+    if _corr(u, np.abs(xyz[:, 0] - mid_x)) < 0:
         u = -u
-
-    def unit(a):
-        return (a - a.min()) / max(np.ptp(a), 1e-9)
-
-    return np.c_[unit(u), unit(v)]
+    return np.stack([_unit(u), _unit(v)], axis=1)
 
 
 def retinotopy(conn, cells, w: int, h: int, overlap: float = 0.1):
     """Screen pixel (px, py), eye and a placed-mask for each neuron in ``cells`` of a connectome
-    with positions (flyzero/eyes.py:37-69): per eye, the neurons' positions are mapped to the
-    visual field by ``_field``; a missing side label falls back to the side of the median x
-    (x above the median: left).  Eyes with fewer than 3 placed cells are not placed."""
+    with positions (the contract of flyzero/eyes.py:37-69).  Each eye's cells with finite
+    positions get visual-field coordinates from ``eye_plane``; a cell without a side label
+    belongs to the eye on its side of the median x of all neurons (above it: left).  An eye with
+    fewer than 3 placed cells is not placed (mask False, pixel of field (0, 0))."""
     cells = np.asarray(cells, np.int64)
-    nrn = conn.neurons.iloc[cells]
-    pos = nrn[["pos_x", "pos_y", "pos_z"]].to_numpy(float)
-    side = nrn["side"].fillna("").to_numpy().astype(object)
-    ok = np.isfinite(pos).all(1)
-    mid = np.nanmedian(conn.neurons["pos_x"].to_numpy(float))
-    unknown = ~np.isin(side, ["left", "right"])
-    side = np.where(unknown, np.where(pos[:, 0] > mid, "left", "right"), side).astype(str)
-    uv = np.full((len(cells), 2), np.nan)
-    for s in ("left", "right"):
-        m = (side == s) & ok
-        if m.sum() >= 3:
-            uv[m] = _field(pos[m], mid)
-    keep = np.isfinite(uv).all(1)
-    uv = np.nan_to_num(uv)
-    px, py = field_to_screen(uv[:, 0], uv[:, 1], side, w, h, overlap)
-    return px, py, side, keep
+    table = conn.neurons
+    xyz = table[["pos_x", "pos_y", "pos_z"]].to_numpy(float)[cells]
+    label = table["side"].to_numpy(dtype=object, na_value="")[cells]
+    mid_x = float(np.nanmedian(table["pos_x"].to_numpy(float)))
+    finite = np.isfinite(xyz).all(axis=1)
+    by_midline = np.where(xyz[:, 0] > mid_x, "left", "right")
+    eye = np.where((label == "left") | (label == "right"), label, by_midline).astype(str)
+    field_uv = np.zeros((len(cells), 2))
+    placed = np.zeros(len(cells), bool)
+    for side in ("left", "right"):
+        members = np.flatnonzero((eye == side) & finite)
+        if members.size >= 3:
+            field_uv[members] = eye_plane(xyz[members], mid_x)
+            placed[members] = True
+    px, py = field_to_screen(field_uv[:, 0], field_uv[:, 1], eye, w, h, overlap)
+    return px, py, eye, placed
 
 
 def motion_cells(conn, frame_shape, params: "MotionParams | None" = None):

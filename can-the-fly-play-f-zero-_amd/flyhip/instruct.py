@@ -45,52 +45,70 @@ class InstructParams:
     cap: float = 3.0
 
 
+# the teacher's intent axes (flyzero/instruct.py:51-68): (button that pushes the axis up, button
+# that pushes it down, or None)
+INTENT_AXES = (("RIGHT", "LEFT"),   # steer: -1 left .. +1 right
+               ("R", "L"),          # lean
+               ("B", None),         # gas
+               ("Y", None),         # brake
+               ("A", None))         # boost
+
+
 def intent(masks: np.ndarray, buttons: list, tau_frames: float = 15.0) -> np.ndarray:
-    """The teacher's smoothed intention per frame (flyzero/instruct.py:51-68): (n, 5) columns
-    steer (-1 left .. +1 right), lean, gas, brake, boost; a causal exponential filter."""
-    ix = {b: i for i, b in enumerate(buttons)}
-    m = np.asarray(masks, np.float32)
-    raw = np.stack([m[:, ix["RIGHT"]] - m[:, ix["LEFT"]], m[:, ix["R"]] - m[:, ix["L"]],
-                    m[:, ix["B"]], m[:, ix["Y"]], m[:, ix["A"]]], 1)
-    out = np.empty_like(raw)
-    a = 1.0 / tau_frames
-    acc = raw[0].copy()
-    for t in range(len(raw)):
-        acc += a * (raw[t] - acc)
-        out[t] = acc
-    return out
+    """The teacher's smoothed intention per frame, (frames, 5): steer, lean, gas, brake, boost
+    (``INTENT_AXES``), each a causal exponential moving average in float32 with time constant
+    ``tau_frames``, started at the first frame's value (flyzero/instruct.py:51-68)."""
+    col = {name: k for k, name in enumerate(buttons)}
+    pressed = np.asarray(masks, np.float32)
+    axes = np.zeros((pressed.shape[0], len(INTENT_AXES)), np.float32)
+    for k, (up, down) in enumerate(INTENT_AXES):
+        axes[:, k] = pressed[:, col[up]]
+        if down is not None:
+            axes[:, k] -= pressed[:, col[down]]
+    gain = np.float32(1.0 / tau_frames)
+    smoothed = np.empty_like(axes)
+    state = axes[0].copy() if len(axes) else np.zeros(len(INTENT_AXES), np.float32)
+    for t, frame in enumerate(axes):
+        state += gain * (frame - state)
+        smoothed[t] = state
+    return smoothed
+
+
+def _opponent(x: float, p: InstructParams) -> tuple:
+    """(left-side, right-side) target rates for a signed command x in [-1, 1]: the side the
+    command points to rises linearly from rest to the high rate, the other falls from rest to
+    the low rate (x = 0: both at rest)."""
+    rise, fall = p.high_hz - p.rest_hz, p.rest_hz - p.low_hz
+    neg, pos = max(-x, 0.0), max(x, 0.0)
+    return p.rest_hz + rise * neg - fall * pos, p.rest_hz + rise * pos - fall * neg
+
+
+def _level(x: float, p: InstructParams) -> float:
+    """Target rate for an unsigned command x in [0, 1]: low .. high."""
+    return p.low_hz + (p.high_hz - p.low_hz) * x
 
 
 def targets_from_intent(x, p: InstructParams) -> dict:
-    """Target rates of the 8 DN groups for an intent vector (flyzero/instruct.py:71-82)."""
+    """Target rates of the 8 DN groups for an intent vector (the continuous form of
+    flyzero/instruct.py:71-82): DNa02 follows steering on its own side, DNg02 (the wings) the
+    opposite side's legs, DNa01 leaning, DNp09 gas and MDN brake."""
     steer, lean, gas, brake = (float(v) for v in x[:4])
-    hi, lo, rest = p.high_hz, p.low_hz, p.rest_hz
-
-    def pair(v):  # v in -1..1 -> (left-side, right-side) rates around rest
-        return (rest + (hi - rest) * max(-v, 0) - (rest - lo) * max(v, 0),
-                rest + (hi - rest) * max(v, 0) - (rest - lo) * max(-v, 0))
-
-    a02L, a02R = pair(steer)
-    return {"a02L": a02L, "a02R": a02R, "g02L": a02R, "g02R": a02L,   # wings mirror the legs
-            "a01L": lo + (hi - lo) * max(-lean, 0), "a01R": lo + (hi - lo) * max(lean, 0),
-            "gas": lo + (hi - lo) * gas, "brake": lo + (hi - lo) * brake}
+    legs_l, legs_r = _opponent(steer, p)
+    return {"a02L": legs_l, "a02R": legs_r, "g02L": legs_r, "g02R": legs_l,
+            "a01L": _level(max(-lean, 0.0), p), "a01R": _level(max(lean, 0.0), p),
+            "gas": _level(gas, p), "brake": _level(brake, p)}
 
 
 def targets_from_buttons(b: dict, p: InstructParams) -> dict:
-    """Target rates for pressed buttons (flyzero/instruct.py:85-99)."""
+    """Target rates for pressed buttons (flyzero/instruct.py:85-99): the all-or-none form of
+    ``targets_from_intent``.  One steering button held: that side's legs (DNa02) and the other
+    side's wings (DNg02) at the high rate, the rest low; both or none: all four at rest."""
+    left, right = bool(b.get("LEFT", False)), bool(b.get("RIGHT", False))
     hi, lo, rest = p.high_hz, p.low_hz, p.rest_hz
-    left, right = b.get("LEFT", False), b.get("RIGHT", False)
-    if left and not right:
-        t = dict(a02L=hi, a02R=lo, g02R=hi, g02L=lo)
-    elif right and not left:
-        t = dict(a02L=lo, a02R=hi, g02R=lo, g02L=hi)
-    else:
-        t = dict(a02L=rest, a02R=rest, g02R=rest, g02L=rest)
-    t["a01L"] = hi if b.get("L") else lo
-    t["a01R"] = hi if b.get("R") else lo
-    t["gas"] = hi if b.get("B") else lo
-    t["brake"] = hi if b.get("Y") else lo
-    return t
+    legs_l, legs_r = (hi, lo) if left and not right else (lo, hi) if right and not left else (rest, rest)
+    on = lambda key: hi if b.get(key) else lo   # noqa: E731
+    return {"a02L": legs_l, "a02R": legs_r, "g02R": legs_l, "g02L": legs_r,
+            "a01L": on("L"), "a01R": on("R"), "gas": on("B"), "brake": on("Y")}
 
 
 def targets(x, p: InstructParams) -> list:

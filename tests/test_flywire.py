@@ -91,3 +91,21 @@ def test_csr_cache_round_trip(dataset, tmp_path):
 def test_missing_files_are_reported(tmp_path):
     with pytest.raises(FileNotFoundError, match="missing"):
         load_flywire(tmp_path)
+
+
+def test_repeated_annotation_ids_take_the_first_row_and_bad_indices_are_refused(dataset):
+    import pandas as pd
+
+    from flyhip.flywire import counts_to_csr
+
+    ann = pd.read_csv(dataset / "neuron_annotations.tsv", sep="\t")
+    extra = ann[ann["root_783"] == ROOTS[0]].copy()
+    extra["cell_type"] = "not_this_one"
+    pd.concat([ann, extra]).to_csv(dataset / "neuron_annotations.tsv", sep="\t", index=False)
+    conn = load_flywire(dataset)
+    assert conn.neurons["cell_type"].iloc[0] == "DNg02_a"
+    with pytest.raises(ValueError, match="outside"):
+        counts_to_csr(np.array([0, 9]), np.array([1, 2]), np.array([1.0, 2.0]), 8)
+    w = counts_to_csr(np.array([2, 0, 2, 0]), np.array([1, 3, 1, 1]), np.array([1.0, 2.0, 4.0, -1.0]), 4)
+    np.testing.assert_array_equal(w.toarray(), [[0, -1, 0, 2], [0, 0, 0, 0], [0, 5, 0, 0], [0, 0, 0, 0]])
+    assert w.has_sorted_indices and w.indices.dtype == np.int32

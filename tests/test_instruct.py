@@ -27,6 +27,11 @@ def test_targets_closed_forms():
     b = targets_from_buttons({"LEFT": True, "B": True}, p)
     assert (b["a02L"], b["a02R"], b["g02R"], b["g02L"], b["gas"]) == (p.high_hz, p.low_hz, p.high_hz, p.low_hz,
                                                                       p.high_hz)
+    both = targets_from_buttons({"LEFT": True, "RIGHT": True, "L": True}, p)
+    assert (both["a02L"], both["a02R"], both["g02L"], both["g02R"], both["a01L"]) == (p.rest_hz,) * 4 + (p.high_hz,)
+    # the button form is the intent form at full commands
+    for buttons, x in (({"RIGHT": True, "Y": True}, [1.0, 0.0, 0.0, 1.0]), ({"LEFT": True, "R": True}, [-1.0, 1.0, 0.0, 0.0])):
+        assert targets_from_buttons(buttons, p) == targets_from_intent(x, p)
     tl = targets([0.0, 0.0, 0.0, 0.0, 1.0], p)
     assert len(tl) == len(GROUPS) == 9 and tl[-1] == p.high_hz     # the giant fiber follows boost
 
