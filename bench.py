@@ -236,14 +236,19 @@ def instruct_line(fly, p, torch, B=64, frames=100, warmup=3):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     rule_ms = sum(ev[2 * i].elapsed_time(ev[2 * i + 1]) for i in range(frames)) / frames
-    changed = (bb.pw != w0)
+    # edits the simulation sees: the plastic weights as the engine quantises them on use,
+    # rint(pw * 2^frac_bits / w_syn) (frac_bits = 8 for a whole-synapse connectome with plastic
+    # synapses: w_syn / 256 resolution)
+    qs = float(bb.pw_qscale)
+    changed = torch.round(bb.pw * qs) != torch.round(w0 * qs)
     res = {"B": B, "flies": int(bi.n_flies), "frames": frames, "dn_synapses": int(len(bi.dn_sel_np)),
            "deep_synapses": int(bi.n_deep), "l1_neurons": int(len(bi.l1)),
            "value": round(B * frames * bb.steps_of(FRAME_MS) * p.dt / 1000.0 / wall, 3),
            "unit": "fly-sim-seconds/wall-second", "ms_per_frame": round(wall / frames * 1000.0, 3),
            "instruct_step_ms": round(rule_ms, 4),
            "weights_changed": {"dn": int(changed[:, torch.from_numpy(bi.dn_sel_np).cuda()].sum()),
-                               "deep": int(changed[:, torch.from_numpy(bi.deep_sel_np).cuda()].sum())},
+                               "deep": int(changed[:, torch.from_numpy(bi.deep_sel_np).cuda()].sum()),
+                               "counted_as": f"rint(pw * 2^{bb.frac_bits} / w_syn) changed"},
            "launch_shape": shape,
            "workload": "tile engine + BatchInstructDeep every frame (4 slots per fly, eta_deep 1e-6)"}
     del bb, bi, probs

@@ -26,7 +26,7 @@ import scipy.sparse as sp
 import torch
 
 from . import _lib
-from .params import LIFParams, derived, frac_bits_for, quantize, scales
+from .params import LIFParams, derived, frac_bits_for, plastic_frac_bits, quantize, scales
 from .state import MembraneView
 
 
@@ -99,7 +99,11 @@ class BatchBrain:
             raise ValueError("refractory counter is int8 on the GPU: need round(t_ref/dt) <= 127")
         if self.n_delay > 256:
             raise ValueError("delay ring longer than 256 steps is not supported")
-        self.frac_bits = frac_bits_for(count, self.indices, n) if frac_bits is None else int(frac_bits)
+        if frac_bits is None:
+            frac_bits = frac_bits_for(count, self.indices, n)
+            if frac_bits == 0 and len(pos):   # sub-synapse resolution for the plastic weights
+                frac_bits = plastic_frac_bits(count, self.indices, n, int(tile) if engine == "tile" else None)
+        self.frac_bits = int(frac_bits)
         self.in_scale, self.pw_qscale = scales(p.w_syn, self.frac_bits)
 
         # fixed (non-plastic) CSR, integer weights

@@ -14,6 +14,11 @@ integers in an int32 delay ring, scaling once on arrival by ``in_scale = w_syn *
 
 Plastic weights stay in mV (the reference's ``pw`` contract, flyzero/gpu.py:134-136) and are
 quantised on use as ``rintf(pw * pw_qscale)`` with ``pw_qscale = f32(2**frac_bits / w_syn)``.
+Learning rules that edit them continuously (the instructed and deep rules) would lose every
+edit under half a synapse with ``frac_bits = 0``, so a whole-synapse connectome with plastic
+synapses gets ``PLASTIC_FRAC_BITS`` fractional bits (``plastic_frac_bits``): plastic weights
+resolve w_syn / 256 = 1.07e-3 mV, and the fixed synapses' arrivals are unchanged bit for bit
+(``(float)(c * 2**f) * (w_syn * 2**-f)`` is ``(float)c * w_syn`` exactly).
 """
 
 from __future__ import annotations
@@ -57,6 +62,25 @@ def frac_bits_for(counts: np.ndarray, col: np.ndarray, n: int) -> int:
     colsum = np.bincount(np.asarray(col), weights=np.abs(c.astype(np.float64)), minlength=n)
     cmax = max(float(colsum.max()), 1e-30)
     return int(min(24, max(0, math.floor(math.log2(2.0 ** 27 / cmax)))))
+
+
+PLASTIC_FRAC_BITS = 8
+
+
+def plastic_frac_bits(counts: np.ndarray, col: np.ndarray, n: int, tile: int | None = None) -> int:
+    """Fractional bits for a whole-synapse connectome with plastic synapses: the most, up to
+    ``PLASTIC_FRAC_BITS``, that keep the largest possible per-step arrival (column sum of
+    |count|) below 2**27 and, for the tile engine, every fixed value beside its target's offset in
+    one packed int32 (|count| * 2**f < 2**(31 - log2 tile))."""
+    c = np.abs(np.asarray(counts, np.float64))
+    if c.size == 0:
+        return PLASTIC_FRAC_BITS
+    colmax = float(np.bincount(np.asarray(col), weights=c, minlength=n).max())
+    lbits = max(0, math.ceil(math.log2(tile))) if tile else 0
+    f = PLASTIC_FRAC_BITS
+    while f > 0 and (colmax * 2.0 ** f >= 2.0 ** 27 or (tile and float(c.max()) * 2.0 ** f >= 2.0 ** (31 - lbits))):
+        f -= 1
+    return f
 
 
 def quantize(counts: np.ndarray, frac_bits: int) -> np.ndarray:

@@ -243,7 +243,8 @@ def test_exam_matches_the_cpu_chain_of_oracles(taps, schema):
     # the same drive on the CPU: oracle eye per fly -> oracle brain -> readout -> buttons -> game
     from flyhip import LIFParams
 
-    ob = OracleBrain(fly.weights, LIFParams(dt=0.25), batch=B, seed=11, plastic_pos=fleet.pos)
+    ob = OracleBrain(fly.weights, LIFParams(dt=0.25), batch=B, seed=11, plastic_pos=fleet.pos,
+                     frac_bits=fleet.brain.frac_bits)
     ob.set_bias(fleet.bias_idx, 7.8)
     e = fleet.eye
     ej = ob.slots(e.idx)

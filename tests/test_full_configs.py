@@ -111,7 +111,7 @@ def test_b64_reward_plasticity_on_7387_synapses_matches_oracles(full_fly):
     gb = BatchBrain(fly.weights, p, batch=B, seed=3, plastic_pos=pos, engine="tile", tile=1024)
     gb.set_input(idx, rates[0])
     gb.set_tune(**autotune.tune(gb, use_cache=False, reps=1))
-    ob = OracleBrain(fly.weights, p, batch=B, seed=3, plastic_pos=pos)
+    ob = OracleBrain(fly.weights, p, batch=B, seed=3, plastic_pos=pos, frac_bits=gb.frac_bits)
     ob.alive = np.isin(np.arange(B), sample).astype(np.uint8)
     lp = LearnParams(tau_elig_ms=200.0, eta=5e-2, w_abs_max=50.0)
     br = BatchReward(gb, fly, lp)                                 # fly_of_slot = arange(B)

@@ -25,8 +25,9 @@ def _np(t):
     return t.detach().cpu().numpy()
 
 
-def _oracle_counts(w, dt=0.25, pw=None, plastic_pos=None, ms=50):
-    ob = OracleBrain(w, LIFParams(dt=dt), batch=1 if pw is None else len(pw), plastic_pos=plastic_pos)
+def _oracle_counts(w, dt=0.25, pw=None, plastic_pos=None, ms=50, frac_bits=None):
+    ob = OracleBrain(w, LIFParams(dt=dt), batch=1 if pw is None else len(pw), plastic_pos=plastic_pos,
+                     frac_bits=frac_bits)
     if pw is not None:
         ob.pw[...] = pw
     ob.v[:, :40] = -40.0
@@ -63,7 +64,7 @@ def test_each_fly_has_its_own_plastic_synapses(engine):
     gb.v[:, :40] = -40.0
     out = _np(gb.run(50))
     pw = np.stack([gb.w0, changed[pos]])
-    ref = _oracle_counts(w, pw=pw, plastic_pos=pos)
+    ref = _oracle_counts(w, pw=pw, plastic_pos=pos, frac_bits=gb.frac_bits)
     np.testing.assert_array_equal(out, ref)
     assert not np.array_equal(out[0], out[1])
 
@@ -78,7 +79,7 @@ def test_plastic_path_equals_fixed_path_for_whole_synapse_counts(engine):
     ws.sort_indices()
     pos = np.flatnonzero(ws.indices < 150)
     gb = BatchBrain(w, LIFParams(dt=0.25), batch=1, plastic_pos=pos, engine=engine)
-    assert gb.frac_bits == 0 and gb.w_bits == 16
+    assert gb.frac_bits == 8 and gb.w_bits == 32        # sub-synapse plastic weights; fixed sums exact
     gb.v[:, :40] = -40.0
     out = _np(gb.run(50))
     np.testing.assert_array_equal(out, _oracle_counts(w))
@@ -110,7 +111,7 @@ def test_long_plastic_rows_cover_every_synapse(engine):
     gb.pw[1] *= 0.5
     gb.v[:, :40] = -40.0
     out = _np(gb.run(30))
-    ob = OracleBrain(w, LIFParams(dt=0.25), batch=2, plastic_pos=pos)
+    ob = OracleBrain(w, LIFParams(dt=0.25), batch=2, plastic_pos=pos, frac_bits=gb.frac_bits)
     ob.pw[1] *= 0.5
     ob.v[:, :40] = -40.0
     ref = ob.run(30)
@@ -127,7 +128,7 @@ def small_fly():
 def _pair(fly, batch=1, seed=0, plastic_pos=None, engine="push", tile=1024):
     p = LIFParams()
     gb = BatchBrain(fly.weights, p, batch=batch, seed=seed, plastic_pos=plastic_pos, engine=engine, tile=tile)
-    ob = OracleBrain(fly.weights, p, batch=batch, seed=seed, plastic_pos=plastic_pos)
+    ob = OracleBrain(fly.weights, p, batch=batch, seed=seed, plastic_pos=plastic_pos, frac_bits=gb.frac_bits)
     idx, rate = fly_inputs(fly)
     gb.set_input(idx, rate)
     ob.set_input(idx, rate)

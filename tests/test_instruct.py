@@ -314,3 +314,30 @@ def test_batch_motor_rates_and_buttons(fly):
             ema[:, k] = ema[:, k] + a * (inst - ema[:, k])
         np.testing.assert_allclose(motor.rates, ema, rtol=1e-5)
         assert len(buttons) == B and all(b["A"] == (motor.rates[i, 8] > 0.0) for i, b in enumerate(buttons))
+
+
+@pytest.mark.gpu
+def test_default_rates_change_the_weights_the_simulation_uses(fly):
+    # the reference's default rates (InstructParams.eta 2e-4, eta_deep 1e-6) make edits far below
+    # half a synapse per frame; with sub-synapse fixed point for plastic weights (frac_bits 8,
+    # w_syn / 256) they reach the quantised weights the engine adds, rint(pw * 2^8 / w_syn)
+    import torch
+
+    from flyhip.instruct import BatchInstructDeep
+
+    gb = _brain(fly, 4, layers=1)
+    assert gb.frac_bits == 8
+    gb.set_input(np.r_[fly.input_idx, fly.drive_idx], np.r_[fly.input_rate, f32(60.0)])
+    bi = BatchInstructDeep(gb, fly, fly_of_slot=[0, 0, 1, 1])
+    assert bi.p.eta == 2e-4 and bi.eta_deep == 1e-6
+    qs = float(gb.pw_qscale)
+    q0 = torch.round(gb.pw * qs)
+    tgt = np.array([targets([1.0, 0.0, 1.0, 0.0, 0.0], bi.p)] * 4, f32)
+    for f in range(40):
+        bi.step(gb.run(16.6), tgt, np.ones(4, bool), 16.6, report=False)
+    moved = _np(torch.round(gb.pw * qs) != q0)
+    assert moved[:, bi.dn_sel_np].any(), "DN synapse edits never reach the simulation"
+    assert moved[:, bi.deep_sel_np].any(), "deep synapse edits never reach the simulation"
+    # edits the engine sees now that whole-synapse rounding (frac_bits 0) would have hidden
+    small = np.abs(_np(gb.pw) - _np(q0 / qs)) < 0.5 * float(gb.p.w_syn)
+    assert (moved & small)[:, bi.deep_sel_np].any()

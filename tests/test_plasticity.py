@@ -193,7 +193,7 @@ def test_closed_loop_simulation_and_learning_match_oracles(small_fly):
     pos = plastic_positions(small_fly)
     B, window = 4, 1000.0 / 60.0988
     gb = BatchBrain(small_fly.weights, LIFParams(), batch=B, seed=2, plastic_pos=pos)
-    ob = OracleBrain(small_fly.weights, LIFParams(), batch=B, seed=2, plastic_pos=pos)
+    ob = OracleBrain(small_fly.weights, LIFParams(), batch=B, seed=2, plastic_pos=pos, frac_bits=gb.frac_bits)
     idx = np.r_[small_fly.input_idx, small_fly.drive_idx]
     rate = np.r_[small_fly.input_rate, 60.0].astype(np.float32)
     for b in (gb, ob):
