@@ -54,12 +54,25 @@ struct GemmArgs {
     int lda, ldb, ldc;
     int mode;           // FZ_SG_*
     float* C32;         // modes STORE32, ACC32; pre-activation for GELU
-    uint16_t* C16;      // modes STORE16, GELU
+    uint16_t* C16;      // modes STORE16, GELU, GELU16
     const float* bias;  // [N] or null
+    long long sa, sb, sc, sbias;  // batched: element offsets between the products of blockIdx.z
 };
+
+// The product of blockIdx.z in a batched launch (every operand offset by z times its stride).
+__device__ __forceinline__ void batch_offsets(GemmArgs& g) {
+    const long long z = blockIdx.z;
+    if (z == 0) return;
+    g.A += z * g.sa;
+    g.B += z * g.sb;
+    if (g.C32) g.C32 += z * g.sc;
+    if (g.C16) g.C16 += z * g.sc;
+    if (g.bias) g.bias += z * g.sbias;
+}
 
 template <bool TA, bool TB>
 __global__ void __launch_bounds__(256) sg_gemm(GemmArgs g) {
+    batch_offsets(g);
     __shared__ uint16_t As[BM][LDK];
     __shared__ uint16_t Bs[BN][LDK];  // B staged as [n][k]
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -133,6 +146,7 @@ __global__ void __launch_bounds__(256) sg_gemm(GemmArgs g) {
                     case FZ_SG_STORE32: g.C32[o] = v; break;
                     case FZ_SG_ACC32: g.C32[o] += v; break;
                     case FZ_SG_STORE16: g.C16[o] = f2bf(v); break;
+                    case FZ_SG_GELU16: g.C16[o] = f2bf(gelu(v)); break;
                     default:  // FZ_SG_GELU
                         g.C32[o] = v;
                         g.C16[o] = f2bf(gelu(v));
@@ -203,6 +217,7 @@ struct TileLoad {
 
 template <bool TA, bool TB>
 __global__ void __launch_bounds__(256) sg_gemm_big(GemmArgs g) {
+    batch_offsets(g);
     __shared__ __attribute__((aligned(16))) uint16_t As[GB_M][LDK];
     __shared__ __attribute__((aligned(16))) uint16_t Bs[GB_N][LDK];  // B staged as [n][k]
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -254,6 +269,7 @@ __global__ void __launch_bounds__(256) sg_gemm_big(GemmArgs g) {
                     case FZ_SG_STORE32: g.C32[o] = v; break;
                     case FZ_SG_ACC32: g.C32[o] += v; break;
                     case FZ_SG_STORE16: g.C16[o] = f2bf(v); break;
+                    case FZ_SG_GELU16: g.C16[o] = f2bf(gelu(v)); break;
                     default:
                         g.C32[o] = v;
                         g.C16[o] = f2bf(gelu(v));
@@ -299,6 +315,89 @@ __global__ void sg_gru_fwd_k(int B, int H, const float* gx, const float* gh, con
         gs[H + j] = z;
         gs[2 * H + j] = nn;
         gs[3 * H + j] = y[2 * H + j];
+    }
+}
+
+// Output layer of every ensemble member and the ensemble's statistics:
+// y_k = ([h2_k, s_k] Wo_k^T + bo_k) * rate_scale for members k < E (bf16 inputs and weights,
+// FP32 sums), then per rate the mean over members and their population variance (Welford), and
+// the relative disagreement ||std|| / (||mean|| + 1 Hz).  h2 [E][B][hid], s [E][B][H] (bf16);
+// Wo_k at w16 + k * wstride + wo_off ([n_pad][hid + H]); bo_k at w32 + k * wstride + bo_off.
+// A workgroup stages the E x n_out used rows of Wo in LDS once (16-byte vectors) and serves
+// ENS_ROWS frames, one wavefront per frame at a time: each lane takes whole 8-feature vectors
+// of the frame's [h2, s] row, then a butterfly sum over the lanes.  hid, H, wstride and wo_off
+// are multiples of 8 and the pointers 16-byte aligned (checked by the host).
+constexpr int ENS_MAX = 8, OUT_MAX = 8, ENS_ROWS = 16;
+__global__ void __launch_bounds__(256) sg_ens_head_k(int B, int E, int hid, int H, int n_out, const uint16_t* h2,
+                                                     const uint16_t* s, const uint16_t* w16, const float* w32,
+                                                     long long wstride, long long wo_off, long long bo_off,
+                                                     float rate_scale, float* mean_out, float* spread_out) {
+    extern __shared__ __attribute__((aligned(16))) uint16_t wl[];  // [E][n_out][F]
+    const int F = hid + H, FV = F >> 3, HV = hid >> 3;
+    u32x4* wv = reinterpret_cast<u32x4*>(wl);
+    for (int q = threadIdx.x; q < E * n_out * FV; q += blockDim.x) {  // rows o < n_out of each Wo_k
+        const int row = q / FV, v = q - row * FV;
+        const int k = row / n_out, o = row - k * n_out;
+        wv[q] = reinterpret_cast<const u32x4*>(w16 + k * wstride + wo_off + (size_t)o * F)[v];
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int r = wave; r < ENS_ROWS; r += nw) {
+        const int b = blockIdx.x * ENS_ROWS + r;
+        if (b >= B) break;  // wavefront-uniform
+        float mean[OUT_MAX], m2[OUT_MAX];
+#pragma unroll
+        for (int o = 0; o < OUT_MAX; o++) mean[o] = m2[o] = 0.f;
+        for (int k = 0; k < E; k++) {
+            const u32x4* xh = reinterpret_cast<const u32x4*>(h2 + ((size_t)k * B + b) * hid);
+            const u32x4* xs = reinterpret_cast<const u32x4*>(s + ((size_t)k * B + b) * H);
+            const u32x4* wk = wv + (size_t)k * n_out * FV;
+            float part[OUT_MAX];
+#pragma unroll
+            for (int o = 0; o < OUT_MAX; o++) part[o] = 0.f;
+            for (int v = lane; v < FV; v += 64) {
+                const u32x4 xv = v < HV ? xh[v] : xs[v - HV];
+                float x[8];
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    x[2 * j] = __uint_as_float(xv[j] << 16);
+                    x[2 * j + 1] = __uint_as_float(xv[j] & 0xffff0000u);
+                }
+#pragma unroll
+                for (int o = 0; o < OUT_MAX; o++) {
+                    if (o >= n_out) continue;
+                    const u32x4 w = wk[o * FV + v];
+                    float acc = part[o];
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        acc += x[2 * j] * __uint_as_float(w[j] << 16);
+                        acc += x[2 * j + 1] * __uint_as_float(w[j] & 0xffff0000u);
+                    }
+                    part[o] = acc;
+                }
+            }
+#pragma unroll
+            for (int o = 0; o < OUT_MAX; o++) {
+                if (o >= n_out) continue;
+                float v = part[o];
+                for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d);
+                const float y = (v + w32[k * wstride + bo_off + o]) * rate_scale;
+                const float dlt = y - mean[o];
+                mean[o] += dlt / (float)(k + 1);
+                m2[o] += dlt * (y - mean[o]);
+            }
+        }
+        if (lane == 0) {
+            float mn2 = 0.f, sd2 = 0.f;
+#pragma unroll
+            for (int o = 0; o < OUT_MAX; o++) {
+                if (o >= n_out) continue;
+                mean_out[(size_t)b * n_out + o] = mean[o];
+                mn2 += mean[o] * mean[o];
+                sd2 += m2[o] / (float)E;
+            }
+            spread_out[b] = sqrtf(sd2) / (sqrtf(mn2) + 1.0f);
+        }
     }
 }
 
@@ -362,25 +461,25 @@ __global__ void sg_cast_k(long long n, const float* x, uint16_t* y) {
 
 inline unsigned blocks(long long n, int t) { return (unsigned)((n + t - 1) / t); }
 
-}  // namespace
-
-extern "C" {
-
-int fz_sg_gemm(int M, int N, int K, const void* A, int lda, int trans_a, const void* B, int ldb, int trans_b,
-               int mode, float* C32, void* C16, int ldc, const float* bias, void* stream) {
+int gemm_launch(int count, int M, int N, int K, const void* A, int lda, long long sa, int trans_a, const void* B,
+                int ldb, long long sb, int trans_b, int mode, float* C32, void* C16, int ldc, long long sc,
+                const float* bias, long long sbias, void* stream) {
+    if (count < 1 || count > 65535) return fzi_fail(FZ_E_ARG, "gemm: batch count %d outside [1, 65535]", count);
     if (M < 1 || N < 1 || K < 1 || K % BK) return fzi_fail(FZ_E_ARG, "gemm %dx%dx%d: K must be a positive multiple of %d", M, N, K, BK);
     if (!A || !B) return fzi_fail(FZ_E_ARG, "gemm: null operand");
     if ((mode == FZ_SG_STORE32 || mode == FZ_SG_ACC32 || mode == FZ_SG_GELU) && !C32)
         return fzi_fail(FZ_E_ARG, "gemm: mode %d needs C32", mode);
-    if ((mode == FZ_SG_STORE16 || mode == FZ_SG_GELU) && !C16) return fzi_fail(FZ_E_ARG, "gemm: mode %d needs C16", mode);
-    if (mode < FZ_SG_STORE32 || mode > FZ_SG_GELU) return fzi_fail(FZ_E_ARG, "gemm: unknown mode %d", mode);
+    if ((mode == FZ_SG_STORE16 || mode == FZ_SG_GELU || mode == FZ_SG_GELU16) && !C16)
+        return fzi_fail(FZ_E_ARG, "gemm: mode %d needs C16", mode);
+    if (mode < FZ_SG_STORE32 || mode > FZ_SG_GELU16) return fzi_fail(FZ_E_ARG, "gemm: unknown mode %d", mode);
+    if (sa < 0 || sb < 0 || sc < 0 || sbias < 0) return fzi_fail(FZ_E_ARG, "gemm: negative batch stride");
     GemmArgs g{M, N, K, static_cast<const uint16_t*>(A), static_cast<const uint16_t*>(B), lda, ldb, ldc, mode, C32,
-               static_cast<uint16_t*>(C16), bias};
+               static_cast<uint16_t*>(C16), bias, sa, sb, sc, sbias};
     hipStream_t s = static_cast<hipStream_t>(stream);
-    const bool big = M >= GB_M && N >= GB_N && lda % 8 == 0 && ldb % 8 == 0 &&
+    const bool big = M >= GB_M && N >= GB_N && lda % 8 == 0 && ldb % 8 == 0 && sa % 8 == 0 && sb % 8 == 0 &&
                      (reinterpret_cast<uintptr_t>(A) & 15) == 0 && (reinterpret_cast<uintptr_t>(B) & 15) == 0;
     if (big) {
-        const dim3 gridb((N + GB_N - 1) / GB_N, (M + GB_M - 1) / GB_M);
+        const dim3 gridb((N + GB_N - 1) / GB_N, (M + GB_M - 1) / GB_M, count);
         if (!trans_a && !trans_b) hipLaunchKernelGGL((sg_gemm_big<false, false>), gridb, dim3(256), 0, s, g);
         else if (!trans_a && trans_b) hipLaunchKernelGGL((sg_gemm_big<false, true>), gridb, dim3(256), 0, s, g);
         else if (trans_a && !trans_b) hipLaunchKernelGGL((sg_gemm_big<true, false>), gridb, dim3(256), 0, s, g);
@@ -388,13 +487,48 @@ int fz_sg_gemm(int M, int N, int K, const void* A, int lda, int trans_a, const v
         if (hipGetLastError() != hipSuccess) return fzi_fail(FZ_E_HIP, "sg_gemm launch failed");
         return FZ_OK;
     }
-    const dim3 grid((N + BN - 1) / BN, (M + BM - 1) / BM);
+    const dim3 grid((N + BN - 1) / BN, (M + BM - 1) / BM, count);
     if (!trans_a && !trans_b) hipLaunchKernelGGL((sg_gemm<false, false>), grid, dim3(256), 0, s, g);
     else if (!trans_a && trans_b) hipLaunchKernelGGL((sg_gemm<false, true>), grid, dim3(256), 0, s, g);
     else if (trans_a && !trans_b) hipLaunchKernelGGL((sg_gemm<true, false>), grid, dim3(256), 0, s, g);
     else hipLaunchKernelGGL((sg_gemm<true, true>), grid, dim3(256), 0, s, g);
     if (hipGetLastError() != hipSuccess) return fzi_fail(FZ_E_HIP, "sg_gemm launch failed");
     return FZ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fz_sg_gemm(int M, int N, int K, const void* A, int lda, int trans_a, const void* B, int ldb, int trans_b,
+               int mode, float* C32, void* C16, int ldc, const float* bias, void* stream) {
+    return gemm_launch(1, M, N, K, A, lda, 0, trans_a, B, ldb, 0, trans_b, mode, C32, C16, ldc, 0, bias, 0, stream);
+}
+
+int fz_sg_gemm_batched(int count, int M, int N, int K, const void* A, int lda, long long stride_a, int trans_a,
+                       const void* B, int ldb, long long stride_b, int trans_b, int mode, float* C32, void* C16,
+                       int ldc, long long stride_c, const float* bias, long long stride_bias, void* stream) {
+    return gemm_launch(count, M, N, K, A, lda, stride_a, trans_a, B, ldb, stride_b, trans_b, mode, C32, C16, ldc,
+                       stride_c, bias, stride_bias, stream);
+}
+
+int fz_sg_ens_head(int B, int members, int hid, int H, int n_out, const void* h2, const void* s, const void* w16,
+                   const float* w32, long long wstride, long long wo_off, long long bo_off, float rate_scale,
+                   float* mean, float* spread, void* stream) {
+    if (B < 1 || members < 1 || members > ENS_MAX || n_out < 1 || n_out > OUT_MAX || hid < 1 || H < 1)
+        return fzi_fail(FZ_E_ARG, "ens_head: B %d, members %d (1..%d), n_out %d (1..%d)", B, members, ENS_MAX, n_out,
+                        OUT_MAX);
+    if (!h2 || !s || !w16 || !w32 || !mean || !spread) return fzi_fail(FZ_E_ARG, "ens_head: null pointer");
+    if (hid % 8 || H % 8 || wstride % 8 || wo_off % 8 || ((reinterpret_cast<uintptr_t>(h2) | reinterpret_cast<uintptr_t>(s) |
+                                                          reinterpret_cast<uintptr_t>(w16)) & 15))
+        return fzi_fail(FZ_E_ARG, "ens_head: widths, strides and offsets must be multiples of 8 elements, pointers "
+                        "16-byte aligned");
+    const size_t lds = (size_t)members * n_out * (hid + H) * sizeof(uint16_t);
+    if (lds > 64 * 1024) return fzi_fail(FZ_E_ARG, "ens_head: %zu B of output weights exceed 64 KB of LDS", lds);
+    hipLaunchKernelGGL(sg_ens_head_k, dim3(blocks(B, ENS_ROWS)), dim3(256), lds, static_cast<hipStream_t>(stream), B,
+                       members, hid, H, n_out, static_cast<const uint16_t*>(h2), static_cast<const uint16_t*>(s),
+                       static_cast<const uint16_t*>(w16), w32, wstride, wo_off, bo_off, rate_scale, mean, spread);
+    return hipGetLastError() == hipSuccess ? FZ_OK : fzi_fail(FZ_E_HIP, "ens_head launch failed");
 }
 
 int fz_sg_gelu_bwd(long long n, const float* dact, const float* pre, float* dpre32, void* dpre16, void* stream) {

@@ -71,6 +71,12 @@ SIGNATURES = {
     "fz_reward_step": (c_int, [c_void_p, c_void_p]),
     "fz_sg_gemm": (c_int, [c_int, c_int, c_int, c_void_p, c_int, c_int, c_void_p, c_int, c_int, c_int, c_void_p,
                            c_void_p, c_int, c_void_p, c_void_p]),
+    "fz_sg_gemm_batched": (c_int, [c_int, c_int, c_int, c_int, c_void_p, c_int, ctypes.c_longlong, c_int, c_void_p,
+                                   c_int, ctypes.c_longlong, c_int, c_int, c_void_p, c_void_p, c_int,
+                                   ctypes.c_longlong, c_void_p, ctypes.c_longlong, c_void_p]),
+    "fz_sg_ens_head": (c_int, [c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+                               ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_float, c_void_p,
+                               c_void_p, c_void_p]),
     "fz_sg_gelu_bwd": (c_int, [ctypes.c_longlong, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "fz_sg_colsum": (c_int, [c_int, c_int, c_void_p, c_void_p, c_int, c_void_p]),
     "fz_sg_gru_fwd": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),

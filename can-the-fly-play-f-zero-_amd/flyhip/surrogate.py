@@ -30,7 +30,7 @@ import torch
 from . import _lib
 
 N_OUT_PAD = 32
-STORE32, ACC32, STORE16, GELU = 0, 1, 2, 3
+STORE32, ACC32, STORE16, GELU, GELU16 = 0, 1, 2, 3, 4
 
 
 def _p(t):
@@ -233,20 +233,78 @@ class RateSurrogate:
 
 class Ensemble:
     """Independently initialised members; the spread of their predictions flags frames for the
-    full simulation."""
+    full simulation.
+
+    ``predict`` runs every member in the same launches: the members' weights are stacked
+    contiguously (``pack``; call it again after training a member further), each layer is one
+    member-batched GEMM (the input shared, stride 0), the GRU one launch over all members' rows,
+    and one kernel computes every member's output layer, the ensemble mean and the disagreement.
+    ``predict_members`` is the same computation member by member."""
 
     def __init__(self, members: list[RateSurrogate]):
         self.members = members
         self.state = None
+        self._W16 = self._W32 = None
+        m0 = members[0]
+        for m in members:
+            if (m.n_in, m.hid, m.H, m.n_out, m.rate_scale) != (m0.n_in, m0.hid, m0.H, m0.n_out, m0.rate_scale):
+                raise ValueError("ensemble members must share their shape and rate scale")
+
+    def pack(self):
+        self._W16 = torch.stack([m.w16 for m in self.members]).contiguous()
+        self._W32 = torch.stack([m.w for m in self.members]).contiguous()
 
     def reset(self, B):
-        H = self.members[0].H
+        m0 = self.members[0]
+        E, H, hid, d = len(self.members), m0.H, m0.hid, m0.dev
         self.state = [torch.zeros((B, H), dtype=torch.float32, device=m.dev) for m in self.members]
+        self._S = torch.zeros((2, E, B, H), dtype=torch.float32, device=d)
+        self._S16 = torch.zeros((2, E, B, H), dtype=torch.bfloat16, device=d)
+        self._cur = 0
+        self._h1 = torch.empty((E, B, hid), dtype=torch.bfloat16, device=d)
+        self._h2 = torch.empty((E, B, hid), dtype=torch.bfloat16, device=d)
+        self._gx = torch.empty((E, B, 3 * H), dtype=torch.float32, device=d)
+        self._gh = torch.empty((E, B, 3 * H), dtype=torch.float32, device=d)
+        self._mean = torch.empty((B, m0.n_out), dtype=torch.float32, device=d)
+        self._spread = torch.empty(B, dtype=torch.float32, device=d)
 
     def predict(self, X16: torch.Tensor):
         """One frame for B sequences: X16 [B, n_in] bf16.  Returns (mean rates [B, n_out] in Hz,
         relative disagreement [B]: the members' standard deviation over the rate vector relative
-        to the mean rate vector, ||std|| / (||mean|| + 1 Hz))."""
+        to the mean rate vector, ||std|| / (||mean|| + 1 Hz)); views into buffers that the next
+        call overwrites."""
+        if self._W16 is None:
+            self.pack()
+        m0 = self.members[0]
+        E, (B, n_in), hid, H = len(self.members), X16.shape, m0.hid, m0.H
+        if n_in != m0.n_in or self._mean.shape[0] != B:
+            raise ValueError("predict: input does not match the ensemble or the batch of reset()")
+        X16 = X16.contiguous()
+        lib, st, sz = m0.lib, m0._stream(), m0.size
+        o = {k: v[0] for k, v in m0.off.items()}
+        W16, W32 = self._W16.data_ptr(), self._W32.data_ptr()
+        w = lambda name: W16 + 2 * o[name]          # noqa: E731  (member 0's block; members at + sz)
+        b = lambda name: W32 + 4 * o[name]          # noqa: E731
+        gb = lambda *a: _lib.check(lib.fz_sg_gemm_batched(*a, st), "fz_sg_gemm_batched")  # noqa: E731
+        cur, nxt = self._cur, 1 - self._cur
+        gb(E, B, hid, n_in, _p(X16), n_in, 0, 0, w("W1"), n_in, sz, 1, GELU16, None, _p(self._h1), hid, B * hid,
+           b("b1"), sz)
+        gb(E, B, hid, hid, _p(self._h1), hid, B * hid, 0, w("W2"), hid, sz, 1, GELU16, None, _p(self._h2), hid,
+           B * hid, b("b2"), sz)
+        gb(E, B, 3 * H, hid, _p(self._h2), hid, B * hid, 0, w("Wih"), hid, sz, 1, STORE32, _p(self._gx), None,
+           3 * H, B * 3 * H, b("bih"), sz)
+        gb(E, B, 3 * H, H, _p(self._S16[cur]), H, B * H, 0, w("Whh"), H, sz, 1, STORE32, _p(self._gh), None,
+           3 * H, B * 3 * H, b("bhh"), sz)
+        _lib.check(lib.fz_sg_gru_fwd(E * B, H, _p(self._gx), _p(self._gh), _p(self._S[cur]), _p(self._S[nxt]),
+                                     _p(self._S16[nxt]), None, st), "fz_sg_gru_fwd")
+        self._cur = nxt
+        _lib.check(lib.fz_sg_ens_head(B, E, hid, H, m0.n_out, _p(self._h2), _p(self._S16[nxt]), W16, W32, sz,
+                                      o["Wo"], o["bo"], m0.rate_scale, _p(self._mean), _p(self._spread), st),
+                   "fz_sg_ens_head")
+        return self._mean, self._spread
+
+    def predict_members(self, X16: torch.Tensor):
+        """``predict`` member by member through each member's own forward (for comparison)."""
         ys = []
         for k, m in enumerate(self.members):
             y, s = m.forward(X16.unsqueeze(0), self.state[k])

@@ -1,19 +1,32 @@
-"""Data, training and judging for the learned DN-rate surrogate (``flyhip.surrogate``).
+"""Data, training, fallback and judging for the learned DN-rate surrogate (``flyhip.surrogate``).
 
-* ``VisualDrive``: per-fly visual input on the GPU: drifting Gaussian blobs over the input
-  neurons' retinotopic positions (a different pattern and gain per fly), rates in Hz.
-* ``generate``: runs the batched LIF simulator frame by frame (one game frame = 166 steps) and
-  records the pooled drive (64 x 32 grid of mean input rates, BF16) and the 8 smoothed DN rates
+The workload is the batched simulator's own (B flies sharing the connectome): per-sequence visual
+drive with a visual gain U(0.5, 2.0) and a drive-neuron rate U(20, 120) Hz, and the fleet's
+steady 7.8 mV bias on the motor descending neurons (flyzero/fleet.py:61-63), which puts them in
+the firing regime the fleet runs them in.
+
+* ``VisualDrive``: per-sequence input on the GPU: drifting Gaussian blobs over the input
+  neurons' retinotopic positions, gain and drive rate; rates for all sequences or any subset.
+* ``generate``: runs the batched simulator frame by frame (one game frame = 166 steps) and records
+  the pooled visual drive (64 x 32 grid of mean input rates, BF16) and the 8 smoothed DN rates
   (``DNReadout``, tau = 80 ms, the reference's readout smoothing) of every fly and frame.
-  ``replicas`` flies share one input sequence with independent spiking noise, so the
-  simulation's own expected rate can be estimated by their mean.
-* ``train_ensemble``: members trained by the package's own kernels on random windows of
-  ``T`` consecutive frames (BPTT through the GRU).
-* ``generate_judged`` / ``judge``: held-out sequences, each simulated by several replica flies;
-  the replicas' mean is the simulation's expected rate (the judge).  Reports the surrogate's
-  error with and without fallback, the error of a constant predictor (the method without the
-  network) and of a single simulation trial (the simulation's own trial-to-trial noise), and
-  frames/s with and without fallback.
+  ``replicas`` flies share one sequence with independent spiking noise, so the simulation's
+  expected rate is estimated by their mean.
+* ``sim_frames_per_s``: the full simulation's throughput on the same setup (B = 256, graph
+  replays of the brain alone, inputs already on the device): the denominator of the speedup.
+* ``train_ensemble``: members trained by the package's own kernels on random windows of ``T``
+  consecutive frames (BPTT through the GRU).
+* ``FallbackSim`` and ``surrogate_loop``: the surrogate run frame by frame for S sequences; a
+  frame whose ensemble disagreement exceeds the threshold is simulated instead.  The simulation
+  has no state for a sequence the surrogate has been running, so the fallback re-warms one: a
+  brain at rest runs the sequence's previous ``warm`` frames of drive and then the frame itself
+  (the network's time constants are 5-20 ms, a frame is 16.6 ms), and the frame's simulated DN
+  counts enter the readout's exponential average in place of the network's prediction.  The
+  loop is timed with the fallback simulations in it.
+* ``judge``: held-out sequences, each simulated by several replica flies; the replicas' mean is
+  the simulation's expected rate.  Errors of the surrogate alone, of the loop with its real
+  fallback, of a constant predictor (the method without the network) and of one simulated
+  trial (the simulation's trial-to-trial noise).
 
 ``python -m flyhip.surrogate_pipeline`` runs all of it and prints one JSON report.
 """
@@ -30,29 +43,36 @@ import numpy as np
 import torch
 
 FRAME_MS = 1000.0 / 60.0988
+BIAS_TYPES = ("DNa02", "DNg02*", "DNa01", "DNp09")   # the fleet's biased motor DNs
 
 
 class VisualDrive:
-    def __init__(self, xy: np.ndarray, batch: int, seed: int, replicas: int = 1, n_blobs: int = 6,
-                 device="cuda", peak_hz: float = 200.0):
+    """S input sequences: visual rates [S, n_in] per frame and a drive-neuron rate per sequence."""
+
+    def __init__(self, xy: np.ndarray, sequences: int, seed: int, n_blobs: int = 6, device="cuda",
+                 peak_hz: float = 200.0, gain=(0.5, 2.0), drive_hz=(20.0, 120.0)):
         self.dev = torch.device(device)
-        n_seq = batch // replicas
+        self.S = sequences
         r = np.random.default_rng(seed)
-        f = lambda a: torch.from_numpy(np.repeat(a.astype(np.float32), replicas, 0)).to(self.dev)  # noqa: E731
-        self.c0 = f(r.random((n_seq, n_blobs, 2)))
-        self.vel = f((r.random((n_seq, n_blobs, 2)) - 0.5) * 0.03)
-        self.width = f(0.06 + 0.14 * r.random((n_seq, n_blobs)))
-        self.amp = f(0.3 + 0.7 * r.random((n_seq, n_blobs)))
-        self.gain = f(0.5 + 1.5 * r.random(n_seq))
+        t = lambda a: torch.from_numpy(np.asarray(a, np.float32)).to(self.dev)  # noqa: E731
+        self.c0 = t(r.random((sequences, n_blobs, 2)))
+        self.vel = t((r.random((sequences, n_blobs, 2)) - 0.5) * 0.03)
+        self.width = t(0.06 + 0.14 * r.random((sequences, n_blobs)))
+        self.amp = t(0.3 + 0.7 * r.random((sequences, n_blobs)))
+        self.gain = t(r.uniform(gain[0], gain[1], sequences))
+        self.drive_hz = t(r.uniform(drive_hz[0], drive_hz[1], sequences))
         self.xy = torch.from_numpy(xy.astype(np.float32)).to(self.dev)
         self.peak = float(peak_hz)
 
-    def rates(self, frame: int) -> torch.Tensor:
-        c = torch.remainder(self.c0 + self.vel * float(frame), 1.0)                 # [B, nb, 2]
-        d = torch.abs(self.xy[None, :, None, :] - c[:, None, :, :])                # [B, n, nb, 2]
+    def rates(self, frame: int, seqs: torch.Tensor | None = None) -> torch.Tensor:
+        """Visual rates (Hz) of the input neurons at ``frame``: [S, n_in], or [len(seqs), n_in]."""
+        pick = (lambda a: a) if seqs is None else (lambda a: a[seqs])  # noqa: E731
+        c = torch.remainder(pick(self.c0) + pick(self.vel) * float(frame), 1.0)        # [s, nb, 2]
+        d = torch.abs(self.xy[None, :, None, :] - c[:, None, :, :])                    # [s, n, nb, 2]
         d = torch.minimum(d, 1.0 - d)
-        f = (self.amp[:, None, :] * torch.exp(-(d * d).sum(-1) / (2 * self.width[:, None, :] ** 2))).sum(-1)
-        return self.peak * torch.clamp(f, 0.0, 1.0) * self.gain[:, None]
+        w = pick(self.width)
+        f = (pick(self.amp)[:, None, :] * torch.exp(-(d * d).sum(-1) / (2 * w[:, None, :] ** 2))).sum(-1)
+        return self.peak * torch.clamp(f, 0.0, 1.0) * pick(self.gain)[:, None]
 
 
 class GridPool:
@@ -73,42 +93,87 @@ class GridPool:
         return out * self.inv
 
 
-def generate(fly, frames: int, batch: int, seed: int, replicas: int = 1, engine: str = "tile", tile: int = 1024,
-             skip: int = 10, drive_hz: float = 60.0, rate_norm: float = 200.0, log=None):
-    """Simulate ``frames`` + ``skip`` frames of ``batch`` flies; returns (X bf16 [frames, B, 2048]
-    normalised pooled drive, Y f32 [frames, B, 8] smoothed DN rates in Hz, sim frames/s)."""
-    from . import BatchBrain, LIFParams, autotune
+class _Driven:
+    """A BatchBrain wired to a ``VisualDrive``: visual slots, the drive neuron and the DN bias."""
+
+    def __init__(self, fly, batch: int, seed: int, dn_bias_mv: float, engine="tile", tile=1024, tune=True):
+        from . import BatchBrain, LIFParams, autotune
+
+        self.p = LIFParams()
+        self.bb = bb = BatchBrain(fly.weights, self.p, batch=batch, seed=seed, engine=engine, tile=tile)
+        self.j_in = bb.slots(fly.input_idx)
+        self.j_dr = bb.slots([fly.drive_idx])
+        if dn_bias_mv:
+            bias_idx = np.concatenate([fly.find(t) for t in BIAS_TYPES])
+            bb.set_bias(bias_idx, dn_bias_mv)
+        self.dt_s = float(np.float32(self.p.dt / 1000.0))
+        self.tuned = autotune.tune(bb, reps=1) if tune else None
+
+    def set_rates(self, visual: torch.Tensor, drive: torch.Tensor, rows=None):
+        if rows is None:
+            self.bb.p_ext[:, self.j_in] = visual * self.dt_s
+            self.bb.p_ext[:, self.j_dr] = drive[:, None] * self.dt_s
+        else:
+            self.bb.p_ext[rows[:, None], self.j_in[None, :]] = visual * self.dt_s
+            self.bb.p_ext[rows, self.j_dr[0]] = drive * self.dt_s
+
+
+def generate(fly, frames: int, batch: int, seed: int, replicas: int = 1, skip: int = 10, dn_bias_mv: float = 7.8,
+             rate_norm: float = 200.0, log=None, sim: "_Driven | None" = None):
+    """Simulate ``frames`` + ``skip`` frames of ``batch`` flies (``batch // replicas`` sequences);
+    returns (X bf16 [frames, S, 2048] normalised pooled visual drive of each sequence, Y f32
+    [frames, batch, 8] smoothed DN rates in Hz, the sequences' drive).  ``sim``: a brain of
+    ``batch`` flies to reuse (reset to rest first)."""
     from .batch import DNReadout
 
-    p = LIFParams()
-    bb = BatchBrain(fly.weights, p, batch=batch, seed=seed, engine=engine, tile=tile)
-    j_in = bb.slots(fly.input_idx)
-    j_dr = bb.slots([fly.drive_idx])
-    dt_s = float(np.float32(p.dt / 1000.0))
-    drive = VisualDrive(fly.input_xy, batch, seed, replicas)
+    if batch % replicas:
+        raise ValueError("batch must be a multiple of replicas")
+    S = batch // replicas
+    own = sim is None
+    if own:
+        sim = _Driven(fly, batch, seed, dn_bias_mv)
+    else:
+        sim.bb.reset()
+    drive = VisualDrive(fly.input_xy, S, seed)
+    seq_of = torch.arange(batch, device=sim.bb.device) // replicas
     pool = GridPool(fly.input_xy)
     readout = DNReadout(fly, batch)
-    bb.p_ext[:, j_dr] = drive_hz * dt_s
-    bb.p_ext[:, j_in] = drive.rates(0) * dt_s
-    autotune.tune(bb, reps=1)
-    X = torch.empty((frames, batch, pool.n), dtype=torch.bfloat16, device=bb.device)
-    Y = torch.empty((frames, batch, 8), dtype=torch.float32, device=bb.device)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
+    X = torch.empty((frames, S, pool.n), dtype=torch.bfloat16, device=sim.bb.device)
+    Y = torch.empty((frames, batch, 8), dtype=torch.float32, device=sim.bb.device)
     for f in range(frames + skip):
         r = drive.rates(f)
-        bb.p_ext[:, j_in] = r * dt_s
-        counts = bb.run(FRAME_MS, copy=False)
-        dn = readout.update(counts, FRAME_MS)
+        sim.set_rates(r[seq_of], drive.drive_hz[seq_of])
+        dn = readout.update(sim.bb.run(FRAME_MS, copy=False), FRAME_MS)
         if f >= skip:
             X[f - skip] = (pool(r) / rate_norm).to(torch.bfloat16)
             Y[f - skip] = dn
-        if log and (f + 1) % 500 == 0:
+        if log and (f + 1) % 1000 == 0:
             log(f"  generate: frame {f + 1}/{frames + skip}")
+    if own:
+        sim.bb.close()
+    return X, Y, drive
+
+
+def sim_frames_per_s(fly, batch: int = 256, frames: int = 20, dn_bias_mv: float = 7.8, seed: int = 5):
+    """Fly-frames per second of the full simulation on the same setup: ``batch`` flies, their
+    rates for every frame staged on the device before the timed region, one graph replay per
+    frame (the brain alone: what a fallback or a surrogate replaces)."""
+    sim = _Driven(fly, batch, seed, dn_bias_mv)
+    drive = VisualDrive(fly.input_xy, batch, seed)
+    staged = [drive.rates(f) * sim.dt_s for f in range(frames + 2)]
+    sim.set_rates(staged[0] / sim.dt_s, drive.drive_hz)
+    for f in range(2):
+        sim.bb.p_ext[:, sim.j_in] = staged[f]
+        sim.bb.run(FRAME_MS, copy=False)
     torch.cuda.synchronize()
-    fps = batch * (frames + skip) / (time.perf_counter() - t0)
-    bb.close()
-    return X, Y, fps
+    t0 = time.perf_counter()
+    for f in range(2, frames + 2):
+        sim.bb.p_ext[:, sim.j_in] = staged[f]
+        sim.bb.run(FRAME_MS, copy=False)
+    torch.cuda.synchronize()
+    fps = batch * frames / (time.perf_counter() - t0)
+    sim.bb.close()
+    return fps, sim.tuned
 
 
 def train_ensemble(X, Y, members=3, T=16, seqs=256, steps=600, lr=1e-3, rate_scale=100.0, seed=0, log=None,
@@ -134,100 +199,187 @@ def train_ensemble(X, Y, members=3, T=16, seqs=256, steps=600, lr=1e-3, rate_sca
             m.adam(lr=lr_t)
             if it % 50 == 0 or it == steps - 1:
                 curve.append((it, float(loss.item())))
-                if log:
+                if log and (it % 1000 == 0 or it == steps - 1):
                     log(f"  member {k} step {it}: loss {curve[-1][1]:.5f}")
         out.append(m)
         curves.append(curve)
     return out, curves
 
 
+class FallbackSim:
+    """Simulates flagged (sequence, frame) pairs for ``surrogate_loop``: a brain at rest runs the
+    sequence's last ``warm`` frames of drive and then the flagged frame, and returns that frame's
+    instantaneous DN group rates (mean count of the group's neurons x 1000 / window).  Brains of
+    several sizes (``sizes`` flies, smallest first) so that a few flagged sequences do not pay
+    for a full batch; the smallest brain that holds them runs, a larger set runs in chunks of the
+    largest.  Every fallback draws a fresh spiking trial."""
+
+    def __init__(self, fly, drive: VisualDrive | None, warm: int = 3, sizes=(4, 32, 256), dn_bias_mv: float = 7.8,
+                 seed: int = 900):
+        from .batch import DNReadout
+
+        self.drive, self.warm = drive, warm
+        self.sims = [_Driven(fly, n, seed + k, dn_bias_mv) for k, n in enumerate(sorted(sizes))]
+        # alpha = 1: the readout kernel returns the frame's instantaneous group rates
+        self.inst = [DNReadout(fly, s.bb.B, tau_ms=1e-9) for s in self.sims]
+        self.frames_simulated = 0
+
+    def run(self, seqs: torch.Tensor, frame: int) -> torch.Tensor:
+        outs = []
+        for chunk in seqs.split(self.sims[-1].bb.B):
+            n = chunk.numel()
+            k = next(i for i, s in enumerate(self.sims) if s.bb.B >= n)
+            sim, inst = self.sims[k], self.inst[k]
+            bb = sim.bb
+            bb.reset()
+            bb.alive.zero_()
+            bb.alive[:n] = 1
+            rows = torch.arange(n, device=bb.device)
+            counts = None
+            for f in range(max(0, frame - self.warm), frame + 1):
+                sim.set_rates(self.drive.rates(f, chunk), self.drive.drive_hz[chunk], rows)
+                counts = bb.run(FRAME_MS, copy=False)
+                self.frames_simulated += n
+            inst.rates.zero_()
+            outs.append(inst.update(counts, FRAME_MS)[:n].clone())
+        return torch.cat(outs)
+
+
+def surrogate_loop(ens, X: torch.Tensor, threshold: float, fallback: FallbackSim | None, tau_ms: float = 80.0):
+    """Runs the ensemble over X [F, S, n_in] frame by frame; frames whose disagreement exceeds
+    ``threshold`` are simulated by ``fallback`` (their simulated counts enter the readout's
+    exponential average: r = r_prev + a (inst - r_prev)).  Returns (rates [F, S, 8] with fallback,
+    the network's own rates [F, S, 8], fallback mask [F, S])."""
+    F, S, _ = X.shape
+    a = float(1.0 - math.exp(-FRAME_MS / tau_ms))
+    ens.reset(S)
+    out = torch.empty((F, S, 8), dtype=torch.float32, device=X.device)
+    net = torch.empty_like(out)
+    flags = torch.zeros((F, S), dtype=torch.bool, device=X.device)
+    prev = None
+    for f in range(F):
+        pred, spread = ens.predict(X[f])
+        net[f] = pred
+        y = pred
+        if fallback is not None:
+            flag = spread > threshold
+            idx = torch.nonzero(flag).squeeze(1)           # the host needs the count to launch
+            if idx.numel():
+                base = prev[idx] if prev is not None else pred[idx]
+                y = pred.clone()
+                y[idx] = base + a * (fallback.run(idx, f) - base)
+                flags[f] = flag
+        out[f] = y
+        prev = out[f]
+    return out, net, flags
+
+
 def generate_judged(fly, frames: int, sequences: int, replicas: int, batch: int = 256, seed: int = 77, log=None,
                     **kw):
-    """Held-out data: ``sequences`` input sequences, each simulated by ``replicas`` flies with
-    independent spiking noise (``batch // replicas`` sequences per batched run).  Returns the
-    shared inputs X [F, S, 2048], the replicas' mean rate (the simulation's expected rate, the
-    judge), one trial, the mean of the other trials, and sim frames/s."""
+    """Held-out data: ``sequences`` sequences, each simulated by ``replicas`` flies.  Returns X
+    [F, S, 2048], the replicas' mean rate (the judge), one trial, the mean of the other trials,
+    and each run's ``VisualDrive`` with the range of sequences it holds."""
     per = batch // replicas
     if per < 1 or batch % replicas:
         raise ValueError("batch must be a multiple of replicas")
-    xs, mean, trial, others, fps = [], [], [], [], []
-    done = 0
-    k = 0
+    sim = _Driven(fly, batch, seed, kw.get("dn_bias_mv", 7.8))
+    xs, mean, trial, others, drives = [], [], [], [], []
+    done, k = 0, 0
     while done < sequences:
-        X, Y, f = generate(fly, frames, batch, seed=seed + 1000 * k, replicas=replicas, **kw)
+        X, Y, drive = generate(fly, frames, batch, seed=seed + 1000 * k, replicas=replicas, sim=sim, **kw)
         Ys = Y.view(frames, per, replicas, 8)
-        xs.append(X.view(frames, per, replicas, -1)[:, :, 0].clone())
+        xs.append(X)
         mean.append(Ys.mean(2))
         trial.append(Ys[:, :, 0].clone())
         others.append((Ys.sum(2) - Ys[:, :, 0]) / max(replicas - 1, 1))
-        fps.append(f)
-        del X, Y, Ys
+        drives.append((done, drive))
+        del Y, Ys
         done += per
         k += 1
         if log:
             log(f"  held-out: {done}/{sequences} sequences")
+    sim.bb.close()
     cat = lambda a: torch.cat(a, 1)[:, :sequences].contiguous()  # noqa: E731
-    return cat(xs), cat(mean), cat(trial), cat(others), float(np.mean(fps))
+    return cat(xs), cat(mean), cat(trial), cat(others), drives
 
 
-def judge(ens, X, ref, trial, others, replicas: int, threshold: float = 0.15, const=None):
-    """X [F, S, n_in] inputs, ref [F, S, 8] the simulation's expected rate (replica mean),
-    trial one simulation trial, others the mean of the remaining replicas.  Errors are
-    mean |a - ref| / mean |ref| over frames, sequences and the 8 groups."""
-    F, S, n_in = X.shape
-    ens.reset(S)
-    pred = torch.empty_like(ref)
-    spread = torch.empty((F, S), dtype=torch.float32, device=X.device)
-    for f in range(F):
-        pred[f], spread[f] = ens.predict(X[f].contiguous())
-    fb = spread > threshold
-    mixed = torch.where(fb[..., None], trial, pred)
+def judge(pred, ref, trial, others, replicas: int, loop=None, flags=None, const=None):
+    """Errors are mean |a - ref| / mean |ref| over frames, sequences and the 8 groups; ref is the
+    simulation's expected rate (replica mean)."""
     denom = ref.abs().mean()
     err = lambda a, b: float((a - b).abs().mean() / denom)  # noqa: E731
     noise = err(trial, others)
+    F, S, _ = ref.shape
     res = {"frames": F * S, "sequences": S, "frames_per_sequence": F, "replicas": replicas,
-           "mean_rate_hz": float(ref.mean()), "err_surrogate": err(pred, ref), "err_with_fallback": err(mixed, ref),
-           "fallback_frac": float(fb.float().mean()), "err_single_sim_trial": err(trial, ref),
+           "mean_rate_hz": float(ref.mean()), "rate_hz_per_group": [float(ref[..., g].mean()) for g in range(8)],
+           "err_surrogate": err(pred, ref), "err_single_sim_trial": err(trial, ref),
            "err_sim_trial_vs_other_trials": noise,
            # a trial's deviation from the mean of R-1 others is ~sqrt(R/(R-1)) x its deviation from
            # the expected rate; the judge (mean of R) deviates ~1/sqrt(R) of that
            "judge_noise_est": noise * math.sqrt((replicas - 1) / replicas) / math.sqrt(replicas),
            "err_per_group": [float((pred[..., g] - ref[..., g]).abs().mean() / ref[..., g].abs().mean().clamp_min(1e-6))
-                             for g in range(ref.shape[-1])],
-           "threshold": threshold}
+                             for g in range(ref.shape[-1])]}
+    if loop is not None:
+        res["err_with_fallback"] = err(loop, ref)
+        res["fallback_frac"] = float(flags.float().mean())
     if const is not None:
         res["err_constant_predictor"] = err(const.expand_as(ref), ref)
     return res
 
 
 def inference_fps(ens, batch: int = 4096, frames: int = 20):
+    """Ensemble frames/s at ``batch`` sequences, timed with HIP events on the current stream."""
     n_in = ens.members[0].n_in
     x = torch.randn((batch, n_in), device=ens.members[0].dev).to(torch.bfloat16)
     ens.reset(batch)
     for _ in range(3):
         ens.predict(x)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
     for _ in range(frames):
         ens.predict(x)
+    e1.record()
+    e1.synchronize()
+    return batch * frames / (e0.elapsed_time(e1) / 1000.0)
+
+
+def timed_loop(fly, ens, threshold: float, warm: int, sequences: int = 4096, frames: int = 30, seed: int = 5151,
+               rate_norm: float = 200.0, fallback: FallbackSim | None = None):
+    """Frames/s of the surrogate with its real fallback: ``sequences`` fresh sequences, their pooled
+    inputs staged on the device first, then ``frames`` frames of ``surrogate_loop`` timed end to
+    end (ensemble, disagreement, fallback simulations and the readout update)."""
+    drive = VisualDrive(fly.input_xy, sequences, seed)
+    pool = GridPool(fly.input_xy)
+    X = torch.stack([(pool(drive.rates(f)) / rate_norm).to(torch.bfloat16) for f in range(frames)])
+    fb = fallback if fallback is not None else FallbackSim(fly, drive, warm=warm)
+    fb.drive, fb.warm = drive, warm
+    surrogate_loop(ens, X[:2], threshold, fb)        # warm-up: graph capture of both fallback brains
+    fb.frames_simulated = 0
     torch.cuda.synchronize()
-    return batch * frames / (time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    _, _, flags = surrogate_loop(ens, X, threshold, fb)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return {"sequences": sequences, "frames": frames, "frames_per_s": sequences * frames / wall,
+            "fallback_frac": float(flags.float().mean()), "fallback_sim_fly_frames": fb.frames_simulated}
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--train-frames", type=int, default=23400, help="frames per fly (x --batch flies)")
+    ap.add_argument("--train-frames", type=int, default=8000, help="frames per fly (x --batch flies)")
     ap.add_argument("--batch", type=int, default=256)
-    ap.add_argument("--test-frames", type=int, default=150, help="frames per held-out sequence")
-    ap.add_argument("--test-sequences", type=int, default=128)
-    ap.add_argument("--replicas", type=int, default=128, help="simulated flies per held-out sequence")
-    ap.add_argument("--train-replicas", type=int, default=8,
-                    help="flies per training sequence; targets are their mean rate (same simulated frames)")
+    ap.add_argument("--train-replicas", type=int, default=4,
+                    help="flies per training sequence; targets are their mean rate")
+    ap.add_argument("--test-frames", type=int, default=200, help="frames per held-out sequence")
+    ap.add_argument("--test-sequences", type=int, default=250)
+    ap.add_argument("--replicas", type=int, default=32, help="simulated flies per held-out sequence")
     ap.add_argument("--members", type=int, default=3)
-    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--steps", type=int, default=12000)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--threshold", type=float, default=0.15)
-    ap.add_argument("--rate-scale", type=float, default=2.0, help="Hz per unit of the network's output")
+    ap.add_argument("--warm", type=int, default=3, help="frames of drive a fallback re-warms its brain on")
+    ap.add_argument("--rate-scale", type=float, default=20.0, help="Hz per unit of the network's output")
+    ap.add_argument("--dn-bias", type=float, default=7.8, help="mV on the motor DNs (the fleet's)")
     ap.add_argument("--n", type=int, default=0, help="neurons (0: the full 138,639)")
     ap.add_argument("--inference-only", action="store_true",
                     help="time ensemble inference at B = 4,096 with freshly initialised members (for profiling)")
@@ -247,14 +399,15 @@ def main(argv=None):
     from .surrogate import Ensemble
 
     fly = flywire_like() if not args.n else flywire_like(n=args.n, n_edges=args.n * 109, seed=11)
+    sim_fps, sim_shape = sim_frames_per_s(fly, args.batch, dn_bias_mv=args.dn_bias)
+    log(f"full simulation: {sim_fps:.0f} fly-frames/s at B = {args.batch} ({sim_shape})")
     t0 = time.perf_counter()
-    X, Y, sim_fps = generate(fly, args.train_frames, args.batch, seed=1, replicas=args.train_replicas, log=log)
-    if args.train_replicas > 1:  # one input per sequence, the replicas' mean rate as its target
-        R, F = args.train_replicas, X.shape[0]
-        X = X.view(F, -1, R, X.shape[-1])[:, :, 0].contiguous()
-        Y = Y.view(F, -1, R, 8).mean(2).contiguous()
-        torch.cuda.empty_cache()
-    log(f"training data: {tuple(X.shape)} in {time.perf_counter() - t0:.1f}s ({sim_fps:.0f} sim frames/s); "
+    X, Y, _ = generate(fly, args.train_frames, args.batch, seed=1, replicas=args.train_replicas,
+                       dn_bias_mv=args.dn_bias, log=log)
+    F, R = X.shape[0], args.train_replicas
+    Y = Y.view(F, -1, R, 8).mean(2).contiguous()          # one input per sequence, the replicas' mean
+    torch.cuda.empty_cache()
+    log(f"training data: {tuple(X.shape)} from {F * args.batch} simulated frames in {time.perf_counter() - t0:.1f}s; "
         f"DN rate mean {float(Y.mean()):.2f} Hz, sd {float(Y.std()):.2f}")
     t0 = time.perf_counter()
     members, curves = train_ensemble(X, Y, members=args.members, steps=args.steps, lr=args.lr, log=log,
@@ -263,20 +416,36 @@ def main(argv=None):
     const = Y.mean((0, 1))
     del X, Y
     torch.cuda.empty_cache()
-    Xt, ref, trial, others, _ = generate_judged(fly, args.test_frames, args.test_sequences, args.replicas,
-                                                args.batch, log=log)
     ens = Ensemble(members)
-    res = judge(ens, Xt, ref, trial, others, args.replicas, args.threshold, const)
+
+    # held-out sequences, each simulated by R replica flies; the surrogate runs on them with its
+    # real fallback (every flagged frame re-simulated)
+    Xt, ref, trial, others, drives = generate_judged(fly, args.test_frames, args.test_sequences, args.replicas,
+                                                     args.batch, log=log, dn_bias_mv=args.dn_bias)
+    loops, nets, flags = [], [], []
+    fb = FallbackSim(fly, None, warm=args.warm, dn_bias_mv=args.dn_bias)
+    for start, drive in drives:
+        S = min(drive.S, args.test_sequences - start)
+        fb.drive = drive
+        lo, ne, fl = surrogate_loop(ens, Xt[:, start:start + S].contiguous(), args.threshold, fb)
+        loops.append(lo)
+        nets.append(ne)
+        flags.append(fl)
+    fb.drive = None
+    loop, net, flag = torch.cat(loops, 1), torch.cat(nets, 1), torch.cat(flags, 1)
+    res = judge(net, ref, trial, others, args.replicas, loop=loop, flags=flag, const=const)
     fps = inference_fps(ens)
-    f = res["fallback_frac"]
-    # every frame runs the ensemble; a fallback frame also costs one simulated frame
-    eff = 1.0 / (1.0 / fps + f / sim_fps)
-    res.update({"train_frames": args.train_frames * args.batch, "train_replicas": args.train_replicas,
-                "train_steps": args.steps, "rate_scale": args.rate_scale,
+    tl = timed_loop(fly, ens, args.threshold, args.warm, fallback=fb)
+    res.update({"train_sim_frames": args.train_frames * args.batch, "train_replicas": args.train_replicas,
+                "train_steps": args.steps, "rate_scale": args.rate_scale, "dn_bias_mv": args.dn_bias,
+                "threshold": args.threshold, "warm_frames": args.warm,
                 "train_s": round(train_s, 1), "members": args.members,
-                "loss_curves": [c[::6] + [c[-1]] for c in curves], "surrogate_frames_per_s": round(fps),
-                "sim_frames_per_s": round(sim_fps), "speedup_vs_sim": round(fps / sim_fps, 1),
-                "frames_per_s_with_fallback": round(eff), "speedup_with_fallback": round(eff / sim_fps, 1)})
+                "loss_curves": [c[::24] + [c[-1]] for c in curves],
+                "surrogate_frames_per_s": round(fps), "sim_frames_per_s": round(sim_fps),
+                "speedup_vs_sim": round(fps / sim_fps, 1),
+                "timed_loop": {k: (round(v, 5) if isinstance(v, float) else v) for k, v in tl.items()},
+                "frames_per_s_with_fallback": round(tl["frames_per_s"]),
+                "speedup_with_fallback": round(tl["frames_per_s"] / sim_fps, 1)})
     print(json.dumps(res), flush=True)
     return res
 

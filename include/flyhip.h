@@ -314,6 +314,7 @@ int fz_eye_step(const fz_eye_args* a, void* stream);
 #define FZ_SG_ACC32 1   /* C32 += A B (+ bias) */
 #define FZ_SG_STORE16 2 /* C16 = bf16(A B (+ bias)) */
 #define FZ_SG_GELU 3    /* C32 = A B + bias (pre-activation), C16 = bf16(gelu(C32)) */
+#define FZ_SG_GELU16 4  /* C16 = bf16(gelu(A B + bias)) (inference: no pre-activation kept) */
 
 /* C[M][N] = sum_k A[m][k] B[k][n]; A, B bf16 (raw uint16 bit patterns), FP32 accumulation on
  * MFMA.  trans_a: A is stored [K][M] (lda = row stride); trans_b: B is stored [N][K].
@@ -321,6 +322,21 @@ int fz_eye_step(const fz_eye_args* a, void* stream);
  * nearest ancestor is the PyTorch MLP of flyzero/implant.py:136-154). */
 int fz_sg_gemm(int M, int N, int K, const void* A, int lda, int trans_a, const void* B, int ldb, int trans_b,
                int mode, float* C32, void* C16, int ldc, const float* bias, void* stream);
+/* `count` products of one shape in one launch (the ensemble's members): product z reads A + z
+ * stride_a, B + z stride_b, bias + z stride_bias and writes C + z stride_c (element strides; 0
+ * shares an operand, e.g. the input every member reads). */
+int fz_sg_gemm_batched(int count, int M, int N, int K, const void* A, int lda, long long stride_a, int trans_a,
+                       const void* B, int ldb, long long stride_b, int trans_b, int mode, float* C32, void* C16,
+                       int ldc, long long stride_c, const float* bias, long long stride_bias, void* stream);
+/* Output layer of `members` ensemble members and the ensemble's statistics for B frames:
+ * y_k = ([h2_k, s_k] Wo_k^T + bo_k) * rate_scale (h2 bf16 [members][B][hid], s bf16
+ * [members][B][H]; member k's bf16 weights at w16 + k wstride, Wo at offset wo_off as
+ * [n_pad][hid + H], its FP32 bias at w32 + k wstride + bo_off), mean[B][n_out] over members, and
+ * spread[B] = ||population std|| / (||mean|| + 1 Hz), the disagreement that sends a frame to the
+ * full simulation.  members <= 8, n_out <= 8. */
+int fz_sg_ens_head(int B, int members, int hid, int H, int n_out, const void* h2, const void* s, const void* w16,
+                   const float* w32, long long wstride, long long wo_off, long long bo_off, float rate_scale,
+                   float* mean, float* spread, void* stream);
 /* dpre = dact * gelu'(pre) (tanh-approximated GELU), as FP32 and/or BF16. */
 int fz_sg_gelu_bwd(long long n, const float* dact, const float* pre, float* dpre32, void* dpre16, void* stream);
 /* db[n] (+)= sum_m d[m][n] over a row-major [M][N] array. */

@@ -124,6 +124,30 @@ def test_forward_backward_matches_oracle_with_bf16_points():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("hid,H,B", [(64, 32, 40), (1024, 256, 4096)])
+def test_member_batched_inference_equals_member_by_member(hid, H, B):
+    # Ensemble.predict (member-batched GEMMs, one GRU launch, the fused output layer and
+    # statistics) against each member's own forward, over 4 frames of a recurrent state
+    import torch
+
+    from flyhip.surrogate import Ensemble, RateSurrogate
+
+    n_in = 2048 if hid == 1024 else 64
+    members = [RateSurrogate(n_in, hid, H, 8, seed=40 + k, rate_scale=20.0) for k in range(3)]
+    fast, slow = Ensemble(members), Ensemble(members)
+    fast.reset(B)
+    slow.reset(B)
+    rng = np.random.default_rng(5)
+    for f in range(4):
+        x = _bf(bf16(rng.random((B, n_in)) * 2.0))
+        m1, s1 = fast.predict(x)
+        m2, s2 = slow.predict_members(x)
+        np.testing.assert_allclose(m1.cpu().numpy(), m2.cpu().numpy(), rtol=1e-4, atol=1e-3)
+        np.testing.assert_allclose(s1.cpu().numpy(), s2.cpu().numpy(), rtol=1e-3, atol=1e-5)
+        assert float(s1.max()) > 0
+
+
+@pytest.mark.gpu
 def test_training_reduces_loss_on_a_recurrent_target():
     import torch
 
@@ -162,22 +186,67 @@ def test_grid_pool_is_the_cell_mean():
 
 @pytest.mark.gpu
 def test_surrogate_pipeline_end_to_end_small():
+    import torch
+
     from flyhip.connectome import flywire_like
     from flyhip.surrogate import Ensemble
-    from flyhip.surrogate_pipeline import generate, generate_judged, judge, train_ensemble
+    from flyhip.surrogate_pipeline import (FallbackSim, generate, generate_judged, judge, surrogate_loop, timed_loop,
+                                           train_ensemble)
 
     fly = flywire_like(n=4000, n_edges=4000 * 60, seed=11)
-    X, Y, fps = generate(fly, frames=40, batch=16, seed=1, skip=2)
-    assert X.shape == (40, 16, 2048) and Y.shape == (40, 16, 8) and fps > 0
+    X, Y, drive = generate(fly, frames=40, batch=16, seed=1, skip=2)
+    assert X.shape == (40, 16, 2048) and Y.shape == (40, 16, 8) and drive.S == 16
     assert np.isfinite(Y.cpu().numpy()).all() and float(Y.sum()) > 0
-    members, curves = train_ensemble(X, Y, members=2, T=8, seqs=16, steps=60, lr=2e-3)
+    members, curves = train_ensemble(X, Y, members=2, T=8, seqs=16, steps=60, lr=2e-3, rate_scale=20.0)
     assert curves[0][-1][1] < curves[0][0][1]
-    Xt, ref, trial, others, _ = generate_judged(fly, frames=20, sequences=6, replicas=4, batch=16, skip=2)
+    Xt, ref, trial, others, drives = generate_judged(fly, frames=20, sequences=6, replicas=4, batch=16, skip=2)
     assert Xt.shape == (20, 6, 2048) and ref.shape == (20, 6, 8)
-    res = judge(Ensemble(members), Xt, ref, trial, others, replicas=4, const=Y.mean((0, 1)))
+    ens = Ensemble(members)
+    fb = FallbackSim(fly, None, warm=2, sizes=(4, 16))
+    loops, nets, flags = [], [], []
+    for start, d in drives:
+        S = min(d.S, 6 - start)
+        fb.drive = d
+        lo, ne, fl = surrogate_loop(ens, Xt[:, start:start + S].contiguous(), 0.0, fb)   # every frame falls back
+        loops.append(lo)
+        nets.append(ne)
+        flags.append(fl)
+    loop, net, flag = torch.cat(loops, 1), torch.cat(nets, 1), torch.cat(flags, 1)
+    res = judge(net, ref, trial, others, 4, loop=loop, flags=flag, const=Y.mean((0, 1)))
     for k in ("err_surrogate", "err_with_fallback", "fallback_frac", "err_single_sim_trial", "err_constant_predictor"):
         assert np.isfinite(res[k]), (k, res)
-    assert 0.0 <= res["fallback_frac"] <= 1.0
+    assert res["fallback_frac"] == 1.0 and not torch.equal(loop, net)
+    tl = timed_loop(fly, ens, 0.15, 2, sequences=32, frames=3, fallback=fb)
+    assert tl["frames_per_s"] > 0 and 0.0 <= tl["fallback_frac"] <= 1.0
+
+
+@pytest.mark.gpu
+def test_fallback_is_a_brain_at_rest_warmed_on_the_sequence():
+    # FallbackSim.run(seqs, f): each flagged sequence's DN group rates of frame f from a brain at
+    # rest that ran the sequence's frames f - warm .. f; restated here with a brain of its own
+    import torch
+
+    from flyhip.batch import DNReadout
+    from flyhip.connectome import flywire_like
+    from flyhip.surrogate_pipeline import FRAME_MS, FallbackSim, VisualDrive, _Driven
+
+    fly = flywire_like(n=4000, n_edges=4000 * 60, seed=11)
+    drive = VisualDrive(fly.input_xy, 10, seed=3)
+    fb = FallbackSim(fly, drive, warm=3, sizes=(4, 8), seed=900)
+    seqs = torch.tensor([7, 2, 5], device="cuda")
+    got = fb.run(seqs, 6)                                   # 3 flagged: the 4-fly brain (seed 900)
+    ref = _Driven(fly, 4, 900, 7.8, tune=False)
+    rd = DNReadout(fly, 4, tau_ms=1e-9)
+    rows = torch.arange(3, device="cuda")
+    ref.bb.alive[3:] = 0
+    for f in range(3, 7):
+        ref.set_rates(drive.rates(f, seqs), drive.drive_hz[seqs], rows)
+        counts = ref.bb.run(FRAME_MS, copy=False)
+    want = rd.update(counts, FRAME_MS)[:3]
+    np.testing.assert_array_equal(got.cpu().numpy(), want.cpu().numpy())
+    assert float(want.sum()) > 0
+    many = fb.run(torch.arange(10, device="cuda").repeat(2), 6)    # 20 > 8: chunks on the large brain
+    assert many.shape == (20, 8) and np.isfinite(many.cpu().numpy()).all()
 
 
 @pytest.mark.gpu
@@ -187,15 +256,15 @@ def test_trained_surrogate_beats_the_method_without_it():
     # training mean rate), the method without the network
     from flyhip.connectome import flywire_like
     from flyhip.surrogate import Ensemble
-    from flyhip.surrogate_pipeline import generate, generate_judged, judge, train_ensemble
+    from flyhip.surrogate_pipeline import generate, generate_judged, judge, surrogate_loop, train_ensemble
 
     fly = flywire_like(n=8000, n_edges=8000 * 109, seed=11)
     X, Y, _ = generate(fly, frames=400, batch=64, seed=1, replicas=4, skip=5)
-    X = X.view(400, 16, 4, -1)[:, :, 0].contiguous()
     Y = Y.view(400, 16, 4, 8).mean(2).contiguous()
-    members, _ = train_ensemble(X, Y, members=2, T=16, seqs=64, steps=500, lr=1e-3, rate_scale=2.0)
+    members, _ = train_ensemble(X, Y, members=2, T=16, seqs=64, steps=500, lr=1e-3, rate_scale=20.0)
     Xt, ref, trial, others, _ = generate_judged(fly, frames=60, sequences=8, replicas=8, batch=64, seed=9, skip=5)
-    res = judge(Ensemble(members), Xt, ref, trial, others, replicas=8, const=Y.mean((0, 1)))
+    _, net, _ = surrogate_loop(Ensemble(members), Xt, 0.15, None)
+    res = judge(net, ref, trial, others, 8, const=Y.mean((0, 1)))
     print(res)
     assert res["mean_rate_hz"] > 0
     assert res["err_surrogate"] < 0.5 * res["err_constant_predictor"], res
