@@ -249,17 +249,20 @@ def surrogate_loop(ens, X: torch.Tensor, threshold: float, fallback: FallbackSim
     """Runs the ensemble over X [F, S, n_in] frame by frame; frames whose disagreement exceeds
     ``threshold`` are simulated by ``fallback`` (their simulated counts enter the readout's
     exponential average: r = r_prev + a (inst - r_prev)).  Returns (rates [F, S, 8] with fallback,
-    the network's own rates [F, S, 8], fallback mask [F, S])."""
+    the network's own rates [F, S, 8], fallback mask [F, S]); the last call's disagreements [F, S]
+    stay in ``surrogate_loop.spread``."""
     F, S, _ = X.shape
     a = float(1.0 - math.exp(-FRAME_MS / tau_ms))
     ens.reset(S)
     out = torch.empty((F, S, 8), dtype=torch.float32, device=X.device)
     net = torch.empty_like(out)
     flags = torch.zeros((F, S), dtype=torch.bool, device=X.device)
+    spreads = torch.empty((F, S), dtype=torch.float32, device=X.device)
     prev = None
     for f in range(F):
         pred, spread = ens.predict(X[f])
         net[f] = pred
+        spreads[f] = spread
         y = pred
         if fallback is not None:
             flag = spread > threshold
@@ -271,6 +274,7 @@ def surrogate_loop(ens, X: torch.Tensor, threshold: float, fallback: FallbackSim
                 flags[f] = flag
         out[f] = y
         prev = out[f]
+    surrogate_loop.spread = spreads
     return out, net, flags
 
 
@@ -378,6 +382,7 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--threshold", type=float, default=0.15)
     ap.add_argument("--warm", type=int, default=3, help="frames of drive a fallback re-warms its brain on")
+    ap.add_argument("--sweep", default="0.03,0.02", help="further disagreement thresholds to run the loop at")
     ap.add_argument("--rate-scale", type=float, default=20.0, help="Hz per unit of the network's output")
     ap.add_argument("--dn-bias", type=float, default=7.8, help="mV on the motor DNs (the fleet's)")
     ap.add_argument("--n", type=int, default=0, help="neurons (0: the full 138,639)")
@@ -422,20 +427,40 @@ def main(argv=None):
     # real fallback (every flagged frame re-simulated)
     Xt, ref, trial, others, drives = generate_judged(fly, args.test_frames, args.test_sequences, args.replicas,
                                                      args.batch, log=log, dn_bias_mv=args.dn_bias)
-    loops, nets, flags = [], [], []
     fb = FallbackSim(fly, None, warm=args.warm, dn_bias_mv=args.dn_bias)
-    for start, drive in drives:
-        S = min(drive.S, args.test_sequences - start)
-        fb.drive = drive
-        lo, ne, fl = surrogate_loop(ens, Xt[:, start:start + S].contiguous(), args.threshold, fb)
-        loops.append(lo)
-        nets.append(ne)
-        flags.append(fl)
-    fb.drive = None
-    loop, net, flag = torch.cat(loops, 1), torch.cat(nets, 1), torch.cat(flags, 1)
+
+    def judged_loop(threshold):
+        loops, nets, flags, spreads = [], [], [], []
+        for start, drive in drives:
+            S = min(drive.S, args.test_sequences - start)
+            fb.drive = drive
+            lo, ne, fl = surrogate_loop(ens, Xt[:, start:start + S].contiguous(), threshold, fb)
+            loops.append(lo)
+            nets.append(ne)
+            flags.append(fl)
+            spreads.append(surrogate_loop.spread)
+        fb.drive = None
+        return torch.cat(loops, 1), torch.cat(nets, 1), torch.cat(flags, 1), torch.cat(spreads, 1)
+
+    loop, net, flag, spread = judged_loop(args.threshold)
     res = judge(net, ref, trial, others, args.replicas, loop=loop, flags=flag, const=const)
+    q = torch.quantile(spread.flatten()[:1_000_000], torch.tensor([0.5, 0.9, 0.99, 0.999, 1.0], device=spread.device))
+    res["disagreement_quantiles"] = {k: round(float(v), 5) for k, v in zip(("p50", "p90", "p99", "p999", "max"), q)}
     fps = inference_fps(ens)
     tl = timed_loop(fly, ens, args.threshold, args.warm, fallback=fb)
+    # the same loop with lower thresholds, so that the fallback runs on a measurable share of frames
+    sweep = []
+    for t in [float(x) for x in args.sweep.split(",") if x]:
+        lo, _, fl, _ = judged_loop(t)
+        tt = timed_loop(fly, ens, t, args.warm, fallback=fb)
+        sweep.append({"threshold": t, "err_with_fallback": judge(net, ref, trial, others, args.replicas, loop=lo,
+                                                                  flags=fl)["err_with_fallback"],
+                      "fallback_frac": float(fl.float().mean()), "timed_fallback_frac": round(tt["fallback_frac"], 5),
+                      "frames_per_s_with_fallback": round(tt["frames_per_s"]),
+                      "speedup_with_fallback": round(tt["frames_per_s"] / sim_fps, 1),
+                      "fallback_sim_fly_frames": tt["fallback_sim_fly_frames"]})
+        log(f"  threshold {t}: {sweep[-1]}")
+    res["threshold_sweep"] = sweep
     res.update({"train_sim_frames": args.train_frames * args.batch, "train_replicas": args.train_replicas,
                 "train_steps": args.steps, "rate_scale": args.rate_scale, "dn_bias_mv": args.dn_bias,
                 "threshold": args.threshold, "warm_frames": args.warm,
