@@ -83,6 +83,46 @@ def measured_traffic(spec: str):
     return None
 
 
+BYTE_MODEL = {
+    "tile": "per K-step block and fly: u, g read + write (16 B) and ref (2 B) per neuron once per block, 8 B per "
+            "ext slot (probability, bias), counts 4 B per neuron once per window; per fired row 8 B of row "
+            "pointers, 4 B per fixed edge (packed value and target) and 8 B of spike list",
+    "update": "per K-step block and fly: u, g read + write and ref (18 B) and K ring slots read (4 B each) per "
+              "neuron, 8 B per ext slot, counts once per window",
+    "push": "per fired row: 8 B of row pointers, 10 B per edge (int32 column, int16 count, int32 atomic)",
+}
+
+
+def algorithmic_bytes(engine, B, n, n_in, K, steps, frames, spikes, edges):
+    """HBM bytes the path needs for ``frames`` windows of ``steps`` steps (B flies, measured spike
+    and fired-edge totals), per kernel: the temporal-blocking model the engines implement (state
+    once per K-step block) and, for reference, the per-step model (26 B per neuron-step)."""
+    blocks = frames * -(-steps // K)
+    per_block = B * (18.0 * n + 8.0 * n_in)
+    counts = frames * B * 4.0 * n
+    rows = spikes * (8.0 + 8.0) + edges * 4.0
+    per_step = frames * steps * B * (26.0 * n + 4.0 * n_in) + spikes * 8.0 + edges * 10.0
+    return {"tile": blocks * per_block + counts + rows, "tile_per_step_model": per_step,
+            "update": blocks * (per_block + B * 4.0 * K * n) + counts,
+            "update_per_step_model": frames * steps * B * (26.0 * n + 4.0 * n_in),
+            "push": spikes * 8.0 + edges * 10.0, "push_per_step_model": spikes * 8.0 + edges * 10.0}
+
+
+def profile_record(tag: str, spec):
+    """A '<tag>: {json}' line of the lif_tile profile summary for a launch shape, or None."""
+    if not spec:
+        return None
+    try:
+        for line in open(PROFILE):
+            if line.startswith(tag + ":"):
+                rec = json.loads(line.split(":", 1)[1])
+                if _same_shape(rec.get("spec", ""), spec):
+                    return rec
+    except OSError:
+        pass
+    return None
+
+
 def shape_spec(engine, tile, B, shape):
     if engine != "tile":
         return None
@@ -153,38 +193,55 @@ def measure_line(fly, p, B, engine, tile, frames, warmup, torch, gain=None, driv
     if kernel_frames:
         deg = np.diff(bb.indptr).astype(np.float64)
         tim = {"update_ms": 0.0, "update_launches": 0, "push_ms": 0.0, "push_launches": 0}
-        spikes = push_bytes = 0.0
+        spikes = edges = 0.0
         for f in range(kernel_frames):
             bb.p_ext[:, j] = probs[f % (warmup + frames)]
             r = bb.run_timed(steps)
             c = bb.counts.to(torch.float64).sum(0).cpu().numpy()
-            push_bytes += float((c * (8.0 + 10.0 * deg)).sum())
+            edges += float((c * deg).sum())
             spikes += float(c.sum())
             for k in tim:
                 tim[k] += r[k]
-        n_in = len(idx)
-        upd_bytes = kernel_frames * steps * B * (26.0 * bb.n + 4.0 * n_in)
-        if engine == "tile":   # one kernel does the whole step
-            dom, nbytes, ms, launches = "lif_tile", upd_bytes + push_bytes, tim["update_ms"], tim["update_launches"]
+        K = shape.get("block_steps") or bb.n_delay
+        model = algorithmic_bytes(engine, B, bb.n, len(idx), K, steps, kernel_frames, spikes, edges)
+        if engine == "tile":   # one kernel does the whole block
+            dom, ms, launches, key = "lif_tile", tim["update_ms"], tim["update_launches"], "tile"
         elif tim["update_ms"] >= tim["push_ms"]:
-            dom, nbytes, ms, launches = "lif_update", upd_bytes, tim["update_ms"], tim["update_launches"]
+            dom, ms, launches, key = "lif_update", tim["update_ms"], tim["update_launches"], "update"
         else:
-            dom, nbytes, ms, launches = "lif_push", push_bytes, tim["push_ms"], tim["push_launches"]
-        avg_ms = ms / max(launches, 1)
-        bpl = nbytes / max(launches, 1)
-        achieved = bpl / (avg_ms / 1000.0) / 1e9
-        tr = measured_traffic(shape_spec(engine, tile, B, shape))
-        res["roofline"] = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": round(achieved / HBM_PEAK_GBS, 4),
-                           "traffic": tr["bytes_per_launch"] if tr else None,
-                           "traffic_gbs": round(tr["bytes_per_launch"] / (avg_ms / 1000.0) / 1e9, 1) if tr else None,
-                           "traffic_source": (f"rocprofv3 FETCH_SIZE x {tr['fetch_factor']} + WRITE_SIZE x "
-                                              f"{tr['write_factor']} (copy-kernel scaled), {PROFILE[len(REPO) + 1:]}")
-                           if tr else "no counter run for this launch shape", "kernel": dom,
-                           "avg_launch_us": round(avg_ms * 1000.0, 3), "bytes_per_launch": round(bpl),
-                           "launches": launches,
-                           "kernel_ms_per_frame": round((tim["update_ms"] + tim["push_ms"]) / kernel_frames, 4),
-                           "spikes_per_step_per_fly": round(spikes / (kernel_frames * steps * B), 1)}
+            dom, ms, launches, key = "lif_push", tim["push_ms"], tim["push_launches"], "push"
+        avg_s = ms / max(launches, 1) / 1000.0
+        bpl = model[key] / max(launches, 1)
+        step_bpl = model[key + "_per_step_model"] / max(launches, 1)
+        achieved = bpl / avg_s / 1e9
+        spec = shape_spec(engine, tile, B, shape)
+        tr = measured_traffic(spec)
+        valu = profile_record("valu-json", spec)
+        roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": tr["bytes_per_launch"] if tr else None,
+                "traffic_gbs": round(tr["bytes_per_launch"] / avg_s / 1e9, 1) if tr else None,
+                "traffic_frac": round(tr["bytes_per_launch"] / avg_s / 1e9 / HBM_PEAK_GBS, 4) if tr else None,
+                "traffic_source": (f"rocprofv3 FETCH_SIZE x {tr['fetch_factor']} + WRITE_SIZE x "
+                                   f"{tr['write_factor']} (copy-kernel scaled), {PROFILE[len(REPO) + 1:]}")
+                if tr else "no counter run for this launch shape", "kernel": dom,
+                "avg_launch_us": round(avg_s * 1e6, 3), "bytes_per_launch": round(bpl), "launches": launches,
+                "byte_model": BYTE_MODEL[key],
+                "per_step_model": {"bytes_per_launch": round(step_bpl),
+                                   "achieved": round(step_bpl / avg_s / 1e9, 2),
+                                   "note": "state charged every step (26 B per neuron-step, 8 + 10 B per edge of "
+                                           "a fired row); the engine holds it in registers for a whole block"},
+                "kernel_ms_per_frame": round((tim["update_ms"] + tim["push_ms"]) / kernel_frames, 4),
+                "spikes_per_step_per_fly": round(spikes / (kernel_frames * steps * B), 1)}
+        if valu:
+            roof["valu_issue_frac"] = valu["valu_share"]
+            roof["limit"] = (f"VALU issue: {valu['valu_share']:.0%} of the SIMD cycles "
+                             f"({valu['valu_instr']:.3g} wave64 VALU instructions per launch)")
+        elif B == 1 and engine == "tile":
+            roof["limit"] = ("latency: each block is a chain of dependent memory round trips (spike list -> segment "
+                             f"table -> edges -> LDS sums) on {-(-bb.n // tile)} workgroups, well under the HBM and "
+                             "VALU rooflines")
+        res["roofline"] = roof
     del bb, probs
     torch.cuda.empty_cache()
     return res
@@ -429,6 +486,73 @@ def plastic_line(fly, p, torch, B=64, frames=300, warmup=3):
     return res
 
 
+SURROGATE_RECORD = os.path.join(REPO, "profiles", "surrogate_config5.json")
+
+
+def surrogate_line(torch, sim_fly_frames_per_s, members=3, batch=4096, frames=50):
+    """The learned DN-rate surrogate: ensemble inference frames/s at B = 4,096 (freshly initialised
+    members of the full 2048 -> 1024 -> 1024 + GRU 256 -> 8 network: the speed does not depend on
+    the weights), timed with HIP events; the MFMA roofline of its dominant kernel (the
+    member-batched first layer, 2 x 4096 x 1024 x 2048 x members flops per launch); the speedup
+    over the full simulation's fly-frames/s on the batched setup; and the judged record of a
+    trained ensemble (error on held-out frames, fallback fraction and frames/s with the real
+    fallback in the loop) from profiles/, with the command that made it."""
+    from flyhip import _lib
+    from flyhip.surrogate import GELU16, Ensemble, RateSurrogate
+    from flyhip.surrogate_pipeline import inference_fps
+
+    ens = Ensemble([RateSurrogate(seed=k, rate_scale=20.0) for k in range(members)])
+    fps = inference_fps(ens, batch, frames)
+    m0 = ens.members[0]
+    ens.pack()
+    x = torch.randn((batch, m0.n_in), device="cuda").to(torch.bfloat16)
+    h1 = torch.empty((members, batch, m0.hid), dtype=torch.bfloat16, device="cuda")
+    o = m0.off["W1"][0]
+    lib, st = _lib.load(), _lib.stream_handle()
+
+    def layer1():
+        _lib.check(lib.fz_sg_gemm_batched(members, batch, m0.hid, m0.n_in, x.data_ptr(), m0.n_in, 0, 0,
+                                          ens._W16.data_ptr() + 2 * o, m0.n_in, m0.size, 1, GELU16, None,
+                                          h1.data_ptr(), m0.hid, batch * m0.hid,
+                                          ens._W32.data_ptr() + 4 * m0.off["b1"][0], m0.size, st), "layer 1")
+
+    for _ in range(3):
+        layer1()
+    reps = 50
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        layer1()
+    e1.record()
+    e1.synchronize()
+    us = e0.elapsed_time(e1) * 1000.0 / reps
+    flops = 2.0 * batch * m0.hid * m0.n_in * members
+    tf = flops / (us * 1e-6) / 1e12
+    res = {"members": members, "batch": batch, "frames_per_s": round(fps), "unit": "frames/s",
+           "sim_fly_frames_per_s": round(sim_fly_frames_per_s) if sim_fly_frames_per_s else None,
+           "speedup_vs_sim": round(fps / sim_fly_frames_per_s, 1) if sim_fly_frames_per_s else None,
+           "roofline": {"bound": "mfma", "achieved": round(tf, 1), "peak": 2500.0, "unit": "TFLOP/s",
+                        "frac": round(tf / 2500.0, 4), "traffic": None,
+                        "kernel": "sg_gemm_big, member-batched first layer (bias + GELU epilogue)",
+                        "flops_per_launch": flops, "avg_launch_us": round(us, 2),
+                        "peak_note": "BF16 dense MFMA, 2.5 PFLOP/s"},
+           "flops_per_frame": 2.0 * (m0.n_in * m0.hid + m0.hid * m0.hid + m0.hid * 3 * m0.H + m0.H * 3 * m0.H
+                                     + (m0.hid + m0.H) * m0.n_out) * members}
+    try:
+        rec = json.load(open(SURROGATE_RECORD))
+        keep = ("frames", "mean_rate_hz", "err_surrogate", "err_with_fallback", "fallback_frac",
+                "err_constant_predictor", "err_single_sim_trial", "judge_noise_est", "threshold", "warm_frames",
+                "train_sim_frames", "train_steps", "members", "surrogate_frames_per_s", "sim_frames_per_s",
+                "frames_per_s_with_fallback", "speedup_with_fallback", "timed_loop", "command")
+        res["judged"] = {k: rec[k] for k in keep if k in rec}
+        res["judged"]["source"] = SURROGATE_RECORD[len(REPO) + 1:]
+    except (OSError, ValueError):
+        res["judged"] = None
+    del ens, x, h1
+    torch.cuda.empty_cache()
+    return res
+
+
 def profile_run(fly, p, spec, torch, frames=3):
     """Counter runs (rocprofv3 --pmc): one engine configuration for a few eager frames at a fixed
     launch shape, then a plain 1 GiB device-to-device copy whose bytes are known, which scales
@@ -449,13 +573,13 @@ def profile_run(fly, p, spec, torch, frames=3):
     bb.set_tune(upd_threads=int(ut), block_steps=int(K), tile_stage=int(stage), tile_source=int(src[0]) if src else 0)
     deg = np.diff(bb.indptr).astype(np.float64)
     steps = bb.steps_of(FRAME_MS)
-    spikes = push_bytes = 0.0
+    spikes = edges = 0.0
     for f in range(frames + 2):
         bb.p_ext[:, j] = probs[f]
         bb.run(FRAME_MS, graph=False, copy=False)
         if f >= 2:
             c = bb.counts.to(torch.float64).sum(0).cpu().numpy()
-            push_bytes += float((c * (8.0 + 10.0 * deg)).sum())
+            edges += float((c * deg).sum())
             spikes += float(c.sum())
     torch.cuda.synchronize()
     x = torch.ones(256 << 20, dtype=torch.int32, device="cuda")
@@ -464,9 +588,11 @@ def profile_run(fly, p, spec, torch, frames=3):
     y.copy_(x)
     torch.cuda.synchronize()
     launches = frames * -(-steps // int(K))
-    upd_bytes = frames * steps * B * (26.0 * bb.n + 4.0 * len(idx))
+    ab = algorithmic_bytes(engine, B, bb.n, len(idx), int(K), steps, frames, spikes, edges)
+    main_kernel = "tile" if engine == "tile" else "update"
     print(json.dumps({"profile": spec, "frames_counted": frames, "launches_counted": launches,
-                      "algorithmic_bytes_per_launch": round((upd_bytes + push_bytes) / launches),
+                      "algorithmic_bytes_per_launch": round(ab[main_kernel] / launches),
+                      "per_step_model_bytes_per_launch": round(ab[main_kernel + "_per_step_model"] / launches),
                       "spikes_per_step_per_fly": round(spikes / (frames * steps * B), 1),
                       "copy_bytes_read": x.numel() * 4, "copy_bytes_written": y.numel() * 4,
                       "note": "the first 2 frames (warm-up) are dispatched but not counted here"}), flush=True)
@@ -479,10 +605,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--batch", type=int, default=256, help="flies in the batched line (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=30)
-    ap.add_argument("--engines1", default="push,tile512,tile1024")
+    ap.add_argument("--engines1", default="push,tile512,tile768,tile1024")
     ap.add_argument("--enginesB", default="tile512,tile1024")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample length")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--plastic-frames", type=int, default=300, help="frames of the B=64 plasticity line (0: skip)")
     ap.add_argument("--fleet-frames", type=int, default=60, help="frames of the B=256 fleet line (0: skip)")
     ap.add_argument("--instruct-frames", type=int, default=100,
@@ -546,6 +673,15 @@ def main():
         batched = dict(bestB, unit="fly-sim-seconds/wall-second", vs_b1=round(bestB["value"] / value, 2),
                        candidates=[{k: r[k] for k in ("engine", "value", "ms_per_frame")} for r in linesB])
 
+    # ---- learned DN-rate surrogate at B = 4,096 ------------------------------------------------
+    surrogate = None
+    if not args.no_surrogate:
+        sim_ff = (batched["alive_fly_frames"] / (batched["frames"] * batched["ms_per_frame"] / 1000.0)
+                  if batched else None)
+        surrogate = surrogate_line(torch, sim_ff)
+        log(f"surrogate B=4096: {surrogate['frames_per_s']} frames/s ({surrogate['speedup_vs_sim']}x the simulation); "
+            f"layer 1 {surrogate['roofline']['achieved']} TFLOP/s")
+
     # ---- B = 64 with reward plasticity every frame ------------------------------------------
     plastic = None
     if args.plastic_frames > 0:
@@ -607,7 +743,7 @@ def main():
         cv2 = nf * steps_per_frame * p.dt / 1000.0 / (time.perf_counter() - t0)
         cpu = {"value": round(cv, 5), "unit": "fly-sim-seconds/wall-second", "cores": 1, "kind": "port",
                "sample": f"{nf} game frames ({nf * steps_per_frame} LIF steps) of the same B=1 workload, "
-                         f"C oracle -O2 single thread, {platform.processor() or platform.machine()}, "
+                         f"C oracle -O3 single thread, {platform.processor() or platform.machine()}, "
                          f"{os.cpu_count()} logical CPUs on host",
                "gpu_over_cpu": round(value / cv, 1),
                "openmp": {"value": round(cv2, 5), "cores": nthr, "gpu_over_cpu": round(value / cv2, 1)}}
@@ -621,7 +757,8 @@ def main():
                                    "rates resampled per frame", "neurons": fly.n,
                        "connections": int(fly.weights.nnz), "batch": 1, "steps_per_frame": steps_per_frame,
                        "engine": best["engine"], "parallelism": "single GPU"},
-            "roofline": best["roofline"], "cpu_baseline": cpu, "batched": batched, "plastic": plastic,
+            "roofline": best["roofline"], "cpu_baseline": cpu, "batched": batched, "surrogate": surrogate,
+            "plastic": plastic,
             "instruct": instruct, "eye": eye, "fleet": fleet,
             "b1_candidates": [{k: r[k] for k in ("engine", "value", "ms_per_frame")} for r in lines1]}
     print(json.dumps(line), flush=True)

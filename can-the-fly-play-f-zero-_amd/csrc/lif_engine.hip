@@ -504,6 +504,44 @@ __device__ __forceinline__ void xcd_order(int n_tiles, int B, int* T, int* b) {
     *T = L - *b * n_tiles;
 }
 
+// Offsets of the per-step spike lists a block's arrivals come from (the spikes of steps
+// t0 + k - n_delay, k < K): s_off[k] = entries of the source steps before k, s_off[K] = all.
+// The first wavefront's lanes load the K list lengths together and scan them with shuffles (K <=
+// KMAX < 64), instead of one thread walking them.  Every thread of the workgroup calls it: it
+// ends with a barrier.
+__device__ __forceinline__ void source_offsets(const int* cnt_b, long long t0, int lt0, int K, int nd, int M, int N,
+                                               int* s_off) {
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        int c = 0;
+        if (lane < K && t0 + lane - nd >= 0) {
+            c = cnt_b[(lt0 + lane - nd + M) % M];
+            c = c < N ? c : N;
+        }
+        int x = c;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(x, o);
+            if (lane >= o) x += y;
+        }
+        if (lane < K) s_off[lane] = x - c;
+        if (lane == K - 1) s_off[K] = x;
+    }
+    __syncthreads();
+}
+
+// The source step of entry e (0 <= e < s_off[K]) of those lists: the last k with s_off[k] <= e
+// (steps without spikes repeat an offset), by bisection.
+__device__ __forceinline__ int entry_step(const int* s_off, int K, int e) {
+    int lo = 0, hi = K;  // s_off[lo] <= e < s_off[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (s_off[mid] <= e) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
 // Inclusive prefix sum over the workgroup; wsum needs MAX_WAVES + 1 ints.
 __device__ __forceinline__ int block_incl_scan(int v, int* wsum, int* total) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
@@ -660,21 +698,7 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
         bkp4 = a.bk + ((size_t)par * a.B + b) * a.bk_stride + a.bk_off[T];
         __syncthreads();
     } else {
-        if (tid < a.K) {
-            const long long src = t0 + tid - nd;
-            const int c = src >= 0 ? cnt_b[(lt0 + tid - nd + M) % M] : 0;
-            s_lcnt[tid] = c < a.N ? c : a.N;
-        }
-        __syncthreads();
-        if (tid == 0) {
-            int o = 0;
-            for (int k = 0; k < a.K; k++) {
-                s_off[k] = o;
-                o += s_lcnt[k];
-            }
-            s_off[a.K] = o;
-        }
-        __syncthreads();
+        source_offsets(cnt_b, t0, lt0, a.K, nd, M, a.N, s_off);
         total = s_off[a.K];
     }
     const WT* w = static_cast<const WT*>(a.w);  // unused with packed edges
@@ -747,8 +771,7 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
                 const int e = base + tid * IPT + r;
                 ok[r] = e < total;
                 const int ec = ok[r] ? e : base;
-                kk[r] = 0;
-                while (s_off[kk[r] + 1] <= ec) kk[r]++;
+                kk[r] = entry_step(s_off, a.K, ec);
                 li_[r] = (size_t)((lt0 + kk[r] - nd + M) % M) * a.N + (ec - s_off[kk[r]]);
             }
 #pragma unroll
@@ -1119,17 +1142,8 @@ __global__ void __launch_bounds__(BK_THREADS, 8) tile_bucket(BucketArgs a) {
         s_cnt[lane] = 0;
         s_fill[lane] = 0;
     }
-    if (tid == 0) {
-        s_flush = 0;
-        int o = 0;
-        for (int k = 0; k < a.K; k++) {
-            s_off[k] = o;
-            const int c = t0 + k - nd >= 0 ? cnt_b[(lt0 + k - nd + M) % M] : 0;
-            o += c < a.N ? c : a.N;
-        }
-        s_off[a.K] = o;
-    }
-    __syncthreads();
+    if (tid == 0) s_flush = 0;
+    source_offsets(cnt_b, t0, lt0, a.K, nd, M, a.N, s_off);
     const int total = s_off[a.K];
     int* const dst = a.bk + s_base[lane];  // this lane's bucket: entry p at dst[p]
     const int capi = s_cap[lane];
@@ -1156,8 +1170,7 @@ __global__ void __launch_bounds__(BK_THREADS, 8) tile_bucket(BucketArgs a) {
         const int nc = total - c0 < BK_CHUNK ? total - c0 : BK_CHUNK;
         for (int q = tid; q < nc; q += blockDim.x) {  // stage the chunk: id << 5 | step, or -1
             const int e = c0 + q;
-            int k = 0;
-            while (s_off[k + 1] <= e) k++;
+            const int k = entry_step(s_off, a.K, e);
             const int id = idx_b[(size_t)((lt0 + k - nd + M) % M) * a.N + (e - s_off[k])];
             s_id[q] = (unsigned)id < (unsigned)a.N ? id << 5 | k : -1;  // id < 2^22 (table < 2^31 entries)
         }
