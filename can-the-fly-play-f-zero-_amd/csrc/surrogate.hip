@@ -322,82 +322,62 @@ __global__ void sg_gru_fwd_k(int B, int H, const float* gx, const float* gh, con
 // y_k = ([h2_k, s_k] Wo_k^T + bo_k) * rate_scale for members k < E (bf16 inputs and weights,
 // FP32 sums), then per rate the mean over members and their population variance (Welford), and
 // the relative disagreement ||std|| / (||mean|| + 1 Hz).  h2 [E][B][hid], s [E][B][H] (bf16);
-// Wo_k at w16 + k * wstride + wo_off ([n_pad][hid + H]); bo_k at w32 + k * wstride + bo_off.
-// A workgroup stages the E x n_out used rows of Wo in LDS once (16-byte vectors) and serves
-// ENS_ROWS frames, one wavefront per frame at a time: each lane takes whole 8-feature vectors
-// of the frame's [h2, s] row, then a butterfly sum over the lanes.  hid, H, wstride and wo_off
-// are multiples of 8 and the pointers 16-byte aligned (checked by the host).
+// Wo_k at w16 + k * wstride + wo_off ([n_pad >= n_out][hid + H]); bo_k at w32 + k * wstride + bo_off.
+// A workgroup serves 16 frames with one wavefront per member: the wavefront multiplies the
+// frames' [h2_k, s_k] rows (A fragments straight from global memory, 16 bytes per lane) by
+// Wo_k^T (B fragments: lane column n < n_out, zero beyond) on v_mfma_f32_16x16x32_bf16 along
+// k = hid + H, four k-chunks of loads in flight, and leaves its 16 x n_out outputs in LDS;
+// sixteen threads then fold the members in order.  hid, H, wstride and wo_off are multiples of
+// 8 and the pointers 16-byte aligned (checked by the host).
 constexpr int ENS_MAX = 8, OUT_MAX = 8, ENS_ROWS = 16;
-__global__ void __launch_bounds__(256) sg_ens_head_k(int B, int E, int hid, int H, int n_out, const uint16_t* h2,
-                                                     const uint16_t* s, const uint16_t* w16, const float* w32,
-                                                     long long wstride, long long wo_off, long long bo_off,
-                                                     float rate_scale, float* mean_out, float* spread_out) {
-    extern __shared__ __attribute__((aligned(16))) uint16_t wl[];  // [E][n_out][F]
-    const int F = hid + H, FV = F >> 3, HV = hid >> 3;
-    u32x4* wv = reinterpret_cast<u32x4*>(wl);
-    for (int q = threadIdx.x; q < E * n_out * FV; q += blockDim.x) {  // rows o < n_out of each Wo_k
-        const int row = q / FV, v = q - row * FV;
-        const int k = row / n_out, o = row - k * n_out;
-        wv[q] = reinterpret_cast<const u32x4*>(w16 + k * wstride + wo_off + (size_t)o * F)[v];
+__global__ void __launch_bounds__(64 * ENS_MAX) sg_ens_head_k(int B, int hid, int H, int n_out, const uint16_t* h2,
+                                                              const uint16_t* s, const uint16_t* w16, const float* w32,
+                                                              long long wstride, long long wo_off, long long bo_off,
+                                                              float rate_scale, float* mean_out, float* spread_out) {
+    __shared__ float ys[ENS_MAX][ENS_ROWS][OUT_MAX];
+    const int E = blockDim.x >> 6, k = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int F = hid + H, b0 = blockIdx.x * ENS_ROWS;
+    const int fr = lane & 15, fk = (lane >> 4) * 8;
+    const int b = b0 + fr;
+    const uint16_t* xh = h2 + ((size_t)k * B + b) * hid;
+    const uint16_t* xs = s + ((size_t)k * B + b) * H - hid;  // indexed by g >= hid
+    const uint16_t* wr = w16 + k * wstride + wo_off + (size_t)fr * F;
+    const bf16x8 zero = {0, 0, 0, 0, 0, 0, 0, 0};
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int kk = 0; kk < F; kk += 4 * 32) {
+        bf16x8 af[4], wf[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int g = kk + 32 * u + fk;  // a whole 8-group lies in h2 or in s (hid % 8 == 0)
+            af[u] = b < B && g < F ? *reinterpret_cast<const bf16x8*>(g < hid ? xh + g : xs + g) : zero;
+            wf[u] = fr < n_out && g < F ? *reinterpret_cast<const bf16x8*>(wr + g) : zero;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[u], wf[u], acc, 0, 0, 0);
+    }
+    // lane holds C[4 * (lane / 16) + i][lane % 16]: frame row, output column
+    if (fr < n_out) {
+        const float bo = w32[k * wstride + bo_off + fr];
+#pragma unroll
+        for (int i = 0; i < 4; i++) ys[k][(lane >> 4) * 4 + i][fr] = (acc[i] + bo) * rate_scale;
     }
     __syncthreads();
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    for (int r = wave; r < ENS_ROWS; r += nw) {
-        const int b = blockIdx.x * ENS_ROWS + r;
-        if (b >= B) break;  // wavefront-uniform
-        float mean[OUT_MAX], m2[OUT_MAX];
-#pragma unroll
-        for (int o = 0; o < OUT_MAX; o++) mean[o] = m2[o] = 0.f;
-        for (int k = 0; k < E; k++) {
-            const u32x4* xh = reinterpret_cast<const u32x4*>(h2 + ((size_t)k * B + b) * hid);
-            const u32x4* xs = reinterpret_cast<const u32x4*>(s + ((size_t)k * B + b) * H);
-            const u32x4* wk = wv + (size_t)k * n_out * FV;
-            float part[OUT_MAX];
-#pragma unroll
-            for (int o = 0; o < OUT_MAX; o++) part[o] = 0.f;
-            for (int v = lane; v < FV; v += 64) {
-                const u32x4 xv = v < HV ? xh[v] : xs[v - HV];
-                float x[8];
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    x[2 * j] = __uint_as_float(xv[j] << 16);
-                    x[2 * j + 1] = __uint_as_float(xv[j] & 0xffff0000u);
-                }
-#pragma unroll
-                for (int o = 0; o < OUT_MAX; o++) {
-                    if (o >= n_out) continue;
-                    const u32x4 w = wk[o * FV + v];
-                    float acc = part[o];
-#pragma unroll
-                    for (int j = 0; j < 4; j++) {
-                        acc += x[2 * j] * __uint_as_float(w[j] << 16);
-                        acc += x[2 * j + 1] * __uint_as_float(w[j] & 0xffff0000u);
-                    }
-                    part[o] = acc;
-                }
+    if (threadIdx.x < ENS_ROWS && b0 + (int)threadIdx.x < B) {
+        const int r = threadIdx.x, bb = b0 + r;
+        float mn2 = 0.f, sd2 = 0.f;
+        for (int o = 0; o < n_out; o++) {
+            float mean = 0.f, m2 = 0.f;
+            for (int q = 0; q < E; q++) {
+                const float y = ys[q][r][o];
+                const float dlt = y - mean;
+                mean += dlt / (float)(q + 1);
+                m2 += dlt * (y - mean);
             }
-#pragma unroll
-            for (int o = 0; o < OUT_MAX; o++) {
-                if (o >= n_out) continue;
-                float v = part[o];
-                for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d);
-                const float y = (v + w32[k * wstride + bo_off + o]) * rate_scale;
-                const float dlt = y - mean[o];
-                mean[o] += dlt / (float)(k + 1);
-                m2[o] += dlt * (y - mean[o]);
-            }
+            mean_out[(size_t)bb * n_out + o] = mean;
+            mn2 += mean * mean;
+            sd2 += m2 / (float)E;
         }
-        if (lane == 0) {
-            float mn2 = 0.f, sd2 = 0.f;
-#pragma unroll
-            for (int o = 0; o < OUT_MAX; o++) {
-                if (o >= n_out) continue;
-                mean_out[(size_t)b * n_out + o] = mean[o];
-                mn2 += mean[o] * mean[o];
-                sd2 += m2[o] / (float)E;
-            }
-            spread_out[b] = sqrtf(sd2) / (sqrtf(mn2) + 1.0f);
-        }
+        spread_out[bb] = sqrtf(sd2) / (sqrtf(mn2) + 1.0f);
     }
 }
 
@@ -523,12 +503,12 @@ int fz_sg_ens_head(int B, int members, int hid, int H, int n_out, const void* h2
                                                           reinterpret_cast<uintptr_t>(w16)) & 15))
         return fzi_fail(FZ_E_ARG, "ens_head: widths, strides and offsets must be multiples of 8 elements, pointers "
                         "16-byte aligned");
-    const size_t lds = (size_t)members * n_out * (hid + H) * sizeof(uint16_t);
-    if (lds > 64 * 1024) return fzi_fail(FZ_E_ARG, "ens_head: %zu B of output weights exceed 64 KB of LDS", lds);
-    hipLaunchKernelGGL(sg_ens_head_k, dim3(blocks(B, ENS_ROWS)), dim3(256), lds, static_cast<hipStream_t>(stream), B,
-                       members, hid, H, n_out, static_cast<const uint16_t*>(h2), static_cast<const uint16_t*>(s),
-                       static_cast<const uint16_t*>(w16), w32, wstride, wo_off, bo_off, rate_scale, mean, spread);
-    return hipGetLastError() == hipSuccess ? FZ_OK : fzi_fail(FZ_E_HIP, "ens_head launch failed");
+    hipLaunchKernelGGL(sg_ens_head_k, dim3(blocks(B, ENS_ROWS)), dim3(64 * members), 0,
+                       static_cast<hipStream_t>(stream), B, hid, H, n_out, static_cast<const uint16_t*>(h2),
+                       static_cast<const uint16_t*>(s), static_cast<const uint16_t*>(w16), w32, wstride, wo_off,
+                       bo_off, rate_scale, mean, spread);
+    if (hipGetLastError() != hipSuccess) return fzi_fail(FZ_E_HIP, "ens_head launch failed");
+    return FZ_OK;
 }
 
 int fz_sg_gelu_bwd(long long n, const float* dact, const float* pre, float* dpre32, void* dpre16, void* stream) {
