@@ -71,6 +71,7 @@ constexpr int MAX_WAVES = MAX_THREADS / 64;
 constexpr int QMIN = 64;          // smallest push chunk (sizes the chunk-start table)
 constexpr int BK_PIECE = 4;  // edges per tile-bucket entry at most (longer segments are split)
 constexpr size_t TILE_STATIC_LDS = 1024;  // room left for lif_tile's static shared arrays
+constexpr size_t TP_MAX_BYTES = 1ull << 30;  // largest (first edge, length) pair table kept
 constexpr int PL_SERIAL = 2;  // plastic rows up to this length are pushed by their own thread
 constexpr int UNROLL = 4;         // tile engine: edge loads in flight per thread
 
@@ -478,6 +479,7 @@ struct TileArgs {
     int* spk_idx;          // [B][M][N]
     int* spk_cnt;          // [B][M]
     const uint16_t* tb16;  // segment table as offsets from the row start, or null
+    const int2* tp;        // [N][n_tiles] (first edge, length) of each segment, or null
     const int* indptr;
     const int* bk;         // tile buckets (source 1): [2][B][bk_stride] entries (first edge << 7 |
                            // (len - 1) << 5 | step), two buffers alternating with the block parity
@@ -777,7 +779,16 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
 #pragma unroll
             for (int r = 0; r < IPT; r++) ids[r] = idx_b[li_[r]];  // ... then all list loads at once
             const unsigned nt1 = (unsigned)(a.n_tiles + 1);
-            if (a.tb16) {
+            if (a.tp) {  // one 8-byte load per (spike, tile): the L2 request rate bounds this phase
+#pragma unroll
+                for (int r = 0; r < IPT; r++) {
+                    ok[r] = ok[r] && (unsigned)ids[r] < (unsigned)a.N;
+                    const int idc = ok[r] ? ids[r] : 0;
+                    const int2 q = a.tp[__umul24((unsigned)idc, (unsigned)a.n_tiles) + (unsigned)T];  // < 2^31
+                    seg[r] = q.x;
+                    len[r] = q.y & -(int)ok[r];
+                }
+            } else if (a.tb16) {
 #pragma unroll
                 for (int r = 0; r < IPT; r++) {
                     ok[r] = ok[r] && (unsigned)ids[r] < (unsigned)a.N;
@@ -1318,6 +1329,7 @@ struct fz_engine {
     int lbits = 0;
     // tile buckets (tile source 1): per-tile offsets in a fly's arena (entries), arena, counts
     uint16_t* tb16 = nullptr;  // segment table as uint16 offsets from each row's start
+    int2* tp = nullptr;        // segment table as (first edge, length) pairs, when it fits TP_MAX_BYTES
     int bk4 = 0;
     std::vector<long long> bk_off_h;
     long long* bk_off = nullptr;
@@ -1353,7 +1365,7 @@ int resolve(const fz_engine* e, const fz_tune* t, Shape* s) {
     s->ipt = (t && t->tile_stage) ? t->tile_stage : 1;
     if (s->ipt != 1 && s->ipt != 4) return fail(FZ_E_ARG, "tile_stage %d: must be 1 or 4", s->ipt);
     s->src = t ? t->tile_source : 0;
-    if (s->src != 0 && s->src != 1) return fail(FZ_E_ARG, "tile_source %d: must be 0 or 1", s->src);
+    if (s->src < 0 || s->src > 2) return fail(FZ_E_ARG, "tile_source %d: must be 0, 1 or 2", s->src);
     if (s->src == 1 && (e->mode != 2 || !e->bk_off)) return fail(FZ_E_ARG, "tile_source 1 needs the tile engine");
     if (s->src == 1 && !e->bk4)
         return fail(FZ_E_ARG, "tile_source 1 needs fewer than 2^24 fixed edges (bucket entries hold a 24-bit edge "
@@ -1365,7 +1377,7 @@ int resolve(const fz_engine* e, const fz_tune* t, Shape* s) {
             return fail(FZ_E_ARG, "tile engine: upd_threads %d must divide the tile (%d neurons) into 1, 2, 4 or 8",
                         s->ut, e->ts);
         // K arrival columns and the hit-mask column per neuron, plus the segment-table staging
-        const size_t lds = (size_t)(s->K + 1) * e->ts * 4 + (s->src == 0 ? (size_t)3 * s->ipt * s->ut * 4 : 0);
+        const size_t lds = (size_t)(s->K + 1) * e->ts * 4 + (s->src != 1 ? (size_t)3 * s->ipt * s->ut * 4 : 0);
         if (lds > e->lds_max - TILE_STATIC_LDS)
             return fail(FZ_E_ARG, "tile engine: %zu B of LDS for %d steps x %d neurons exceeds %zu", lds, s->K,
                         e->ts, e->lds_max);
@@ -1495,6 +1507,7 @@ TileArgs make_tile(const fz_engine* e, const fz_inputs* in, int K, int KP, int k
     a.lbits = e->lbits;
     a.tb = e->tb;
     a.tb16 = e->tb16;
+    a.tp = e->tp;
     a.indptr = e->indptr;
     a.bk = e->bk;
     a.bk_off = e->bk_off;
@@ -1539,7 +1552,7 @@ void* tile_kernel(int fmt, int npt, int ipt, int src) {
 int tile_fmt(const fz_engine* e) { return e->pk ? 2 : (e->w_bits == 16 ? 0 : 1); }
 
 // LDS staging of the segment-table source (segment, prefix end, step per staged entry).
-size_t tile_staging(const Shape& s) { return s.src == 0 ? (size_t)3 * s.ipt * s.ut * 4 : 0; }
+size_t tile_staging(const Shape& s) { return s.src != 1 ? (size_t)3 * s.ipt * s.ut * 4 : 0; }
 
 // Row stride of the LDS accumulator: K arrival columns and the hit-mask column, rounded up to
 // odd (no bank conflicts in the step loop) when that fits the LDS budget.
@@ -1615,6 +1628,18 @@ int validate_tiles(fz_engine* e, const int* tb, int n_tiles, int tile) {
     return FZ_OK;
 }
 
+// (first edge, length) of every (row, tile) segment, [n][n_tiles]: the table source of lif_tile
+// reads one pair per spike instead of two table entries and the row start.
+__global__ void tile_pairs(int n, int n_tiles, const int* tb, int2* tp) {
+    const long long total = (long long)n * n_tiles;
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+         q += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(q / n_tiles), t = (int)(q - (long long)i * n_tiles);
+        const long long at = (long long)i * (n_tiles + 1) + t;
+        tp[q] = make_int2(tb[at], tb[at + 1] - tb[at]);
+    }
+}
+
 __global__ void tile_rel16(int n, int n_tiles, const int* indptr, const int* tb, uint16_t* tb16, int* over) {
     const long long total = (long long)n * (n_tiles + 1);
     for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
@@ -1678,6 +1703,21 @@ int bucket_layout(fz_engine* e, const int* tb, int n_tiles) {
         (void)hipGetLastError();
         (void)hipFree(t16);
         (void)hipFree(over);
+    }
+    // the pair table, when it takes at most TP_MAX_BYTES (the lookups fall back to the tables above)
+    (void)hipFree(e->tp);
+    e->tp = nullptr;
+    {
+        const size_t np = (size_t)e->n * (size_t)n_tiles;
+        if (np * sizeof(int2) <= TP_MAX_BYTES && hipMalloc(&e->tp, np * sizeof(int2)) == hipSuccess) {
+            const unsigned g = (unsigned)(np / 256 + 1 < 65536 ? np / 256 + 1 : 65536);
+            hipLaunchKernelGGL(tile_pairs, dim3(g), dim3(256), 0, 0, e->n, n_tiles, tb, e->tp);
+            if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+                (void)hipFree(e->tp);
+                e->tp = nullptr;
+            }
+        }
+        (void)hipGetLastError();
     }
     HIPCHK(hipMalloc(&e->bk_off, sizeof(long long) * ((size_t)n_tiles + 1)));
     HIPCHK(hipMemcpy(e->bk_off, e->bk_off_h.data(), sizeof(long long) * ((size_t)n_tiles + 1),
@@ -1768,6 +1808,7 @@ int enqueue(fz_engine* e, const fz_inputs* in, int steps, const unsigned* hits, 
                 if ((rc = mark(ev_push))) return rc;
             }
             TileArgs ta = make_tile(e, in, K, tile_kp(e, s, K), kb, blk, kb == 0, hits, raster);
+            if (s.src == 2) ta.tp = nullptr;
             void* args[] = {&ta};
             if ((rc = mark(ev_upd))) return rc;
             HIPCHK(hipLaunchKernel(fn, tgrid, dim3(s.ut), args, tile_lds(e, s, K), stream));
@@ -1933,6 +1974,7 @@ int fz_destroy(fz_engine* e) {
     (void)hipFree(e->bk);
     (void)hipFree(e->bk_cnt);
     (void)hipFree(e->tb16);
+    (void)hipFree(e->tp);
     delete e;
     return FZ_OK;
 }
@@ -2113,7 +2155,7 @@ int fz_run_static(fz_engine* e, const fz_inputs* in, int steps, const unsigned* 
     if (dev != e->dev)
         return fail(FZ_E_STATE, "fz_run_static: the stream's device %d is not the engine's device %d", dev, e->dev);
     if (e->mode == 2) {  // the segment-table source reads only caller state: no engine scratch
-        if (s.src != 0)
+        if (s.src == 1)
             return fail(FZ_E_ARG, "fz_run_static: the tile engine's bucket source (tile_source 1) needs the bucket "
                         "arena; use tile_source 0 or fz_run");
         return enqueue(e, in, steps, hits, raster, s, hs, nullptr, nullptr, e->wl);
@@ -2252,6 +2294,7 @@ int fz_graph_create(fz_engine* e, const fz_inputs* in, int steps, const fz_tune*
                         return bail("hipGraphAddKernelNode(bucket)", err);
                 }
                 TileArgs ta = make_tile(e, in, K, tile_kp(e, s, K), kb, blk, kb == 0, nullptr, nullptr);
+                if (s.src == 2) ta.tp = nullptr;
                 ta.b0 = c.b0;
                 ta.nb = c.nb;
                 void* targs[] = {&ta};

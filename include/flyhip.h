@@ -80,11 +80,14 @@ typedef struct fz_tune {
     int push_grid;     /* push workgroups (grid-stride) */
     int block_steps;   /* steps per temporal block, 1..min(n_delay, 32) */
     int tile_stage;    /* tile engine: spike-list entries staged per thread, 1 or 4 */
-    int tile_source;   /* tile engine: 0 = every tile looks its segments up in the segment table;
-                        * 1 = a prologue kernel per block expands each spike once into per-tile
-                        * buckets of pieces (first edge, length <= 4, step): engine scratch,
-                        * allocated on first use, 2 x 4 B x pieces bound per fly; needs fewer
-                        * than 2^24 fixed edges */
+    int tile_source;   /* tile engine: 0 = every tile looks its segments up in the segment table
+                        * (one 8-byte (first edge, length) pair per spike and tile: an engine copy
+                        * of the table, built by fz_set_tiles when it takes at most 1 GiB, else
+                        * as in 2); 1 = a prologue kernel per block expands each spike once into
+                        * per-tile buckets of pieces (first edge, length <= 4, step): engine
+                        * scratch, allocated on first use, 2 x 4 B x pieces bound per fly; needs
+                        * fewer than 2^24 fixed edges; 2 = as 0 from the caller's int32 table (or
+                        * its uint16 row-offset copy): three loads per spike and tile */
 } fz_tune;
 
 int fz_version(void);

@@ -289,6 +289,8 @@ def test_extreme_launch_shapes_are_exact(small_fly, shape):
                                         (2048, dict(upd_threads=512, block_steps=9, tile_stage=4, tile_source=1)),
                                         (256, dict(upd_threads=256, block_steps=7, tile_source=1)),
                                         (768, dict(upd_threads=384, tile_source=1)),
+                                        (1024, dict(upd_threads=256, tile_source=2)),
+                                        (2048, dict(upd_threads=512, block_steps=9, tile_stage=4, tile_source=2)),
                                         (640, dict(upd_threads=640, tile_stage=4))])
 def test_tile_engine_shapes_are_exact(small_fly, tile, shape):
     # shapes that the LDS budget admits, incl. 8 neurons per thread, 5-step blocks and tiles
@@ -341,7 +343,7 @@ def test_full_connectome_raster_1000_steps_and_one_second_of_frames(full_fly, en
     _assert_state_equal(gb, ob)
 
 
-@pytest.mark.parametrize("source", [0, 1])
+@pytest.mark.parametrize("source", [0, 1, 2])
 def test_tile_sources_give_identical_rasters_and_plastic_arrivals(small_fly, source):
     # per-tile buckets versus segment-table lookups: same rasters, counts and state as the
     # oracle, with plastic synapses (pushed through the ring) and a retired fly
@@ -370,7 +372,7 @@ def test_tile_sources_give_identical_rasters_and_plastic_arrivals(small_fly, sou
 
 
 @pytest.mark.parametrize("t_ref", [0.5, 1.8])
-@pytest.mark.parametrize("source", [0, 1])
+@pytest.mark.parametrize("source", [0, 1, 2])
 def test_tile_engine_refractory_shorter_than_or_equal_to_a_block(small_fly, t_ref, source):
     # 18-step blocks: with t_ref = 0.5 ms (5 steps) a neuron can fire several times in one block
     # (the general step loop); with t_ref = 1.8 ms (18 steps) at most once, at the limit of the
@@ -425,7 +427,7 @@ def _random_graph(n, deg, seed, hub_deg=0):
     return w
 
 
-@pytest.mark.parametrize("source", [0, 1])
+@pytest.mark.parametrize("source", [0, 1, 2])
 def test_tile_engine_row_longer_than_65535_edges(source):
     # the uint16 segment-table copy cannot hold this row: the int32 table is used instead
     w = _random_graph(80_000, 20, seed=31, hub_deg=70_000)
