@@ -480,6 +480,10 @@ struct TileArgs {
     int* spk_cnt;          // [B][M]
     const uint16_t* tb16;  // segment table as offsets from the row start, or null
     const int2* tp;        // [N][n_tiles] (first edge, length) of each segment, or null
+    const int2* ptp;       // [N][n_tiles] the same over the plastic CSR: plastic edges are pulled
+                           // for spikes that arrive in their window; null: all are pushed
+    long long nnz;         // fixed edges: a segment or bucket entry nnz + q is plastic edge q
+    int win;               // steps of this window
     const int* indptr;
     const int* bk;         // tile buckets (source 1): [2][B][bk_stride] entries (first edge << 7 |
                            // (len - 1) << 5 | step), two buffers alternating with the block parity
@@ -624,7 +628,9 @@ __device__ __forceinline__ int once_steps(const int* row, int K, float& u_io, fl
 // IPT: spike-list entries staged per thread (more loads in flight, more LDS).
 // BK: segments come from the tile's bucket (filled by tile_bucket) instead of being looked up
 // in the segment table for every spike of the fly.
-template <typename WT, int NPT, int IPT, bool BK>
+// PL: plastic synapses are pulled here, with the fly's own weights, for spikes that arrive in
+// the window they were emitted in (a.ptp); only the others are pushed at spike time.
+template <typename WT, int NPT, int IPT, bool BK, bool PL>
 __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
     extern __shared__ __attribute__((aligned(16))) int lds[];
     __shared__ int s_off[KMAX + 1];
@@ -705,7 +711,16 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
     }
     const WT* w = static_cast<const WT*>(a.w);  // unused with packed edges
     const int lmask = (1 << a.lbits) - 1;
+    const float* pwf = PL ? a.pw + (size_t)b * a.n_pl : nullptr;  // the fly's plastic weights
     auto edge = [&](int ek, int* loc, int* val) {
+        if constexpr (PL) {
+            if (ek >= a.nnz) {  // plastic edge q = ek - nnz, the fly's weight in this window
+                const int q = ek - (int)a.nnz;
+                *loc = a.pl_post[q] - i0;
+                *val = (int)rintf(pwf[q] * a.pw_qscale);
+                return;
+            }
+        }
         if constexpr (std::is_same<WT, PackedEdge>::value) {
             const int x = a.pk[ek];  // one load: value and tile-local target
             *loc = x & lmask;
@@ -779,6 +794,14 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
 #pragma unroll
             for (int r = 0; r < IPT; r++) ids[r] = idx_b[li_[r]];  // ... then all list loads at once
             const unsigned nt1 = (unsigned)(a.n_tiles + 1);
+            int2 pq[PL ? IPT : 1];  // plastic segments, loaded beside the fixed ones
+            if constexpr (PL) {
+#pragma unroll
+                for (int r = 0; r < IPT; r++) {
+                    const int idc = ok[r] && (unsigned)ids[r] < (unsigned)a.N ? ids[r] : 0;
+                    pq[r] = a.ptp[__umul24((unsigned)idc, (unsigned)a.n_tiles) + (unsigned)T];
+                }
+            }
             if (a.tp) {  // one 8-byte load per (spike, tile): the L2 request rate bounds this phase
 #pragma unroll
                 for (int r = 0; r < IPT; r++) {
@@ -809,51 +832,66 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
                     len[r] = (a.tb[at + 1] - seg[r]) & -(int)ok[r];  // masked, not branched on
                 }
             }
+            // pass 0: the fixed segments; pass 1: the plastic ones of spikes emitted in this window
+            // (emitted at window step kb - n_delay + k >= 0), staged the same way
 #pragma unroll
-            for (int r = 0; r < IPT; r++) sum += len[r];
-            int tot;
-            int run = block_incl_scan(sum, s_wsum, &tot) - sum;
+            for (int pass = 0; pass < (PL ? 2 : 1); pass++) {
+                if (PL && pass == 1) {
 #pragma unroll
-            for (int r = 0; r < IPT; r++) {
-                run += len[r];
-                st_seg[tid * IPT + r] = seg[r];
-                st_end[tid * IPT + r] = run;
-                st_k[tid * IPT + r] = kk[r];
-            }
-            __syncthreads();
-            const int ns = TPB * IPT;
-            for (int q0 = tid; q0 < tot; q0 += TPB * UNROLL) {
-                int ek[UNROLL], slot[UNROLL];
-#pragma unroll
-                for (int u = 0; u < UNROLL; u++) {
-                    const int q = q0 + u * TPB;
-                    ek[u] = -1;
-                    slot[u] = 0;
-                    if (q < tot) {
-                        int lo = 0, hi = ns - 1;  // first j with st_end[j] > q
-                        while (lo < hi) {
-                            const int mid = (lo + hi) >> 1;
-                            if (st_end[mid] > q) hi = mid;
-                            else lo = mid + 1;
-                        }
-                        ek[u] = st_seg[lo] + (q - (lo ? st_end[lo - 1] : 0));
-                        slot[u] = st_k[lo];
+                    for (int r = 0; r < IPT; r++) {
+                        seg[r] = (int)a.nnz + pq[PL ? r : 0].x;
+                        len[r] = ok[r] && a.kb + kk[r] - nd >= 0 ? pq[PL ? r : 0].y : 0;
                     }
                 }
-                int loc[UNROLL], val[UNROLL];
+                sum = 0;
 #pragma unroll
-                for (int u = 0; u < UNROLL; u++) {  // all UNROLL edge loads in flight
-                    edge(ek[u] >= 0 ? ek[u] : ek[0], &loc[u], &val[u]);
-                    if (ek[u] < 0) val[u] = 0;
+                for (int r = 0; r < IPT; r++) sum += len[r];
+                int tot;
+                int run = block_incl_scan(sum, s_wsum, &tot) - sum;
+#pragma unroll
+                for (int r = 0; r < IPT; r++) {
+                    run += len[r];
+                    st_seg[tid * IPT + r] = seg[r];
+                    st_end[tid * IPT + r] = run;
+                    st_k[tid * IPT + r] = kk[r];
                 }
+                __syncthreads();
+                const int ns = TPB * IPT;
+                for (int q0 = tid; q0 < tot; q0 += TPB * UNROLL) {
+                    int ek[UNROLL], slot[UNROLL];
 #pragma unroll
-                for (int u = 0; u < UNROLL; u++)
-                    if (val[u] != 0) atomicAdd(&acc[__umul24(loc[u], KP) + slot[u]], val[u]);
+                    for (int u = 0; u < UNROLL; u++) {
+                        const int q = q0 + u * TPB;
+                        ek[u] = -1;
+                        slot[u] = 0;
+                        if (q < tot) {
+                            int lo = 0, hi = ns - 1;  // first j with st_end[j] > q
+                            while (lo < hi) {
+                                const int mid = (lo + hi) >> 1;
+                                if (st_end[mid] > q) hi = mid;
+                                else lo = mid + 1;
+                            }
+                            ek[u] = st_seg[lo] + (q - (lo ? st_end[lo - 1] : 0));
+                            slot[u] = st_k[lo];
+                        }
+                    }
+                    int loc[UNROLL], val[UNROLL];
+#pragma unroll
+                    for (int u = 0; u < UNROLL; u++) {  // all UNROLL edge loads in flight
+                        edge(ek[u] >= 0 ? ek[u] : ek[0], &loc[u], &val[u]);
+                        if (ek[u] < 0) val[u] = 0;
+                    }
+#pragma unroll
+                    for (int u = 0; u < UNROLL; u++)
+                        if (val[u] != 0) atomicAdd(&acc[__umul24(loc[u], KP) + slot[u]], val[u]);
+                }
+                __syncthreads();
             }
-            __syncthreads();
         }
     }
-    if (a.ptg_ptr) {  // plastic arrivals (pushed at spike time), tile's targets only
+    // plastic arrivals pushed at spike time (all of them, or with plastic pulls only those from
+    // earlier windows, which land in a window's first n_delay steps), tile's targets only
+    if (a.ptg_ptr && (!PL || a.kb < nd)) {
         const int p0 = a.ptg_ptr[T], np_ = a.ptg_ptr[T + 1] - p0;
         for (int q = tid; q < np_ * a.K; q += TPB) {
             const int k = q / np_;
@@ -1008,20 +1046,28 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
         int* p_end = acc + TS;      // [TS] inclusive prefix of row lengths
         int* p_k = acc + 2 * TS;    // [TS] block step of the spike
         const float* pwb = a.pw + (size_t)b * a.n_pl;
+        // with plastic pulls only the spikes whose arrival (block step k + n_delay) lies past this
+        // window are pushed: block steps k >= win - kb - n_delay
+        uint32_t cross = 0xffffffffu;
+        if constexpr (PL) {
+            const int first = a.win - a.kb - nd;
+            cross = first <= 0 ? 0xffffffffu : (first >= 32 ? 0u : ~((1u << first) - 1u));
+        }
         uint32_t rem[NPT];
         bool more = false;
 #pragma unroll
         for (int p = 0; p < NPT; p++) {
             rem[p] = 0;
             const int i = i0 + p * TPB + tid;
-            if (masks[p] == 0) continue;
+            const uint32_t pm = masks[p] & cross;
+            if (pm == 0) continue;
             const int q0 = a.pl_indptr[i], q1 = a.pl_indptr[i + 1];
             if (q1 - q0 > PL_SERIAL) {
-                rem[p] = masks[p];
+                rem[p] = pm;
                 more = true;
                 continue;
             }
-            for (uint32_t m = masks[p]; m; m &= m - 1) {
+            for (uint32_t m = pm; m; m &= m - 1) {
                 int* r = a.ring + ((size_t)b * a.ring_slots + (rs0 + __ffs(m) - 1 + nd) % a.ring_slots) * a.N;
                 for (int q = q0; q < q1; q++) {
                     const int val = (int)rintf(pwb[q] * a.pw_qscale);
@@ -1097,6 +1143,8 @@ struct BucketArgs {
     const int* spk_cnt;
     const int* tb;
     const uint16_t* tb16;
+    const int2* ptp;  // plastic (first edge, length) pairs, or null: no plastic entries
+    long long nnz;
     const int* indptr;
     const long long* bk_off;
     long long bk_stride;
@@ -1115,7 +1163,9 @@ struct BucketArgs {
 // them straight to memory would cost one memory transaction per 4-byte entry.  A flush runs
 // after a round (every wavefront one batch) that left some stage more than half full; entries
 // past a full stage go straight to memory.
-template <bool T16>
+// PL: also the plastic segments (entries nnz + first plastic edge) of the spikes emitted in this
+// window, which lif_tile pulls with the fly's own weights.
+template <bool T16, bool PL>
 __global__ void __launch_bounds__(BK_THREADS, 8) tile_bucket(BucketArgs a) {
     __shared__ int s_off[KMAX + 1];
     __shared__ int s_cnt[64];          // next position in each tile's bucket
@@ -1204,8 +1254,10 @@ __global__ void __launch_bounds__(BK_THREADS, 8) tile_bucket(BucketArgs a) {
                     kk[u] = v & 31;
                 }
                 int l16[BK_UNROLL], h16[BK_UNROLL], r0_[BK_UNROLL];
+                int2 pq[PL ? BK_UNROLL : 1];
 #pragma unroll
                 for (int u = 0; u < BK_UNROLL; u++) {  // every load first ...
+                    if constexpr (PL) pq[u] = a.ptp[(unsigned)id[u] * (unsigned)a.n_tiles + (unsigned)Tc];
                     if constexpr (T16) {
                         const uint16_t* row = a.tb16 + (unsigned)id[u] * (unsigned)nt1;  // < 2^31 (checked)
                         r0_[u] = a.indptr[id[u]];
@@ -1228,6 +1280,15 @@ __global__ void __launch_bounds__(BK_THREADS, 8) tile_bucket(BucketArgs a) {
                 int tot = 0;
 #pragma unroll
                 for (int u = 0; u < BK_UNROLL; u++) tot += (int)(((unsigned)len[u] + (BK_PIECE - 1)) / BK_PIECE);
+                int plo[PL ? BK_UNROLL : 1], plen[PL ? BK_UNROLL : 1];
+                if constexpr (PL) {
+#pragma unroll
+                    for (int u = 0; u < BK_UNROLL; u++) {  // emitted at window step kb - n_delay + k
+                        plo[u] = (int)a.nnz + pq[u].x;
+                        plen[u] = use[u] && a.kb + kk[u] - nd >= 0 ? pq[u].y : 0;
+                        tot += (int)(((unsigned)plen[u] + (BK_PIECE - 1)) / BK_PIECE);
+                    }
+                }
                 if (tot > 0) {
                     int slot = atomicAdd(&s_fill[lane], tot);
                     const int over = slot + tot - BK_STAGE;
@@ -1242,13 +1303,19 @@ __global__ void __launch_bounds__(BK_THREADS, 8) tile_bucket(BucketArgs a) {
                         }
                         slot++;
                     };
+                    auto pieces = [&](int first, int n, int k) {
+                        put((first << 7) | (((n < BK_PIECE ? n : BK_PIECE) - 1) << 5) | k);
+                        for (int c = BK_PIECE; c < n; c += BK_PIECE)  // rare: longer segments
+                            put(((first + c) << 7) | (((n - c < BK_PIECE ? n - c : BK_PIECE) - 1) << 5) | k);
+                    };
 #pragma unroll
                     for (int u = 0; u < BK_UNROLL; u++)
-                        if (len[u] > 0) {
-                            put((lo[u] << 7) | (((len[u] < BK_PIECE ? len[u] : BK_PIECE) - 1) << 5) | kk[u]);
-                            for (int c = BK_PIECE; c < len[u]; c += BK_PIECE)  // rare: longer segments
-                                put(((lo[u] + c) << 7) | (((len[u] - c < BK_PIECE ? len[u] - c : BK_PIECE) - 1) << 5) | kk[u]);
-                        }
+                        if (len[u] > 0) pieces(lo[u], len[u], kk[u]);
+                    if constexpr (PL) {
+#pragma unroll
+                        for (int u = 0; u < BK_UNROLL; u++)
+                            if (plen[u] > 0) pieces(plo[u], plen[u], kk[u]);
+                    }
                 }
             }
             __syncthreads();
@@ -1330,6 +1397,7 @@ struct fz_engine {
     // tile buckets (tile source 1): per-tile offsets in a fly's arena (entries), arena, counts
     uint16_t* tb16 = nullptr;  // segment table as uint16 offsets from each row's start
     int2* tp = nullptr;        // segment table as (first edge, length) pairs, when it fits TP_MAX_BYTES
+    int2* ptp = nullptr;       // the plastic CSR's (first edge, length) per (row, tile): plastic pulls
     int bk4 = 0;
     std::vector<long long> bk_off_h;
     long long* bk_off = nullptr;
@@ -1354,6 +1422,7 @@ namespace {
 
 struct Shape {
     int ut, pt, chunk, grid, K, ipt, src;
+    int pl;  // plastic pulls in effect (requested, and the engine has the plastic pair table)
 };
 
 int resolve(const fz_engine* e, const fz_tune* t, Shape* s) {
@@ -1365,6 +1434,10 @@ int resolve(const fz_engine* e, const fz_tune* t, Shape* s) {
     s->ipt = (t && t->tile_stage) ? t->tile_stage : 1;
     if (s->ipt != 1 && s->ipt != 4) return fail(FZ_E_ARG, "tile_stage %d: must be 1 or 4", s->ipt);
     s->src = t ? t->tile_source : 0;
+    s->pl = 0;
+    if (t && t->plastic_pull != 0 && t->plastic_pull != 1)
+        return fail(FZ_E_ARG, "plastic_pull %d: must be 0 or 1", t->plastic_pull);
+    if (t && t->plastic_pull && e->mode == 2 && e->n_pl > 0 && e->ptp && e->pk) s->pl = 1;
     if (s->src < 0 || s->src > 2) return fail(FZ_E_ARG, "tile_source %d: must be 0, 1 or 2", s->src);
     if (s->src == 1 && (e->mode != 2 || !e->bk_off)) return fail(FZ_E_ARG, "tile_source 1 needs the tile engine");
     if (s->src == 1 && !e->bk4)
@@ -1466,9 +1539,12 @@ dim3 push_grid(const fz_engine* e, const Shape& s) {
     return dim3(gx, e->st.B);
 }
 
-TileArgs make_tile(const fz_engine* e, const fz_inputs* in, int K, int KP, int kb, int blk, int first,
+TileArgs make_tile(const fz_engine* e, const fz_inputs* in, int K, int KP, int kb, int blk, int first, int win,
                    const unsigned* hits, unsigned* raster) {
     TileArgs a{};
+    a.win = win;
+    a.nnz = e->nnz;
+    a.ptp = nullptr;  // set by the caller when the shape pulls plastic synapses
     a.blk = blk;
     a.clock = e->st.clock;
     a.N = e->n;
@@ -1526,28 +1602,37 @@ TileArgs make_tile(const fz_engine* e, const fz_inputs* in, int K, int KP, int k
     return a;
 }
 
-template <typename WT, int IPT, bool BK>
+template <typename WT, int IPT, bool BK, bool PL>
 void* tile_kernel_npt(int npt) {
     switch (npt) {
-        case 1: return reinterpret_cast<void*>(&lif_tile<WT, 1, IPT, BK>);
-        case 2: return reinterpret_cast<void*>(&lif_tile<WT, 2, IPT, BK>);
-        case 4: return reinterpret_cast<void*>(&lif_tile<WT, 4, IPT, BK>);
-        default: return reinterpret_cast<void*>(&lif_tile<WT, 8, IPT, BK>);
+        case 1: return reinterpret_cast<void*>(&lif_tile<WT, 1, IPT, BK, PL>);
+        case 2: return reinterpret_cast<void*>(&lif_tile<WT, 2, IPT, BK, PL>);
+        case 4: return reinterpret_cast<void*>(&lif_tile<WT, 4, IPT, BK, PL>);
+        default: return reinterpret_cast<void*>(&lif_tile<WT, 8, IPT, BK, PL>);
     }
 }
 
+template <typename WT, bool PL>
+void* tile_kernel_src(int npt, int ipt, int src) {
+    if (src == 1) return ipt == 1 ? tile_kernel_npt<WT, 1, true, PL>(npt) : tile_kernel_npt<WT, 4, true, PL>(npt);
+    return ipt == 1 ? tile_kernel_npt<WT, 1, false, PL>(npt) : tile_kernel_npt<WT, 4, false, PL>(npt);
+}
+
+// Plastic pulls are compiled for packed edges only (the format of every connectome whose tile
+// indices fit beside the weights); with the others plastic rows are pushed.
 template <typename WT>
-void* tile_kernel_fmt(int npt, int ipt, int src) {
-    if (src == 1) return ipt == 1 ? tile_kernel_npt<WT, 1, true>(npt) : tile_kernel_npt<WT, 4, true>(npt);
-    return ipt == 1 ? tile_kernel_npt<WT, 1, false>(npt) : tile_kernel_npt<WT, 4, false>(npt);
+void* tile_kernel_fmt(int npt, int ipt, int src, int pl) {
+    if constexpr (std::is_same<WT, PackedEdge>::value)
+        if (pl) return tile_kernel_src<WT, true>(npt, ipt, src);
+    return tile_kernel_src<WT, false>(npt, ipt, src);
 }
 
 // The lif_tile instantiation for an edge format (0: int16 col/w, 1: int32 col/w, 2: packed),
-// neurons per thread, staging depth and segment source.
-void* tile_kernel(int fmt, int npt, int ipt, int src) {
-    if (fmt == 2) return tile_kernel_fmt<PackedEdge>(npt, ipt, src);
-    if (fmt == 0) return tile_kernel_fmt<int16_t>(npt, ipt, src);
-    return tile_kernel_fmt<int32_t>(npt, ipt, src);
+// neurons per thread, staging depth, segment source and plastic pulls.
+void* tile_kernel(int fmt, int npt, int ipt, int src, int pl) {
+    if (fmt == 2) return tile_kernel_fmt<PackedEdge>(npt, ipt, src, pl);
+    if (fmt == 0) return tile_kernel_fmt<int16_t>(npt, ipt, src, 0);
+    return tile_kernel_fmt<int32_t>(npt, ipt, src, 0);
 }
 int tile_fmt(const fz_engine* e) { return e->pk ? 2 : (e->w_bits == 16 ? 0 : 1); }
 
@@ -1640,6 +1725,58 @@ __global__ void tile_pairs(int n, int n_tiles, const int* tb, int2* tp) {
     }
 }
 
+// Plastic CSR checks for the pair table: row pointers from 0 to n_pl without decreasing, targets
+// in [0, n) and non-decreasing within each row (so that each tile's targets form one segment).
+__global__ void pl_check(int n, int n_pl, const int* pl_indptr, const int* pl_post, int* bad) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int r0 = pl_indptr[i], r1 = pl_indptr[i + 1];
+        if ((i == 0 && r0 != 0) || (i == n - 1 && r1 != n_pl) || r1 < r0 || r0 < 0 || r1 > n_pl) {
+            *bad = 1;
+            continue;
+        }
+        for (int q = r0; q < r1; q++)
+            if ((unsigned)pl_post[q] >= (unsigned)n || (q > r0 && pl_post[q] < pl_post[q - 1])) *bad = 1;
+    }
+}
+
+// (first plastic edge, length) of every (row, tile) segment of the plastic CSR, [n][n_tiles].
+__global__ void tile_pl_pairs(int n, int n_tiles, int ts, const int* pl_indptr, const int* pl_post, int2* ptp) {
+    const long long total = (long long)n * n_tiles;
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+         q += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(q / n_tiles), t = (int)(q - (long long)i * n_tiles);
+        const int r0 = pl_indptr[i], r1 = pl_indptr[i + 1];
+        auto first_ge = [&](int v) {  // first edge of the row with target >= v
+            int lo = r0, hi = r1;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (pl_post[mid] < v) lo = mid + 1;
+                else hi = mid;
+            }
+            return lo;
+        };
+        const int lo = first_ge(t * ts), hi = first_ge((t + 1) * ts);
+        ptp[q] = make_int2(lo, hi - lo);
+    }
+}
+
+// Bucket entries that plastic segments add to each tile (as tile_seg_count for the fixed ones).
+__global__ void tile_pl_count(int n, int n_tiles, const int2* ptp, int* cnt) {
+    __shared__ int part[MAX_WAVES];
+    const int T = blockIdx.x;
+    int c = 0;
+    for (int i = blockIdx.y * blockDim.x + threadIdx.x; i < n; i += gridDim.y * blockDim.x)
+        c += (ptp[(size_t)i * n_tiles + T].y + BK_PIECE - 1) / BK_PIECE;
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += part[w];
+        if (t) atomicAdd(&cnt[T], t);
+    }
+}
+
 __global__ void tile_rel16(int n, int n_tiles, const int* indptr, const int* tb, uint16_t* tb16, int* over) {
     const long long total = (long long)n * (n_tiles + 1);
     for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
@@ -1649,6 +1786,47 @@ __global__ void tile_rel16(int n, int n_tiles, const int* indptr, const int* tb,
         if (d < 0 || d > 65535) *over = 1;
         tb16[q] = (uint16_t)d;
     }
+}
+
+// The plastic pair table, for plastic pulls: when the plastic CSR is sorted by target within rows
+// and the table takes at most TP_MAX_BYTES.  Otherwise there is none and every plastic row is
+// pushed at spike time.
+int plastic_pairs(fz_engine* e, int n_tiles, int tile) {
+    (void)hipFree(e->ptp);
+    e->ptp = nullptr;
+    const size_t np = (size_t)e->n * (size_t)n_tiles;
+    if (e->n_pl <= 0 || e->nnz + e->n_pl >= (1ll << 31) || np * sizeof(int2) > TP_MAX_BYTES) return FZ_OK;
+    int* bad = nullptr;
+    hipError_t r = hipMalloc(&bad, sizeof(int));
+    if (r == hipSuccess) r = hipMemset(bad, 0, sizeof(int));
+    int h = 1;
+    if (r == hipSuccess) {
+        const unsigned g = (unsigned)(e->n / 256 + 1 < 4096 ? e->n / 256 + 1 : 4096);
+        hipLaunchKernelGGL(pl_check, dim3(g), dim3(256), 0, 0, e->n, e->n_pl, e->pl_indptr, e->pl_post, bad);
+        r = hipGetLastError();
+    }
+    if (r == hipSuccess) r = hipMemcpy(&h, bad, sizeof(int), hipMemcpyDeviceToHost);
+    (void)hipFree(bad);
+    if (r != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(FZ_E_HIP, "checking the plastic CSR failed: %s", hipGetErrorString(r));
+    }
+    if (h) return FZ_OK;  // not sorted by target within rows: plastic rows are pushed
+    if (hipMalloc(&e->ptp, np * sizeof(int2)) != hipSuccess) {
+        (void)hipGetLastError();
+        e->ptp = nullptr;
+        return FZ_OK;
+    }
+    const unsigned g = (unsigned)(np / 256 + 1 < 65536 ? np / 256 + 1 : 65536);
+    hipLaunchKernelGGL(tile_pl_pairs, dim3(g), dim3(256), 0, 0, e->n, n_tiles, tile, e->pl_indptr, e->pl_post, e->ptp);
+    if ((r = hipGetLastError()) == hipSuccess) r = hipDeviceSynchronize();
+    if (r != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipFree(e->ptp);
+        e->ptp = nullptr;
+        return fail(FZ_E_HIP, "building the plastic pair table failed: %s", hipGetErrorString(r));
+    }
+    return FZ_OK;
 }
 
 // Bucket capacities: tile t can receive, per block, ceil(segment length / BK_PIECE) entries per
@@ -1669,6 +1847,8 @@ int bucket_layout(fz_engine* e, const int* tb, int n_tiles) {
     HIPCHK(hipMemset(cnt, 0, sizeof(int) * (size_t)n_tiles));
     const int gy = e->n / 256 + 1 < 64 ? e->n / 256 + 1 : 64;
     hipLaunchKernelGGL(tile_seg_count, dim3(n_tiles, gy), dim3(256), 0, 0, e->n, n_tiles, tb, cnt);
+    if (e->n_pl > 0 && e->ptp)  // the plastic segments that buckets carry for plastic pulls
+        hipLaunchKernelGGL(tile_pl_count, dim3(n_tiles, gy), dim3(256), 0, 0, e->n, n_tiles, e->ptp, cnt);
     std::vector<int> h((size_t)n_tiles);
     hipError_t r = hipGetLastError();
     if (r == hipSuccess) r = hipMemcpy(h.data(), cnt, sizeof(int) * (size_t)n_tiles, hipMemcpyDeviceToHost);
@@ -1681,7 +1861,8 @@ int bucket_layout(fz_engine* e, const int* tb, int n_tiles) {
     const long long maxspk = (e->kmax + per - 1) / per;
     for (int t = 0; t < n_tiles; t++) e->bk_off_h[t + 1] = e->bk_off_h[t] + (long long)h[t] * maxspk;
     e->bk_stride = e->bk_off_h[n_tiles];
-    e->bk4 = e->nnz < (1ll << 24) && e->kmax <= 32;  // first edge fits 24 bits beside length and step
+    // first edge (plastic ones after the nnz fixed ones) fits 24 bits beside length and step
+    e->bk4 = e->nnz + (e->n_pl > 0 && e->ptp ? e->n_pl : 0) < (1ll << 24) && e->kmax <= 32;
     // uint16 copy of the table (row offsets) when every row has fewer than 65,536 fixed edges
     (void)hipFree(e->tb16);
     e->tb16 = nullptr;
@@ -1747,6 +1928,12 @@ int ensure_buckets(fz_engine* e) {
     return FZ_OK;
 }
 
+void* bucket_kernel(const fz_engine* e, int pl) {
+    if (e->tb16)
+        return pl ? reinterpret_cast<void*>(&tile_bucket<true, true>) : reinterpret_cast<void*>(&tile_bucket<true, false>);
+    return pl ? reinterpret_cast<void*>(&tile_bucket<false, true>) : reinterpret_cast<void*>(&tile_bucket<false, false>);
+}
+
 BucketArgs make_bucket(const fz_engine* e, const fz_inputs* in, int K, int kb, int blk) {
     BucketArgs a{};
     a.blk = blk;
@@ -1762,6 +1949,8 @@ BucketArgs make_bucket(const fz_engine* e, const fz_inputs* in, int K, int kb, i
     a.spk_cnt = e->st.spk_cnt;
     a.tb = e->tb;
     a.tb16 = e->tb16;
+    a.ptp = nullptr;  // set by the caller when the shape pulls plastic synapses
+    a.nnz = e->nnz;
     a.indptr = e->indptr;
     a.bk_off = e->bk_off;
     a.bk_stride = e->bk_stride;
@@ -1793,22 +1982,22 @@ int enqueue(fz_engine* e, const fz_inputs* in, int steps, const unsigned* hits, 
     if (e->mode == 2) {
         const dim3 tgrid(e->n_tiles, e->st.B);
         const dim3 bgrid((e->n_tiles + 63) / 64, e->st.B);
-        void* fn = tile_kernel(tile_fmt(e), e->ts / s.ut, s.ipt, s.src);
+        void* fn = tile_kernel(tile_fmt(e), e->ts / s.ut, s.ipt, s.src, s.pl);
         if (s.src == 1 && (rc = ensure_buckets(e))) return rc;
         while (kb < steps) {
             const int K = (steps - kb) < s.K ? (steps - kb) : s.K;
             if (s.src == 1) {  // timed with the push timers: it is the spike expansion
                 BucketArgs ba = make_bucket(e, in, K, kb, blk);
+                if (s.pl) ba.ptp = e->ptp;
                 if ((rc = mark(ev_push))) return rc;
-                if (e->tb16)
-                    hipLaunchKernelGGL(tile_bucket<true>, bgrid, dim3(BK_THREADS), 0, stream, ba);
-                else
-                    hipLaunchKernelGGL(tile_bucket<false>, bgrid, dim3(BK_THREADS), 0, stream, ba);
+                void* bargs[] = {&ba};
+                HIPCHK(hipLaunchKernel(bucket_kernel(e, s.pl), bgrid, dim3(BK_THREADS), bargs, 0, stream));
                 HIPCHK(hipGetLastError());
                 if ((rc = mark(ev_push))) return rc;
             }
-            TileArgs ta = make_tile(e, in, K, tile_kp(e, s, K), kb, blk, kb == 0, hits, raster);
+            TileArgs ta = make_tile(e, in, K, tile_kp(e, s, K), kb, blk, kb == 0, steps, hits, raster);
             if (s.src == 2) ta.tp = nullptr;
+            if (s.pl) ta.ptp = e->ptp;
             void* args[] = {&ta};
             if ((rc = mark(ev_upd))) return rc;
             HIPCHK(hipLaunchKernel(fn, tgrid, dim3(s.ut), args, tile_lds(e, s, K), stream));
@@ -1975,6 +2164,7 @@ int fz_destroy(fz_engine* e) {
     (void)hipFree(e->bk_cnt);
     (void)hipFree(e->tb16);
     (void)hipFree(e->tp);
+    (void)hipFree(e->ptp);
     delete e;
     return FZ_OK;
 }
@@ -1987,6 +2177,15 @@ int fz_set_plastic(fz_engine* e, int n_pl, const int* pl_indptr, const int* pl_p
     e->n_pl = n_pl;
     e->pl_indptr = n_pl > 0 ? pl_indptr : nullptr;
     e->pl_post = n_pl > 0 ? pl_post : nullptr;
+    if (e->mode == 2) {  // the tile engine's plastic pair table and bucket capacities follow the new set
+        int rc = plastic_pairs(e, e->n_tiles, e->ts);
+        if (!rc) rc = bucket_layout(e, e->tb, e->n_tiles);
+        if (rc) {
+            e->mode = 1;
+            e->bound = false;
+            return rc;
+        }
+    }
     if (e->bound) {  // re-check capacity limits with the new edge count
         fz_state st = e->st;
         e->bound = false;
@@ -2101,20 +2300,21 @@ int fz_set_tiles(fz_engine* e, int tile, int n_tiles, const int* tb, const int* 
     for (int fmt : {0, 1, 2})
         for (int npt : {1, 2, 4, 8})
             for (int ipt : {1, 4})
-                for (int src : {0, 1}) {
-                    hipError_t r = hipFuncSetAttribute(tile_kernel(fmt, npt, ipt, src),
-                                                       hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                       (int)(e->lds_max - TILE_STATIC_LDS));
-                    if (r != hipSuccess) {
-                        (void)hipGetLastError();  // do not leave a sticky error for the caller's next HIP call
-                        return fail(FZ_E_HIP, "hipFuncSetAttribute(max dynamic LDS %zu) failed: %s",
-                                    e->lds_max - TILE_STATIC_LDS, hipGetErrorString(r));
+                for (int src : {0, 1})
+                    for (int pl : {0, 1}) {
+                        hipError_t r = hipFuncSetAttribute(tile_kernel(fmt, npt, ipt, src, pl),
+                                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                           (int)(e->lds_max - TILE_STATIC_LDS));
+                        if (r != hipSuccess) {
+                            (void)hipGetLastError();  // do not leave a sticky error for the caller's next HIP call
+                            return fail(FZ_E_HIP, "hipFuncSetAttribute(max dynamic LDS %zu) failed: %s",
+                                        e->lds_max - TILE_STATIC_LDS, hipGetErrorString(r));
+                        }
                     }
-                }
     int rc = validate_tiles(e, tb, n_tiles, tile);
     if (rc) return rc;
     // from here on a failure leaves the engine unbound, in push mode, until set up again
-    if ((rc = pack_edges(e, tile)) || (rc = bucket_layout(e, tb, n_tiles))) {
+    if ((rc = pack_edges(e, tile)) || (rc = plastic_pairs(e, n_tiles, tile)) || (rc = bucket_layout(e, tb, n_tiles))) {
         e->mode = 1;
         e->bound = false;
         return rc;
@@ -2241,7 +2441,7 @@ int fz_graph_create(fz_engine* e, const fz_inputs* in, int steps, const fz_tune*
     int kb = 0, blk = 0;
     std::vector<hipGraphNode_t> lasts;  // tile engine: the last node of each chain
     if (e->mode == 2) {
-        void* fn = tile_kernel(tile_fmt(e), e->ts / s.ut, s.ipt, s.src);
+        void* fn = tile_kernel(tile_fmt(e), e->ts / s.ut, s.ipt, s.src, s.pl);
         if (s.src == 1 && (rc = ensure_buckets(e))) {
             (void)hipGraphDestroy(gr->graph);
             delete gr;
@@ -2272,6 +2472,7 @@ int fz_graph_create(fz_engine* e, const fz_inputs* in, int steps, const fz_tune*
                 if (s.src == 1) {
                     BucketArgs ba = make_bucket(e, in, K, kb, blk);
                     ba.b0 = c.b0;
+                    if (s.pl) ba.ptp = e->ptp;
                     void* bargs[] = {&ba};
                     const int need = kb - e->c.n_delay + K - 1;  // last spike step it reads
                     int dep = blk - 2;
@@ -2282,8 +2483,7 @@ int fz_graph_create(fz_engine* e, const fz_inputs* in, int steps, const fz_tune*
                         }
                     if (dep >= blk) dep = blk - 1;
                     hipKernelNodeParams kp{};
-                    kp.func = e->tb16 ? reinterpret_cast<void*>(&tile_bucket<true>)
-                                      : reinterpret_cast<void*>(&tile_bucket<false>);
+                    kp.func = bucket_kernel(e, s.pl);
                     kp.gridDim = dim3((e->n_tiles + 63) / 64, c.nb);
                     kp.blockDim = dim3(BK_THREADS);
                     kp.sharedMemBytes = 0;
@@ -2293,8 +2493,9 @@ int fz_graph_create(fz_engine* e, const fz_inputs* in, int steps, const fz_tune*
                     if ((err = hipGraphAddKernelNode(&bnode, gr->graph, d ? &d : nullptr, d ? 1 : 0, &kp)) != hipSuccess)
                         return bail("hipGraphAddKernelNode(bucket)", err);
                 }
-                TileArgs ta = make_tile(e, in, K, tile_kp(e, s, K), kb, blk, kb == 0, nullptr, nullptr);
+                TileArgs ta = make_tile(e, in, K, tile_kp(e, s, K), kb, blk, kb == 0, steps, nullptr, nullptr);
                 if (s.src == 2) ta.tp = nullptr;
+                if (s.pl) ta.ptp = e->ptp;
                 ta.b0 = c.b0;
                 ta.nb = c.nb;
                 void* targs[] = {&ta};

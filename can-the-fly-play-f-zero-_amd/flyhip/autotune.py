@@ -5,14 +5,18 @@ chunk, push grid) with HIP events on real windows of the caller's own workload, 
 the same HIP graphs ``run`` uses, then restores the brain's state, so tuning never advances
 the simulation.  Search is coordinate descent:
 the update kernel's shape first, then the push kernel's, each over a small grid of shapes that
-respect the kernels' limits (multiples of 64 threads, chunk >= 64 edges).  Results are cached
-per (device, neurons, connections, flies, plastic synapses, delay steps) in memory and in a
-JSON file.
+respect the kernels' limits (multiples of 64 threads, chunk >= 64 edges).  The tile engine
+takes a full grid instead (and, with plastic synapses, each shape with plastic rows pushed and
+pulled).  Results are cached per (device, engine, neurons, connections, flies, plastic synapses,
+delay steps, activity) in memory and in a JSON file; the activity tag is log2(1 + expected
+Poisson kicks per step and fly) and whether any bias drive is on, so a shape tuned on a silent
+brain is not reused for a driven one.
 """
 
 from __future__ import annotations
 
 import json
+import math
 import os
 
 import torch
@@ -28,11 +32,17 @@ PUSH_CHUNK = (64, 128, 256, 512, 1024)
 PUSH_GRID_PER_CU = (2, 4, 8, 16)
 
 
+def _activity(bb) -> str:
+    kicks = float(bb.p_ext.sum()) / bb.B if bb.p_ext.numel() else 0.0
+    bias = bool((bb.bias_ext != 0).any()) if bb.bias_ext.numel() else False
+    return f"{int(math.log2(1.0 + kicks))}{'b' if bias else ''}"
+
+
 def _key(bb) -> str:
     name = torch.cuda.get_device_name(bb.device)
     eng = bb.engine + (str(bb.tile) if bb.engine == "tile" else "")
     return (f"{name}|{eng}|n={bb.n}|nnz={int(bb.d_col.numel())}|B={bb.B}|npl={len(bb.plastic_pos)}"
-            f"|nd={bb.n_delay}")
+            f"|nd={bb.n_delay}|act={_activity(bb)}")
 
 
 def _state(bb):
@@ -99,21 +109,25 @@ def tune(bb, duration_ms: float = 1000.0 / 60.0988, reps: int = 2, use_cache: bo
     _time(bb, {}, steps, 1, snap)
     if bb.engine == "tile":
         # full grid (small): threads per workgroup (1, 2, 4 or 8 neurons per thread) x block length
-        # x staging depth x segment source (table lookups or buckets); shapes over the LDS budget
-        # are skipped
+        # x staging depth x segment source (table lookups or buckets) x, with plastic synapses,
+        # plastic rows pushed or pulled; shapes over the LDS budget are skipped
         best_t, cur = float("inf"), None
+        pulls = (0, 1) if len(bb.plastic_pos) else (0,)
         for ut in (bb.tile // k for k in (8, 4, 2, 1)):
             if not 64 <= ut <= 1024:
                 continue
             for ks in sorted({bb.n_delay, max(1, bb.n_delay // 2)}):
                 for stage in (1, 4):
                     for src in (0, 1):
-                        trial = {"upd_threads": ut, "block_steps": ks, "tile_stage": stage, "tile_source": src}
-                        t = _time(bb, trial, steps, reps, snap)
-                        if verbose:
-                            print(f"autotune {trial}: {t * 1000:.1f} us", flush=True)
-                        if t < best_t:
-                            best_t, cur = t, trial
+                        for pull in pulls:
+                            trial = {"upd_threads": ut, "block_steps": ks, "tile_stage": stage, "tile_source": src}
+                            if pull:
+                                trial["plastic_pull"] = 1
+                            t = _time(bb, trial, steps, reps, snap)
+                            if verbose:
+                                print(f"autotune {trial}: {t * 1000:.1f} us", flush=True)
+                            if t < best_t:
+                                best_t, cur = t, trial
         space = []
     else:
         cur = {"upd_threads": 256, "push_threads": 256, "push_chunk": 256, "push_grid": n_cu * 8}

@@ -208,7 +208,18 @@ class Fleet:
             from . import autotune
 
             for part in self.parts:
+                # tuned on a driven workload: the eye sees a coarse random scene and the same scene
+                # moved by 3 pixels (motion everywhere, as in a race); then its filters restart
+                # and p_ext is cleared, so the fleet's first frame is untouched
+                rng = np.random.default_rng(seed + part.b0)
+                h, w = FRAME_SHAPE
+                scene = rng.integers(0, 256, (part.nb, -(-h // 8), -(-w // 8), 3), dtype=np.uint8)
+                scene = np.repeat(np.repeat(scene, 8, axis=1), 8, axis=2)[:, :h, :w]
+                for shift in (0, 3):
+                    part.see(np.ascontiguousarray(np.roll(scene, shift, axis=2)))
                 autotune.tune(part.brain)
+                part.brain.p_ext.zero_()
+                part.eye.reset()
 
     def close(self):
         close = getattr(self.pool, "close", None)

@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define FZ_ABI_VERSION 1
+#define FZ_ABI_VERSION 2
 
 #define FZ_OK 0
 #define FZ_E_ARG (-1)      /* invalid argument (null pointer, size out of range) */
@@ -88,6 +88,14 @@ typedef struct fz_tune {
                         * scratch, allocated on first use, 2 x 4 B x pieces bound per fly; needs
                         * fewer than 2^24 fixed edges; 2 = as 0 from the caller's int32 table (or
                         * its uint16 row-offset copy): three loads per spike and tile */
+    int plastic_pull;  /* tile engine with plastic synapses: 0 = every spiking plastic row is pushed
+                        * at spike time into the arrival ring (global atomics); 1 = a spike whose
+                        * arrival lies in the window it was emitted in is pulled by the target
+                        * tiles with the fly's weights (plastic edges ride in the segment lookups
+                        * and buckets), only the others are pushed.  Arrivals are the same; pushes
+                        * are global atomics per plastic edge, pulls cost a table load per spike
+                        * and tile.  1 needs packed edges and a plastic CSR sorted by target within
+                        * rows (checked by fz_set_tiles); otherwise plastic rows are pushed. */
 } fz_tune;
 
 int fz_version(void);

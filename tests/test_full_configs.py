@@ -110,7 +110,9 @@ def test_b64_reward_plasticity_on_7387_synapses_matches_oracles(full_fly):
     idx = np.r_[fly.input_idx, fly.drive_idx]
     gb = BatchBrain(fly.weights, p, batch=B, seed=3, plastic_pos=pos, engine="tile", tile=1024)
     gb.set_input(idx, rates[0])
-    gb.set_tune(**autotune.tune(gb, use_cache=False, reps=1))
+    shape = dict(autotune.tune(gb, use_cache=False, reps=1))
+    shape["plastic_pull"] = 1                                    # plastic synapses pulled in-window
+    gb.set_tune(**shape)
     ob = OracleBrain(fly.weights, p, batch=B, seed=3, plastic_pos=pos, frac_bits=gb.frac_bits)
     ob.alive = np.isin(np.arange(B), sample).astype(np.uint8)
     lp = LearnParams(tau_elig_ms=200.0, eta=5e-2, w_abs_max=50.0)

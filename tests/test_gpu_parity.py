@@ -343,15 +343,16 @@ def test_full_connectome_raster_1000_steps_and_one_second_of_frames(full_fly, en
     _assert_state_equal(gb, ob)
 
 
+@pytest.mark.parametrize("pull", [0, 1])
 @pytest.mark.parametrize("source", [0, 1, 2])
-def test_tile_sources_give_identical_rasters_and_plastic_arrivals(small_fly, source):
+def test_tile_sources_give_identical_rasters_and_plastic_arrivals(small_fly, source, pull):
     # per-tile buckets versus segment-table lookups: same rasters, counts and state as the
     # oracle, with plastic synapses (pushed through the ring) and a retired fly
     fly = small_fly
     rng = np.random.default_rng(13)
     pos = np.sort(rng.choice(fly.weights.nnz, 3000, replace=False))
     gb, ob = _pair(fly, batch=3, seed=4, plastic_pos=pos, engine="tile", tile=1024)
-    gb.set_tune(upd_threads=512, tile_source=source)
+    gb.set_tune(upd_threads=512, tile_source=source, plastic_pull=pull)
     gb.pw[1] *= 1.5
     ob.pw[1] *= 1.5
     gb.alive[2] = 0
@@ -369,6 +370,34 @@ def test_tile_sources_give_identical_rasters_and_plastic_arrivals(small_fly, sou
         np.testing.assert_array_equal(_np(gb.run(FRAME_MS)), ob.run(FRAME_MS), err_msg=f"frame {frame}")
     np.testing.assert_array_equal(_np(gb.u)[:2], ob.u[:2])
     np.testing.assert_array_equal(_np(gb.g)[:2], ob.g[:2])
+
+
+@pytest.mark.parametrize("pull", [0, 1])
+@pytest.mark.parametrize("source", [0, 1, 2])
+def test_plastic_arrivals_carry_the_weights_of_the_window_they_were_emitted_in(small_fly, source, pull):
+    # With plastic pulls the tile engine pulls a plastic synapse at its target when the arrival
+    # falls in the window the spike was emitted in, and pushes it at spike time otherwise;
+    # without, every plastic row is pushed at spike time.  Windows of 2 to 166 steps,
+    # several shorter than n_delay = 18 (an arrival can land two windows later), eager and as a
+    # graph, with new plastic weights before every window: each window's counts and the final
+    # state equal the oracle's, which pushes every spike with the weights of its own window.
+    fly = small_fly
+    rng = np.random.default_rng(17)
+    pos = np.sort(rng.choice(fly.weights.nnz, 6000, replace=False))
+    gb, ob = _pair(fly, batch=2, seed=9, plastic_pos=pos, engine="tile", tile=1024)
+    gb.set_tune(upd_threads=512, tile_source=source, plastic_pull=pull)
+    base = ob.pw.copy()
+    total = 0
+    for wi, steps in enumerate([40, 5, 3, 17, 18, 19, 2, 166, 36, 9, 30, 166]):
+        pw = (base * rng.uniform(0.0, 4.0, base.shape)).astype(np.float32)
+        gb.pw.copy_(torch.from_numpy(pw))
+        ob.pw[...] = pw
+        cg = gb.run(FRAME_MS) if steps == 166 else gb.run_steps(steps)
+        co = ob.run_steps(steps)
+        np.testing.assert_array_equal(_np(cg), co, err_msg=f"window {wi} ({steps} steps)")
+        total += int(co.sum())
+    assert total > 2000
+    _assert_state_equal(gb, ob)
 
 
 @pytest.mark.parametrize("t_ref", [0.5, 1.8])
