@@ -622,7 +622,7 @@ def main():
         sys.exit(2)
 
     import torch
-    from flyhip import LIFParams
+    from flyhip import LIFParams, _lib
     from flyhip.connectome import flywire_like
 
     t = time.perf_counter()
@@ -760,7 +760,8 @@ def main():
             "roofline": best["roofline"], "cpu_baseline": cpu, "batched": batched, "surrogate": surrogate,
             "plastic": plastic,
             "instruct": instruct, "eye": eye, "fleet": fleet,
-            "b1_candidates": [{k: r[k] for k in ("engine", "value", "ms_per_frame")} for r in lines1]}
+            "b1_candidates": [{k: r[k] for k in ("engine", "value", "ms_per_frame")} for r in lines1],
+            "library": _lib.library_info()}
     print(json.dumps(line), flush=True)
 
 

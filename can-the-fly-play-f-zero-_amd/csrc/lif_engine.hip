@@ -2104,6 +2104,11 @@ extern "C" {
 
 int fz_version(void) { return FZ_ABI_VERSION; }
 
+#ifndef FZ_SRC_HASH
+#define FZ_SRC_HASH "unknown"
+#endif
+const char* fz_build_info(void) { return "src " FZ_SRC_HASH "; " __VERSION__; }
+
 const char* fz_last_error(void) { return g_err.c_str(); }
 
 int fz_create(int n, long long nnz, const int* indptr, const int* col, const void* w, int w_bits, int frac_bits,

@@ -48,6 +48,7 @@ class FzTune(ctypes.Structure):
 # name -> (restype, argtypes); every symbol declared in include/flyhip.h
 SIGNATURES = {
     "fz_version": (c_int, []),
+    "fz_build_info": (ctypes.c_char_p, []),
     "fz_last_error": (ctypes.c_char_p, []),
     "fz_create": (c_int, [c_int, ctypes.c_longlong, c_void_p, c_void_p, c_void_p, c_int, c_int,
                           ctypes.POINTER(FzParams), ctypes.POINTER(c_void_p)]),
@@ -131,6 +132,21 @@ def load() -> ctypes.CDLL:
 
 class FlyHipError(RuntimeError):
     pass
+
+
+def library_info() -> dict:
+    """The native library this process loaded: its path, the SHA-256 of the file, the source hash
+    compiled into it and whether that equals the hash of the sources present now (the library
+    is built in the development tree and travels with it; nothing rebuilds it on load)."""
+    import hashlib
+
+    from .build import source_hash
+
+    lib = load()
+    built = lib.fz_build_info().decode(errors="replace")
+    src_now = source_hash()
+    return {"path": LIB_PATH, "sha256": hashlib.sha256(open(LIB_PATH, "rb").read()).hexdigest()[:16],
+            "build_info": built, "sources_now": src_now, "matches_sources": built.startswith(f"src {src_now};")}
 
 
 def check(rc: int, what: str = ""):

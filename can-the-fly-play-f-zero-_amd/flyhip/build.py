@@ -5,6 +5,7 @@
 
 from __future__ import annotations
 
+import hashlib
 import os
 import subprocess
 import sys
@@ -35,10 +36,21 @@ def sources() -> list[str]:
     return [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
 
 
+def source_hash() -> str:
+    """SHA-256 (first 16 hex digits) of every file the library is compiled from: the HIP
+    sources, the internal header and the C-ABI header, in a fixed order."""
+    h = hashlib.sha256()
+    for path in sorted(sources() + [os.path.join(CSRC, "errors.h"), os.path.join(INCLUDE, "flyhip.h")]):
+        if os.path.exists(path):
+            h.update(os.path.basename(path).encode())
+            h.update(open(path, "rb").read())
+    return h.hexdigest()[:16]
+
+
 def build(verbose: bool = False) -> str:
     """Compile every HIP source into ``libflyhip.so`` (always recompiles)."""
     tmp = LIB_PATH + ".tmp"
-    cmd = [HIPCC, *FLAGS, f"-I{INCLUDE}", *sources(), "-o", tmp]
+    cmd = [HIPCC, *FLAGS, f'-DFZ_SRC_HASH="{source_hash()}"', f"-I{INCLUDE}", *sources(), "-o", tmp]
     if verbose:
         print(" ".join(cmd), file=sys.stderr)
     r = subprocess.run(cmd, capture_output=True, text=True)

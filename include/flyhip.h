@@ -100,6 +100,10 @@ typedef struct fz_tune {
 
 int fz_version(void);
 const char* fz_last_error(void);
+/* Which build is loaded: "src <hash>; <compiler version>", the hash covering the HIP sources and
+ * headers the library was compiled from (no counterpart in the reference: for callers and
+ * harnesses that must show which native code ran). */
+const char* fz_build_info(void);
 
 /* Engine over a device CSR of the FIXED (non-plastic) synapses.  Replaces the
  * device upload in flyzero/gpu.py:143-163.  indptr int32[n+1], col int32[nnz],

@@ -50,3 +50,11 @@ def test_argument_errors_are_codes_not_crashes():
     assert lib.fz_destroy(None) == _lib.FZ_OK
     with pytest.raises(ValueError):
         _lib.check(_lib.FZ_E_ARG, "probe")
+
+
+def test_loaded_library_was_built_from_these_sources():
+    # the library carries the hash of the sources it was compiled from; a stale build (sources
+    # edited, library not rebuilt) is reported here instead of being tested silently
+    info = _lib.library_info()
+    assert info["build_info"].startswith("src ")
+    assert info["matches_sources"], info
