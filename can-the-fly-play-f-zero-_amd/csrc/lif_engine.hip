@@ -69,7 +69,8 @@ constexpr int KMAX = 32;          // steps per block held in registers
 constexpr int MAX_THREADS = 1024;
 constexpr int MAX_WAVES = MAX_THREADS / 64;
 constexpr int QMIN = 64;          // smallest push chunk (sizes the chunk-start table)
-constexpr int BK_PIECE = 4;  // edges per tile-bucket entry at most (longer segments are split)
+constexpr int BK_PIECE = 2;  // edges per tile-bucket entry at most (longer segments are split)
+constexpr int BK_ROUND = 4;  // tile-bucket entries per thread and round in lif_tile
 constexpr size_t TILE_STATIC_LDS = 1024;  // room left for lif_tile's static shared arrays
 constexpr size_t TP_MAX_BYTES = 1ull << 30;  // largest (first edge, length) pair table kept
 constexpr int PL_SERIAL = 2;  // plastic rows up to this length are pushed by their own thread
@@ -668,26 +669,20 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
             }
         return;
     }
-    // the tile's neuron state, loaded first so that its latency overlaps the pull
+    // The tile's neuron state, loaded first so that its latency overlaps the pull.  Every load
+    // is issued from a valid (clamped) address and nothing here waits for one; the loads that
+    // need a neuron's ext slot (its Poisson probability and bias) follow the next barrier.  The
+    // values of neurons past the end of the last tile are never used.
     float u_[NPT], g_[NPT], bias_[NPT], pp_[NPT];
     int r_[NPT], j_[NPT];
 #pragma unroll
     for (int p = 0; p < NPT; p++) {
         const int i = i0 + p * TPB + tid;
-        u_[p] = g_[p] = bias_[p] = pp_[p] = 0.f;
-        r_[p] = 0;
-        j_[p] = -1;
-        if (i < a.N) {
-            u_[p] = a.u[bn + i];
-            g_[p] = a.g[bn + i];
-            r_[p] = a.ref[bn + i];
-            j_[p] = a.ext_idx ? a.ext_idx[i] : -1;
-            if (j_[p] >= 0) {
-                const size_t e = (size_t)b * a.n_ext + j_[p];
-                if (a.bias_ext) bias_[p] = a.bias_ext[e];
-                if (a.p_ext) pp_[p] = a.p_ext[e];
-            }
-        }
+        const int ic = i < a.N ? i : a.N - 1;
+        j_[p] = a.ext_idx ? a.ext_idx[ic] : -1;
+        u_[p] = a.u[bn + ic];
+        g_[p] = a.g[bn + ic];
+        r_[p] = a.ref[bn + ic];
     }
     // KP * TS is a multiple of 4 ints (TS is a multiple of 64): 16-byte stores
     for (int q = tid; q < (KP * TS) >> 2; q += TPB) reinterpret_cast<int4*>(acc)[q] = make_int4(0, 0, 0, 0);
@@ -708,6 +703,17 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
     } else {
         source_offsets(cnt_b, t0, lt0, a.K, nd, M, a.N, s_off);
         total = s_off[a.K];
+    }
+    // the ext-slot loads, also from valid addresses (a neuron without a slot reads slot 0); the
+    // slot test is applied where the values are used
+#pragma unroll
+    for (int p = 0; p < NPT; p++) {
+        bias_[p] = pp_[p] = 0.f;
+        if (a.ext_idx && a.n_ext > 0) {
+            const size_t e = (size_t)b * a.n_ext + (j_[p] >= 0 ? j_[p] : 0);
+            if (a.bias_ext) bias_[p] = a.bias_ext[e];
+            if (a.p_ext) pp_[p] = a.p_ext[e];
+        }
     }
     const WT* w = static_cast<const WT*>(a.w);  // unused with packed edges
     const int lmask = (1 << a.lbits) - 1;
@@ -730,13 +736,43 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
             *val = wval(w[ek], a.frac_bits);
         }
     };
-    if constexpr (BK) {
-        // Bucket pieces hold at most BK_PIECE edges (tile segments are short: ~1 edge per
-        // (row, tile) on a FlyWire-shaped graph), so each thread takes whole pieces: no scan
-        // and no search.  The two entry loads and the first edge of each piece (most pieces
-        // are one edge) are issued unconditionally from clamped, valid indices and masked
-        // afterwards, so they are in flight together; the rare further edges are loaded on
-        // demand.
+    if constexpr (BK && !PL && std::is_same<WT, PackedEdge>::value && BK_PIECE == 2) {
+        // Bucket pieces hold at most two edges (tile segments are short: ~1 edge per (row, tile)
+        // on a FlyWire-shaped graph), so each thread takes whole pieces: no scan and no search.
+        // Rounds of BK_ROUND entries per thread: every entry load first, then both edges of every
+        // piece (the second from the next packed edge, a valid address even past the last fixed
+        // edge: the packed array ends with one padding entry), then the LDS atomics -- two memory
+        // round trips per round and no wait in between.
+        for (int e0 = tid; e0 < total; e0 += BK_ROUND * TPB) {
+            int v[BK_ROUND];
+#pragma unroll
+            for (int r = 0; r < BK_ROUND; r++) {
+                const int e = e0 + r * TPB;
+                v[r] = bkp4[e < total ? e : e0];  // first edge << 7 | (len - 1) << 5 | step
+            }
+            int x0[BK_ROUND], x1[BK_ROUND];
+#pragma unroll
+            for (int r = 0; r < BK_ROUND; r++) {
+                x0[r] = a.pk[v[r] >> 7];
+                x1[r] = a.pk[(v[r] >> 7) + 1];
+            }
+#pragma unroll
+            for (int r = 0; r < BK_ROUND; r++) {
+                if (e0 + r * TPB >= total) continue;
+                const int slot = v[r] & 31;
+                const int v0 = x0[r] >> a.lbits, v1 = x1[r] >> a.lbits;
+                if (v0 != 0) atomicAdd(&acc[__umul24((unsigned)(x0[r] & lmask), (unsigned)KP) + slot], v0);
+                if ((v[r] & (1 << 5)) && v1 != 0)
+                    atomicAdd(&acc[__umul24((unsigned)(x1[r] & lmask), (unsigned)KP) + slot], v1);
+            }
+        }
+        // no barrier: the plastic arrivals and Poisson hits below are LDS atomics too, and one
+        // barrier before the step loop orders them all
+    } else if constexpr (BK) {
+        // The general form (any piece length, plastic edges, unpacked formats): the two entry
+        // loads and the first edge of each piece are issued unconditionally from clamped, valid
+        // indices and masked afterwards, so they are in flight together; the further edges of a
+        // piece are loaded on demand.
         for (int e0 = tid; e0 < total; e0 += 2 * TPB) {
             int v[2];
             bool ok[2];
@@ -915,7 +951,7 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
     // (neuron, step, seed + fly), or the precomputed hit bits.
 #pragma unroll
     for (int p = 0; p < NPT; p++) {
-        const bool want = j_[p] >= 0 && (a.hits || pp_[p] > 0.f);
+        const bool want = i0 + p * TPB + tid < a.N && j_[p] >= 0 && (a.hits || pp_[p] > 0.f);
         const unsigned long long wm = __ballot(want);
         const int ne = __popcll(wm);  // wavefront-uniform
         if (ne == 0) continue;
@@ -970,7 +1006,7 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
             const uint32_t hits = (uint32_t)row[a.K];
             float u = u_[p], g = g_[p];
             int r = r_[p];
-            const float bias = bias_[p];
+            const float bias = j_[p] >= 0 ? bias_[p] : 0.f;
             if (once) {
                 int rend;
                 // the block lengths of dt = 0.1 ms and 0.25 ms (n_delay 18 and 7) unrolled
@@ -2251,8 +2287,12 @@ int pack_edges(fz_engine* e, int tile) {
     while ((1 << lbits) < tile) lbits++;
     int* pk = nullptr;
     int* over = nullptr;
-    if (hipMalloc(&pk, (size_t)e->nnz * sizeof(int)) != hipSuccess) {
+    // one padding entry (zero) after the last edge: the tile kernel reads the edge after a
+    // piece's first one unconditionally
+    if (hipMalloc(&pk, ((size_t)e->nnz + 1) * sizeof(int)) != hipSuccess ||
+        hipMemset(pk + e->nnz, 0, sizeof(int)) != hipSuccess) {
         (void)hipGetLastError();
+        (void)hipFree(pk);
         return fail(FZ_E_HIP, "allocating %lld packed edges failed", e->nnz);
     }
     if (hipMalloc(&over, sizeof(int)) != hipSuccess || hipMemset(over, 0, sizeof(int)) != hipSuccess) {
