@@ -57,10 +57,15 @@ PROFILE = os.path.join(REPO, "profiles", "lif_tile_flywire138k.md")
 
 
 def _same_shape(a: str, b: str) -> bool:
-    """Launch specs engine:B:upd_threads:block_steps:tile_stage:tile_source; the staging depth
-    only changes the segment-table source, so it is ignored for the bucket source."""
+    """Launch specs engine:B:upd_threads:block_steps:tile_stage:tile_source[:block_parts] (parts 1
+    when absent); the staging depth only changes the segment-table source, so it is ignored for
+    the bucket source."""
     x, y = a.split(":"), b.split(":")
-    if len(x) != 6 or len(y) != 6:
+    if len(x) == 6:
+        x.append("1")
+    if len(y) == 6:
+        y.append("1")
+    if len(x) != 7 or len(y) != 7:
         return a == b
     if x[5] == y[5] == "1":
         x[4] = y[4] = "*"
@@ -126,8 +131,9 @@ def profile_record(tag: str, spec):
 def shape_spec(engine, tile, B, shape):
     if engine != "tile":
         return None
+    parts = shape.get("block_parts", 0) or 1
     return (f"tile{tile}:{B}:{shape.get('upd_threads', 0)}:{shape.get('block_steps', 0)}:"
-            f"{shape.get('tile_stage', 0)}:{shape.get('tile_source', 0)}")
+            f"{shape.get('tile_stage', 0)}:{shape.get('tile_source', 0)}" + (f":{parts}" if parts != 1 else ""))
 
 
 def measure_line(fly, p, B, engine, tile, frames, warmup, torch, gain=None, drive=None, seed=0,
@@ -556,8 +562,8 @@ def surrogate_line(torch, sim_fly_frames_per_s, members=3, batch=4096, frames=50
 def profile_run(fly, p, spec, torch, frames=3):
     """Counter runs (rocprofv3 --pmc): one engine configuration for a few eager frames at a fixed
     launch shape, then a plain 1 GiB device-to-device copy whose bytes are known, which scales
-    the counters.  spec = engine:B:upd_threads:block_steps:tile_stage[:tile_source], e.g.
-    tile1024:1:256:18:1."""
+    the counters.  spec = engine:B:upd_threads:block_steps:tile_stage[:tile_source[:block_parts]],
+    e.g. tile1024:1:256:18:1."""
     from flyhip import BatchBrain
 
     eng, B, ut, K, stage, *src = spec.split(":")
@@ -570,7 +576,8 @@ def profile_run(fly, p, spec, torch, frames=3):
     idx = np.r_[fly.input_idx, fly.drive_idx]
     j = bb.slots(idx)
     probs = torch.from_numpy(frame_rates(fly, frames + 2, gain, drive)).cuda() * float(np.float32(p.dt / 1000.0))
-    bb.set_tune(upd_threads=int(ut), block_steps=int(K), tile_stage=int(stage), tile_source=int(src[0]) if src else 0)
+    bb.set_tune(upd_threads=int(ut), block_steps=int(K), tile_stage=int(stage), tile_source=int(src[0]) if src else 0,
+                block_parts=int(src[1]) if len(src) > 1 else 0)
     deg = np.diff(bb.indptr).astype(np.float64)
     steps = bb.steps_of(FRAME_MS)
     spikes = edges = 0.0
@@ -614,7 +621,8 @@ def main():
     ap.add_argument("--fleet-frames", type=int, default=60, help="frames of the B=256 fleet line (0: skip)")
     ap.add_argument("--instruct-frames", type=int, default=100,
                     help="frames of the B=64 instructed-learning line (0: skip)")
-    ap.add_argument("--profile", default="", help="counter-run mode: engine:B:upd_threads:block_steps:tile_stage[:tile_source]")
+    ap.add_argument("--profile", default="",
+                    help="counter-run mode: engine:B:upd_threads:block_steps:tile_stage[:tile_source[:block_parts]]")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the LIF path shards flies across GPUs with no exchange, and "

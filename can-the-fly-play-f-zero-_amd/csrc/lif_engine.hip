@@ -448,6 +448,7 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_push(PushArgs<WT> a) {
 struct TileArgs {
     const long long* clock;
     int N, B, K, KP, kb, blk, first, TS, n_tiles, nd, M, ring_slots;
+    int kh;                // steps of the block's first part (K: the block runs as one part)
     float* u;
     float* g;
     signed char* ref;
@@ -487,10 +488,11 @@ struct TileArgs {
     int win;               // steps of this window
     const int* indptr;
     const int* bk;         // tile buckets (source 1): [2][B][bk_stride] entries (first edge << 7 |
-                           // (len - 1) << 5 | step), two buffers alternating with the block parity
+                           // (len - 1) << 5 | step), two buffers alternating with the block parity;
+                           // entries of steps < kh from a bucket's front, the others from its back
     const long long* bk_off;  // [n_tiles + 1] offset of each tile's bucket in a fly's arena
     long long bk_stride;
-    const int* bk_cnt;     // [B][n_tiles] entries in each bucket
+    const int* bk_cnt;     // [2][B][n_tiles][2] front and back entries of each bucket
     int b0, nb;            // this launch covers flies [b0, b0 + nb) (grid n_tiles x nb)
     Consts c;
 };
@@ -639,10 +641,10 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
     __shared__ int s_wsum[MAX_WAVES + 1];
     const int TPB = blockDim.x, tid = threadIdx.x, lane = tid & 63;
     const int TS = TPB * NPT;
-    // acc[neuron][KP]: a neuron's K arrivals are adjacent (two steps per ds_read2 with immediate
-    // offsets in the step loop), and column K holds its Poisson hits as a K-bit mask.  KP >= K + 1
-    // is odd when that fits, so the 64 lanes of a wavefront hit 64 different banks.  Offsets are
-    // 24-bit products (neuron < 8192, KP <= 33): full-rate multiplies.
+    // acc[neuron][KP]: a neuron's arrivals of the block part are adjacent (two steps per ds_read2
+    // with immediate offsets in the step loop), and column kh holds its Poisson hits as a K-bit
+    // mask.  KP >= kh + 1 is odd when that fits, so the 64 lanes of a wavefront hit 64 different
+    // banks.  Offsets are 24-bit products (neuron < 8192, KP <= 33): full-rate multiplies.
     const int KP = a.KP;
     int* acc = lds;                      // [TS][KP]
     int* st_seg = acc + KP * TS;         // [TPB * IPT] (segment-table source only)
@@ -653,26 +655,24 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
     b += a.b0;
     const int i0 = T * TS;
     const size_t bn = (size_t)b * a.N;
-    const long long t0 = a.clock[0] + a.kb;
-    const uint32_t step0 = (uint32_t)a.clock[1] + (uint32_t)a.kb;
-    const int nd = a.nd, M = a.M;
-    const int lt0 = step_slot(t0, M);                      // list slot of step t0
-    const int rs0 = step_slot(t0, a.ring_slots);
-    int* cnt_b = a.spk_cnt + (size_t)b * M;
-    int* idx_b = a.spk_idx + (size_t)b * M * a.N;
-    if (T == 0 && tid < nd) cnt_b[(lt0 + a.K + tid) % M] = 0;   // lists of the next n_delay steps
-    if (a.alive && !a.alive[b]) {                               // retired fly
-        if (a.first)
-            for (int p = 0; p < NPT; p++) {
-                const int i = i0 + p * TPB + tid;
-                if (i < a.N) a.counts[bn + i] = 0;
-            }
-        return;
+    // Every load the set-up needs is issued before any of them is used -- the clock, the fly's
+    // alive flag, the bucket's place and both parities' entry counts, and the tile's neuron state
+    // -- so their latencies overlap (a retired fly's state loads are wasted).  The state loads
+    // come from valid (clamped) addresses and nothing waits for them until the step loop; the
+    // loads that need a neuron's ext slot (its Poisson probability and bias) follow the next
+    // barrier.  The values of neurons past the end of the last tile are never used.
+    const long long clk0 = a.clock[0], clk2 = a.clock[2];
+    const uint32_t clk1 = (uint32_t)a.clock[1];
+    const bool live = a.alive ? a.alive[b] != 0 : true;
+    long long bko0 = 0, bko1 = 0;
+    int2 bkc[2] = {make_int2(0, 0), make_int2(0, 0)};
+    if constexpr (BK) {
+        bko0 = a.bk_off[T];
+        bko1 = a.bk_off[T + 1];
+        const int2* bc = reinterpret_cast<const int2*>(a.bk_cnt);
+        bkc[0] = bc[(size_t)b * a.n_tiles + T];
+        bkc[1] = bc[((size_t)a.B + b) * a.n_tiles + T];
     }
-    // The tile's neuron state, loaded first so that its latency overlaps the pull.  Every load
-    // is issued from a valid (clamped) address and nothing here waits for one; the loads that
-    // need a neuron's ext slot (its Poisson probability and bias) follow the next barrier.  The
-    // values of neurons past the end of the last tile are never used.
     float u_[NPT], g_[NPT], bias_[NPT], pp_[NPT];
     int r_[NPT], j_[NPT];
 #pragma unroll
@@ -684,6 +684,23 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
         g_[p] = a.g[bn + ic];
         r_[p] = a.ref[bn + ic];
     }
+    const long long t0 = clk0 + a.kb;
+    const uint32_t step0 = clk1 + (uint32_t)a.kb;
+    const int nd = a.nd, M = a.M;
+    const int lt0 = step_slot(t0, M);                      // list slot of step t0
+    const int rs0 = step_slot(t0, a.ring_slots);
+    int* cnt_b = a.spk_cnt + (size_t)b * M;
+    int* idx_b = a.spk_idx + (size_t)b * M * a.N;
+    if (T == 0 && tid < nd) cnt_b[(lt0 + a.K + tid) % M] = 0;   // lists of the next n_delay steps
+    if (!live) {                                                // retired fly
+        if (a.first)
+            for (int p = 0; p < NPT; p++) {
+                const int i = i0 + p * TPB + tid;
+                if (i < a.N) a.counts[bn + i] = 0;
+            }
+        return;
+    }
+    if (tid < KMAX) s_lcnt[tid] = 0;  // spikes per block step (read after the step loop)
     // KP * TS is a multiple of 4 ints (TS is a multiple of 64): 16-byte stores
     for (int q = tid; q < (KP * TS) >> 2; q += TPB) reinterpret_cast<int4*>(acc)[q] = make_int4(0, 0, 0, 0);
     // per-step keys of the Poisson draw (read after the next barrier)
@@ -691,18 +708,20 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
     const uint32_t hb = (uint32_t)b + a.fly_base;  // the fly's number in the draw
     const uint32_t hseed = mix32(a.seed + hb);
     if (tid < a.K) s_sk[tid] = mix32((step0 + (uint32_t)tid) * 0x9e3779b9U ^ hseed);
+    // Source 1: the tile's bucket holds the entries of the block's first part at its front and
+    // those of its second part at its back (an undivided block has only the front part).
     const int* bkp4 = nullptr;  // this tile's bucket: entry e at bkp4[e]
-    int total;
+    int bcap = 0, bcf = 0, bcb = 0;  // bucket capacity, front and back entries
     if constexpr (BK) {
-        const int par = (int)((a.clock[2] + a.blk) & 1);
-        const long long cap = a.bk_off[T + 1] - a.bk_off[T];
-        const int c = a.bk_cnt[((size_t)par * a.B + b) * a.n_tiles + T];
-        total = c < cap ? c : (int)cap;
-        bkp4 = a.bk + ((size_t)par * a.B + b) * a.bk_stride + a.bk_off[T];
+        const int par = (int)((clk2 + a.blk) & 1);
+        bcap = (int)(bko1 - bko0);
+        const int2 c = par ? bkc[1] : bkc[0];
+        bcf = c.x < bcap ? c.x : bcap;
+        bcb = c.y < bcap - bcf ? c.y : bcap - bcf;
+        bkp4 = a.bk + ((size_t)par * a.B + b) * a.bk_stride + bko0;
         __syncthreads();
     } else {
         source_offsets(cnt_b, t0, lt0, a.K, nd, M, a.N, s_off);
-        total = s_off[a.K];
     }
     // the ext-slot loads, also from valid addresses (a neuron without a slot reads slot 0); the
     // slot test is applied where the values are used
@@ -736,217 +755,230 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
             *val = wval(w[ek], a.frac_bits);
         }
     };
-    if constexpr (BK && !PL && std::is_same<WT, PackedEdge>::value && BK_PIECE == 2) {
-        // Bucket pieces hold at most two edges (tile segments are short: ~1 edge per (row, tile)
-        // on a FlyWire-shaped graph), so each thread takes whole pieces: no scan and no search.
-        // Rounds of BK_ROUND entries per thread: every entry load first, then both edges of every
-        // piece (the second from the next packed edge, a valid address even past the last fixed
-        // edge: the packed array ends with one padding entry), then the LDS atomics -- two memory
-        // round trips per round and no wait in between.
-        for (int e0 = tid; e0 < total; e0 += BK_ROUND * TPB) {
-            int v[BK_ROUND];
+    // The arrivals of block steps [k0, k1) into acc columns k - k0 (the LDS atomics of the last
+    // round may still be in flight when it returns: a barrier orders them before the step loop).
+    auto pull = [&](int k0, int k1) {
+        if constexpr (BK) {
+            // entries nf of the front part (first part), then nbk of the back part (second part)
+            const int nf = k0 == 0 ? bcf : 0, nbk = k1 == a.K ? bcb : 0;
+            const int total = nf + nbk, boff = bcap - bcb - nf;  // entry e >= nf sits at e + boff
+            if constexpr (!PL && std::is_same<WT, PackedEdge>::value && BK_PIECE == 2) {
+                // Pieces hold at most two edges (tile segments are short: ~1 edge per (row, tile) on a
+                // FlyWire-shaped graph), so each thread takes whole pieces: no scan and no search.
+                // Rounds of BK_ROUND entries per thread: every entry load first, then both edges of
+                // every piece (the second from the next packed edge, a valid address even past the
+                // last fixed edge: the packed array ends with one padding entry), then the LDS
+                // atomics -- two memory round trips per round and no wait in between.
+                for (int e0 = tid; e0 < total; e0 += BK_ROUND * TPB) {
+                    int v[BK_ROUND];
 #pragma unroll
-            for (int r = 0; r < BK_ROUND; r++) {
-                const int e = e0 + r * TPB;
-                v[r] = bkp4[e < total ? e : e0];  // first edge << 7 | (len - 1) << 5 | step
-            }
-            int x0[BK_ROUND], x1[BK_ROUND];
+                    for (int r = 0; r < BK_ROUND; r++) {
+                        const int e = e0 + r * TPB < total ? e0 + r * TPB : e0;
+                        v[r] = bkp4[e < nf ? e : e + boff];  // first edge << 7 | (len - 1) << 5 | step
+                    }
+                    int x0[BK_ROUND], x1[BK_ROUND];
 #pragma unroll
-            for (int r = 0; r < BK_ROUND; r++) {
-                x0[r] = a.pk[v[r] >> 7];
-                x1[r] = a.pk[(v[r] >> 7) + 1];
-            }
+                    for (int r = 0; r < BK_ROUND; r++) {
+                        x0[r] = a.pk[v[r] >> 7];
+                        x1[r] = a.pk[(v[r] >> 7) + 1];
+                    }
 #pragma unroll
-            for (int r = 0; r < BK_ROUND; r++) {
-                if (e0 + r * TPB >= total) continue;
-                const int slot = v[r] & 31;
-                const int v0 = x0[r] >> a.lbits, v1 = x1[r] >> a.lbits;
-                if (v0 != 0) atomicAdd(&acc[__umul24((unsigned)(x0[r] & lmask), (unsigned)KP) + slot], v0);
-                if ((v[r] & (1 << 5)) && v1 != 0)
-                    atomicAdd(&acc[__umul24((unsigned)(x1[r] & lmask), (unsigned)KP) + slot], v1);
-            }
-        }
-        // no barrier: the plastic arrivals and Poisson hits below are LDS atomics too, and one
-        // barrier before the step loop orders them all
-    } else if constexpr (BK) {
-        // The general form (any piece length, plastic edges, unpacked formats): the two entry
-        // loads and the first edge of each piece are issued unconditionally from clamped, valid
-        // indices and masked afterwards, so they are in flight together; the further edges of a
-        // piece are loaded on demand.
-        for (int e0 = tid; e0 < total; e0 += 2 * TPB) {
-            int v[2];
-            bool ok[2];
-#pragma unroll
-            for (int r = 0; r < 2; r++) {
-                const int e = e0 + r * TPB;
-                ok[r] = e < total;
-                v[r] = bkp4[ok[r] ? e : e0];  // first edge << 7 | (len - 1) << 5 | step
-            }
-            int loc[2][BK_PIECE], val[2][BK_PIECE];
-#pragma unroll
-            for (int r = 0; r < 2; r++) {
-                edge(v[r] >> 7, &loc[r][0], &val[r][0]);
-                if (!ok[r]) val[r][0] = 0;
-            }
-#pragma unroll
-            for (int r = 0; r < 2; r++) {
-                const int seg = v[r] >> 7, len = ok[r] ? ((v[r] >> 5) & 3) + 1 : 0;
-#pragma unroll
-                for (int j = 1; j < BK_PIECE; j++) {
-                    loc[r][j] = val[r][j] = 0;
-                    if (j < len) edge(seg + j, &loc[r][j], &val[r][j]);
-                }
-            }
-#pragma unroll
-            for (int r = 0; r < 2; r++) {
-                const int slot = v[r] & 31;
-#pragma unroll
-                for (int j = 0; j < BK_PIECE; j++)
-                    if (val[r][j] != 0) atomicAdd(&acc[__umul24(loc[r][j], KP) + slot], val[r][j]);
-            }
-        }
-        // no barrier: the plastic arrivals and Poisson hits below are LDS atomics too, and one
-        // barrier before the step loop orders them all
-    } else {
-        // Entries in chunks of TPB * IPT; each thread owns IPT consecutive entries so that all of
-        // its list and table loads are in flight together; segments (any length) are staged with
-        // their prefix sums and the edges split evenly over the threads.
-        for (int base = 0; base < total; base += TPB * IPT) {
-            int seg[IPT], len[IPT], kk[IPT];
-            int sum = 0;
-            int ids[IPT];
-            // loads from clamped, valid positions (results masked afterwards) so that the IPT
-            // list reads, then the IPT table reads, of a thread are in flight together
-            bool ok[IPT];
-            size_t li_[IPT];
-#pragma unroll
-            for (int r = 0; r < IPT; r++) {  // list positions first (LDS only) ...
-                const int e = base + tid * IPT + r;
-                ok[r] = e < total;
-                const int ec = ok[r] ? e : base;
-                kk[r] = entry_step(s_off, a.K, ec);
-                li_[r] = (size_t)((lt0 + kk[r] - nd + M) % M) * a.N + (ec - s_off[kk[r]]);
-            }
-#pragma unroll
-            for (int r = 0; r < IPT; r++) ids[r] = idx_b[li_[r]];  // ... then all list loads at once
-            const unsigned nt1 = (unsigned)(a.n_tiles + 1);
-            int2 pq[PL ? IPT : 1];  // plastic segments, loaded beside the fixed ones
-            if constexpr (PL) {
-#pragma unroll
-                for (int r = 0; r < IPT; r++) {
-                    const int idc = ok[r] && (unsigned)ids[r] < (unsigned)a.N ? ids[r] : 0;
-                    pq[r] = a.ptp[__umul24((unsigned)idc, (unsigned)a.n_tiles) + (unsigned)T];
-                }
-            }
-            if (a.tp) {  // one 8-byte load per (spike, tile): the L2 request rate bounds this phase
-#pragma unroll
-                for (int r = 0; r < IPT; r++) {
-                    ok[r] = ok[r] && (unsigned)ids[r] < (unsigned)a.N;
-                    const int idc = ok[r] ? ids[r] : 0;
-                    const int2 q = a.tp[__umul24((unsigned)idc, (unsigned)a.n_tiles) + (unsigned)T];  // < 2^31
-                    seg[r] = q.x;
-                    len[r] = q.y & -(int)ok[r];
-                }
-            } else if (a.tb16) {
-#pragma unroll
-                for (int r = 0; r < IPT; r++) {
-                    ok[r] = ok[r] && (unsigned)ids[r] < (unsigned)a.N;
-                    const int idc = ok[r] ? ids[r] : 0;
-                    const unsigned at = __umul24((unsigned)idc, nt1) + (unsigned)T;  // < 2^31 (checked)
-                    const int r0 = a.indptr[idc];
-                    const int l16 = a.tb16[at], h16 = a.tb16[at + 1];
-                    seg[r] = r0 + l16;
-                    len[r] = (h16 - l16) & -(int)ok[r];
+                    for (int r = 0; r < BK_ROUND; r++) {
+                        if (e0 + r * TPB >= total) continue;
+                        const int col = (v[r] & 31) - k0;
+                        const int v0 = x0[r] >> a.lbits, v1 = x1[r] >> a.lbits;
+                        if (v0 != 0) atomicAdd(&acc[__umul24((unsigned)(x0[r] & lmask), (unsigned)KP) + col], v0);
+                        if ((v[r] & (1 << 5)) && v1 != 0)
+                            atomicAdd(&acc[__umul24((unsigned)(x1[r] & lmask), (unsigned)KP) + col], v1);
+                    }
                 }
             } else {
+                // The general form (plastic edges, unpacked formats): the two entry loads and the
+                // first edge of each piece are issued unconditionally from clamped, valid indices and
+                // masked afterwards, so they are in flight together; the further edges of a piece are
+                // loaded on demand.
+                for (int e0 = tid; e0 < total; e0 += 2 * TPB) {
+                    int v[2];
+                    bool ok[2];
 #pragma unroll
-                for (int r = 0; r < IPT; r++) {
-                    ok[r] = ok[r] && (unsigned)ids[r] < (unsigned)a.N;
-                    const int idc = ok[r] ? ids[r] : 0;
-                    const unsigned at = __umul24((unsigned)idc, nt1) + (unsigned)T;
-                    seg[r] = a.tb[at];
-                    len[r] = (a.tb[at + 1] - seg[r]) & -(int)ok[r];  // masked, not branched on
-                }
-            }
-            // pass 0: the fixed segments; pass 1: the plastic ones of spikes emitted in this window
-            // (emitted at window step kb - n_delay + k >= 0), staged the same way
-#pragma unroll
-            for (int pass = 0; pass < (PL ? 2 : 1); pass++) {
-                if (PL && pass == 1) {
-#pragma unroll
-                    for (int r = 0; r < IPT; r++) {
-                        seg[r] = (int)a.nnz + pq[PL ? r : 0].x;
-                        len[r] = ok[r] && a.kb + kk[r] - nd >= 0 ? pq[PL ? r : 0].y : 0;
+                    for (int r = 0; r < 2; r++) {
+                        const int e = e0 + r * TPB;
+                        ok[r] = e < total;
+                        const int ec = ok[r] ? e : e0;
+                        v[r] = bkp4[ec < nf ? ec : ec + boff];
                     }
-                }
-                sum = 0;
+                    int loc[2][BK_PIECE], val[2][BK_PIECE];
 #pragma unroll
-                for (int r = 0; r < IPT; r++) sum += len[r];
-                int tot;
-                int run = block_incl_scan(sum, s_wsum, &tot) - sum;
+                    for (int r = 0; r < 2; r++) {
+                        edge(v[r] >> 7, &loc[r][0], &val[r][0]);
+                        if (!ok[r]) val[r][0] = 0;
+                    }
 #pragma unroll
-                for (int r = 0; r < IPT; r++) {
-                    run += len[r];
-                    st_seg[tid * IPT + r] = seg[r];
-                    st_end[tid * IPT + r] = run;
-                    st_k[tid * IPT + r] = kk[r];
-                }
-                __syncthreads();
-                const int ns = TPB * IPT;
-                for (int q0 = tid; q0 < tot; q0 += TPB * UNROLL) {
-                    int ek[UNROLL], slot[UNROLL];
+                    for (int r = 0; r < 2; r++) {
+                        const int seg = v[r] >> 7, len = ok[r] ? ((v[r] >> 5) & 3) + 1 : 0;
 #pragma unroll
-                    for (int u = 0; u < UNROLL; u++) {
-                        const int q = q0 + u * TPB;
-                        ek[u] = -1;
-                        slot[u] = 0;
-                        if (q < tot) {
-                            int lo = 0, hi = ns - 1;  // first j with st_end[j] > q
-                            while (lo < hi) {
-                                const int mid = (lo + hi) >> 1;
-                                if (st_end[mid] > q) hi = mid;
-                                else lo = mid + 1;
-                            }
-                            ek[u] = st_seg[lo] + (q - (lo ? st_end[lo - 1] : 0));
-                            slot[u] = st_k[lo];
+                        for (int j = 1; j < BK_PIECE; j++) {
+                            loc[r][j] = val[r][j] = 0;
+                            if (j < len) edge(seg + j, &loc[r][j], &val[r][j]);
                         }
                     }
-                    int loc[UNROLL], val[UNROLL];
 #pragma unroll
-                    for (int u = 0; u < UNROLL; u++) {  // all UNROLL edge loads in flight
-                        edge(ek[u] >= 0 ? ek[u] : ek[0], &loc[u], &val[u]);
-                        if (ek[u] < 0) val[u] = 0;
+                    for (int r = 0; r < 2; r++) {
+                        const int col = (v[r] & 31) - k0;
+#pragma unroll
+                        for (int j = 0; j < BK_PIECE; j++)
+                            if (val[r][j] != 0) atomicAdd(&acc[__umul24(loc[r][j], KP) + col], val[r][j]);
                     }
-#pragma unroll
-                    for (int u = 0; u < UNROLL; u++)
-                        if (val[u] != 0) atomicAdd(&acc[__umul24(loc[u], KP) + slot[u]], val[u]);
                 }
-                __syncthreads();
+            }
+        } else {
+            // The list entries of source steps k0 .. k1 - 1 in chunks of TPB * IPT; each thread owns
+            // IPT consecutive entries so that all of its list and table loads are in flight
+            // together; segments (any length) are staged with their prefix sums and the edges split
+            // evenly over the threads.
+            const int lo = s_off[k0], hi = s_off[k1];
+            for (int base = lo; base < hi; base += TPB * IPT) {
+                int seg[IPT], len[IPT], kk[IPT];
+                int sum = 0;
+                int ids[IPT];
+                // loads from clamped, valid positions (results masked afterwards) so that the IPT
+                // list reads, then the IPT table reads, of a thread are in flight together
+                bool ok[IPT];
+                size_t li_[IPT];
+#pragma unroll
+                for (int r = 0; r < IPT; r++) {  // list positions first (LDS only) ...
+                    const int e = base + tid * IPT + r;
+                    ok[r] = e < hi;
+                    const int ec = ok[r] ? e : base;
+                    kk[r] = entry_step(s_off, a.K, ec);
+                    li_[r] = (size_t)((lt0 + kk[r] - nd + M) % M) * a.N + (ec - s_off[kk[r]]);
+                }
+#pragma unroll
+                for (int r = 0; r < IPT; r++) ids[r] = idx_b[li_[r]];  // ... then all list loads at once
+                const unsigned nt1 = (unsigned)(a.n_tiles + 1);
+                int2 pq[PL ? IPT : 1];  // plastic segments, loaded beside the fixed ones
+                if constexpr (PL) {
+#pragma unroll
+                    for (int r = 0; r < IPT; r++) {
+                        const int idc = ok[r] && (unsigned)ids[r] < (unsigned)a.N ? ids[r] : 0;
+                        pq[r] = a.ptp[__umul24((unsigned)idc, (unsigned)a.n_tiles) + (unsigned)T];
+                    }
+                }
+                if (a.tp) {  // one 8-byte load per (spike, tile): the L2 request rate bounds this phase
+#pragma unroll
+                    for (int r = 0; r < IPT; r++) {
+                        ok[r] = ok[r] && (unsigned)ids[r] < (unsigned)a.N;
+                        const int idc = ok[r] ? ids[r] : 0;
+                        const int2 q = a.tp[__umul24((unsigned)idc, (unsigned)a.n_tiles) + (unsigned)T];  // < 2^31
+                        seg[r] = q.x;
+                        len[r] = q.y & -(int)ok[r];
+                    }
+                } else if (a.tb16) {
+#pragma unroll
+                    for (int r = 0; r < IPT; r++) {
+                        ok[r] = ok[r] && (unsigned)ids[r] < (unsigned)a.N;
+                        const int idc = ok[r] ? ids[r] : 0;
+                        const unsigned at = __umul24((unsigned)idc, nt1) + (unsigned)T;  // < 2^31 (checked)
+                        const int r0 = a.indptr[idc];
+                        const int l16 = a.tb16[at], h16 = a.tb16[at + 1];
+                        seg[r] = r0 + l16;
+                        len[r] = (h16 - l16) & -(int)ok[r];
+                    }
+                } else {
+#pragma unroll
+                    for (int r = 0; r < IPT; r++) {
+                        ok[r] = ok[r] && (unsigned)ids[r] < (unsigned)a.N;
+                        const int idc = ok[r] ? ids[r] : 0;
+                        const unsigned at = __umul24((unsigned)idc, nt1) + (unsigned)T;
+                        seg[r] = a.tb[at];
+                        len[r] = (a.tb[at + 1] - seg[r]) & -(int)ok[r];  // masked, not branched on
+                    }
+                }
+                // pass 0: the fixed segments; pass 1: the plastic ones of spikes emitted in this window
+                // (emitted at window step kb - n_delay + k >= 0), staged the same way
+#pragma unroll
+                for (int pass = 0; pass < (PL ? 2 : 1); pass++) {
+                    if (PL && pass == 1) {
+#pragma unroll
+                        for (int r = 0; r < IPT; r++) {
+                            seg[r] = (int)a.nnz + pq[PL ? r : 0].x;
+                            len[r] = ok[r] && a.kb + kk[r] - nd >= 0 ? pq[PL ? r : 0].y : 0;
+                        }
+                    }
+                    sum = 0;
+#pragma unroll
+                    for (int r = 0; r < IPT; r++) sum += len[r];
+                    int tot;
+                    int run = block_incl_scan(sum, s_wsum, &tot) - sum;
+#pragma unroll
+                    for (int r = 0; r < IPT; r++) {
+                        run += len[r];
+                        st_seg[tid * IPT + r] = seg[r];
+                        st_end[tid * IPT + r] = run;
+                        st_k[tid * IPT + r] = kk[r] - k0;  // acc column
+                    }
+                    __syncthreads();
+                    const int ns = TPB * IPT;
+                    for (int q0 = tid; q0 < tot; q0 += TPB * UNROLL) {
+                        int ek[UNROLL], slot[UNROLL];
+#pragma unroll
+                        for (int u = 0; u < UNROLL; u++) {
+                            const int q = q0 + u * TPB;
+                            ek[u] = -1;
+                            slot[u] = 0;
+                            if (q < tot) {
+                                int l = 0, h = ns - 1;  // first j with st_end[j] > q
+                                while (l < h) {
+                                    const int mid = (l + h) >> 1;
+                                    if (st_end[mid] > q) h = mid;
+                                    else l = mid + 1;
+                                }
+                                ek[u] = st_seg[l] + (q - (l ? st_end[l - 1] : 0));
+                                slot[u] = st_k[l];
+                            }
+                        }
+                        int loc[UNROLL], val[UNROLL];
+#pragma unroll
+                        for (int u = 0; u < UNROLL; u++) {  // all UNROLL edge loads in flight
+                            edge(ek[u] >= 0 ? ek[u] : ek[0], &loc[u], &val[u]);
+                            if (ek[u] < 0) val[u] = 0;
+                        }
+#pragma unroll
+                        for (int u = 0; u < UNROLL; u++)
+                            if (val[u] != 0) atomicAdd(&acc[__umul24(loc[u], KP) + slot[u]], val[u]);
+                    }
+                    __syncthreads();
+                }
             }
         }
-    }
-    // plastic arrivals pushed at spike time (all of them, or with plastic pulls only those from
-    // earlier windows, which land in a window's first n_delay steps), tile's targets only
-    if (a.ptg_ptr && (!PL || a.kb < nd)) {
-        const int p0 = a.ptg_ptr[T], np_ = a.ptg_ptr[T + 1] - p0;
-        for (int q = tid; q < np_ * a.K; q += TPB) {
-            const int k = q / np_;
-            const int i = a.ptg_idx[p0 + (q - k * np_)];
-            int* r = a.ring + ((size_t)b * a.ring_slots + (rs0 + k) % a.ring_slots) * a.N + i;
-            const int v = *r;
-            if (v != 0) {
-                atomicAdd(&acc[__umul24(i - i0, KP) + k], v);
-                *r = 0;
+        // plastic arrivals pushed at spike time (all of them, or with plastic pulls only those from
+        // earlier windows, which land in a window's first n_delay steps), tile's targets only
+        if (a.ptg_ptr && (!PL || a.kb < nd)) {
+            const int p0 = a.ptg_ptr[T], np_ = a.ptg_ptr[T + 1] - p0, kn = k1 - k0;
+            for (int q = tid; q < np_ * kn; q += TPB) {
+                const int kq = q / np_;
+                const int i = a.ptg_idx[p0 + (q - kq * np_)];
+                int* r = a.ring + ((size_t)b * a.ring_slots + (rs0 + k0 + kq) % a.ring_slots) * a.N + i;
+                const int v = *r;
+                if (v != 0) {
+                    atomicAdd(&acc[__umul24(i - i0, KP) + kq], v);
+                    *r = 0;
+                }
             }
         }
-    }
+    };
+    // A block runs as one part, or as two (steps [0, kh) and [kh, K)) when its arrivals of all K
+    // steps would not leave LDS for enough workgroups per CU: acc then holds one part's columns
+    // (and the hit masks, read into registers before the first part's steps).
+    const int KH = a.kh;
+    pull(0, KH);
 
-    // Poisson hits of the tile's input neurons for the K steps, as K-bit masks in column K of
+    // Poisson hits of the tile's input neurons for all K steps, as K-bit masks in column KH of
     // acc.  Input neurons are scattered, so hashing inside the step loop would run the hash on
     // every lane of almost every wavefront; instead each wavefront spreads the (neuron, step)
     // pairs of its input neurons over its 64 lanes, in registers: a lane permutation lists the
     // input lanes in order (src[q] = lane of the q-th input neuron) and each pair fetches its
-    // neuron's payload from that lane.  No LDS staging and no barrier; the column-K atomics
+    // neuron's payload from that lane.  No LDS staging and no barrier; the column-KH atomics
     // touch no word the pull adds into.  The draw is flyzero/gpu.py:66-67: lowbias32 of
     // (neuron, step, seed + fly), or the precomputed hit bits.
 #pragma unroll
@@ -975,7 +1007,7 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
                     const uint32_t h = mix32((hb * (uint32_t)a.N + (uint32_t)(i0 + lq)) ^ s_sk[k]);
                     hit = (float)(h >> 8) * (1.0f / 16777216.0f) < __int_as_float(pl);
                 }
-                if (hit) atomicOr(&acc[__umul24(lq, KP) + a.K], 1 << k);
+                if (hit) atomicOr(&acc[__umul24(lq, KP) + KH], 1 << k);
             }
             q += dq;
             k += dk;
@@ -995,50 +1027,95 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
     // max(rend - k, 0), so "r > 0" is k < rend, "r <= 0 after the decrement" is k + 1 >= rend,
     // and a spike at step k sets rend = k + 1 + n_ref (strictly larger than before, so the
     // spike and its step are read back from rend after the loop, without a per-step mask).
+    // State stays in registers across the parts of a block.
     const bool once = a.c.n_ref >= a.K;
-    uint32_t masks[NPT];
+    uint32_t masks[NPT], hm[NPT];
+    float uu[NPT], gg[NPT];
+    int rr[NPT];  // once: the block step at which the refractory period ends; else the counter
 #pragma unroll
     for (int p = 0; p < NPT; p++) {
-        const int li = p * TPB + tid, i = i0 + li;
-        const int* row = acc + __umul24(li, KP);
-        uint32_t mask = 0;
-        if (i < a.N) {
-            const uint32_t hits = (uint32_t)row[a.K];
-            float u = u_[p], g = g_[p];
-            int r = r_[p];
-            const float bias = j_[p] >= 0 ? bias_[p] : 0.f;
-            if (once) {
-                int rend;
-                // the block lengths of dt = 0.1 ms and 0.25 ms (n_delay 18 and 7) unrolled
-                if (a.K == 18) rend = once_steps<18>(row, a.K, u, g, r, bias, hits, a.c);
-                else if (a.K == 7) rend = once_steps<7>(row, a.K, u, g, r, bias, hits, a.c);
-                else rend = once_steps<0>(row, a.K, u, g, r, bias, hits, a.c);
-                if (rend != r) mask = 1u << (rend - 1 - a.c.n_ref);
-                r = rend - min(max(rend, 0), a.K);  // the counter after the last step
-            } else {
-                for (int k = 0; k < a.K; k++) {
-                    const float inj = (float)row[k] * a.c.in_scale;
-                    g = g * a.c.decay_syn + inj;
-                    const bool refr = r > 0;
-                    const float u_int = u + (g + bias - u) * a.c.decay_m;
-                    u = refr ? u : u_int;
-                    r = refr ? r - 1 : r;
-                    const float u_kick = u + a.c.kick;
-                    u = ((hits >> k) & 1u) ? u_kick : u;
-                    const bool sp = u >= a.c.u_th && r <= 0;
-                    u = sp ? a.c.u_reset : u;
-                    r = sp ? a.c.n_ref : r;
-                    mask |= (uint32_t)sp << k;
+        masks[p] = hm[p] = 0;
+        uu[p] = u_[p];
+        gg[p] = g_[p];
+        rr[p] = r_[p];
+    }
+    auto steps = [&](int k0, int k1) {  // block steps [k0, k1) from acc columns k - k0
+        const int kn = k1 - k0;
+#pragma unroll
+        for (int p = 0; p < NPT; p++) {
+            const int li = p * TPB + tid, i = i0 + li;
+            const int* row = acc + __umul24(li, KP);
+            if (i < a.N) {
+                if (k0 == 0) hm[p] = (uint32_t)row[KH];
+                const uint32_t hits = hm[p] >> k0;
+                float u = uu[p], g = gg[p];
+                const float bias = j_[p] >= 0 ? bias_[p] : 0.f;
+                if (once) {
+                    const int r0 = rr[p] - k0;  // relative to the part's first step
+                    int rend;
+                    // the block lengths of dt = 0.1 ms and 0.25 ms (n_delay 18 and 7), and the parts
+                    // of the former, unrolled
+                    if (kn == 18) rend = once_steps<18>(row, kn, u, g, r0, bias, hits, a.c);
+                    else if (kn == 9) rend = once_steps<9>(row, kn, u, g, r0, bias, hits, a.c);
+                    else if (kn == 7) rend = once_steps<7>(row, kn, u, g, r0, bias, hits, a.c);
+                    else rend = once_steps<0>(row, kn, u, g, r0, bias, hits, a.c);
+                    if (rend != r0) masks[p] |= 1u << (rend - 1 - a.c.n_ref + k0);
+                    rr[p] = rend + k0;
+                } else {
+                    int r = rr[p];
+                    uint32_t mask = 0;
+                    for (int k = 0; k < kn; k++) {
+                        const float inj = (float)row[k] * a.c.in_scale;
+                        g = g * a.c.decay_syn + inj;
+                        const bool refr = r > 0;
+                        const float u_int = u + (g + bias - u) * a.c.decay_m;
+                        u = refr ? u : u_int;
+                        r = refr ? r - 1 : r;
+                        const float u_kick = u + a.c.kick;
+                        u = ((hits >> k) & 1u) ? u_kick : u;
+                        const bool sp = u >= a.c.u_th && r <= 0;
+                        u = sp ? a.c.u_reset : u;
+                        r = sp ? a.c.n_ref : r;
+                        mask |= (uint32_t)sp << k;
+                    }
+                    masks[p] |= mask << k0;
+                    rr[p] = r;
                 }
+                uu[p] = u;
+                gg[p] = g;
             }
-            if (u != u_[p]) a.u[bn + i] = u;
-            if (g != g_[p]) a.g[bn + i] = g;
-            if (r != r_[p]) a.ref[bn + i] = (signed char)r;
-            const int ns = __popc(mask);
-            if (a.first) a.counts[bn + i] = ns;
-            else if (ns) a.counts[bn + i] += ns;
         }
-        masks[p] = mask;
+    };
+    steps(0, KH);
+    if (KH < a.K) {  // the second part: acc cleared and refilled with the arrivals of [KH, K)
+        __syncthreads();
+        for (int q = tid; q < (KP * TS) >> 2; q += TPB) reinterpret_cast<int4*>(acc)[q] = make_int4(0, 0, 0, 0);
+        __syncthreads();
+        pull(KH, a.K);
+        __syncthreads();
+        steps(KH, a.K);
+    }
+    // Spikes per block step (LDS), then one global atomic per step reserves their places in the
+    // per-step lists; its latency overlaps the state stores that follow.
+#pragma unroll
+    for (int p = 0; p < NPT; p++)
+        for (uint32_t m = masks[p]; m; m &= m - 1) atomicAdd(&s_lcnt[__ffs(m) - 1], 1);
+    __syncthreads();
+    int lpos = 0;
+    if (tid < a.K && s_lcnt[tid] > 0) lpos = atomicAdd(&cnt_b[(lt0 + tid) % M], s_lcnt[tid]);
+    // state and counts (a window's later blocks add theirs with no-return atomics: no load waits)
+#pragma unroll
+    for (int p = 0; p < NPT; p++) {
+        const int i = i0 + p * TPB + tid;
+        if (i < a.N) {
+            const int r = once ? rr[p] - min(max(rr[p], 0), a.K) : rr[p];  // the counter after the block
+            if (uu[p] != u_[p]) a.u[bn + i] = uu[p];
+            if (gg[p] != g_[p]) a.g[bn + i] = gg[p];
+            if (r != r_[p]) a.ref[bn + i] = (signed char)r;
+            const int ns = __popc(masks[p]);
+            if (a.first) a.counts[bn + i] = ns;
+            else if (ns) atomicAdd(&a.counts[bn + i], ns);
+        }
     }
 
     if (a.raster) {
@@ -1054,14 +1131,9 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
         }
     }
 
-    // append spikes to the per-step lists, then push the spikers' plastic rows n_delay steps ahead
-    if (tid < a.K) s_lcnt[tid] = 0;
-    __syncthreads();
-#pragma unroll
-    for (int p = 0; p < NPT; p++)
-        for (uint32_t m = masks[p]; m; m &= m - 1) atomicAdd(&s_lcnt[__ffs(m) - 1], 1);
-    __syncthreads();
-    if (tid < a.K && s_lcnt[tid] > 0) s_lcnt[tid] = atomicAdd(&cnt_b[(lt0 + tid) % M], s_lcnt[tid]);
+    // append the spikes to the per-step lists at the reserved places, then push the spikers'
+    // plastic rows n_delay steps ahead
+    if (tid < a.K) s_lcnt[tid] = lpos;
     __syncthreads();
 #pragma unroll
     for (int p = 0; p < NPT; p++) {
@@ -1175,6 +1247,7 @@ constexpr int BK_SP = BK_STAGE + 1;  // odd row stride: the 64 lanes' rows start
 struct BucketArgs {
     const long long* clock;
     int N, B, K, kb, blk, nd, M, n_tiles;
+    int kh;           // entries of steps < kh go to a bucket's front, the others to its back
     const int* spk_idx;
     const int* spk_cnt;
     const int* tb;
@@ -1204,7 +1277,8 @@ struct BucketArgs {
 template <bool T16, bool PL>
 __global__ void __launch_bounds__(BK_THREADS, 8) tile_bucket(BucketArgs a) {
     __shared__ int s_off[KMAX + 1];
-    __shared__ int s_cnt[64];          // next position in each tile's bucket
+    __shared__ int s_cnt[64];          // entries written to the front of each tile's bucket
+    __shared__ int s_cntb[64];         // entries written to its back
     __shared__ int s_fill[64];         // entries reserved in each tile's stage since the last flush
     __shared__ int s_cap[64];          // each tile's bucket capacity
     __shared__ long long s_base[64];   // each tile's bucket in the arena
@@ -1237,6 +1311,7 @@ __global__ void __launch_bounds__(BK_THREADS, 8) tile_bucket(BucketArgs a) {
         s_base[lane] = base;
         s_cap[lane] = cap < 0x7fffffff ? (int)cap : 0x7fffffff;
         s_cnt[lane] = 0;
+        s_cntb[lane] = 0;
         s_fill[lane] = 0;
     }
     if (tid == 0) s_flush = 0;
@@ -1246,20 +1321,38 @@ __global__ void __launch_bounds__(BK_THREADS, 8) tile_bucket(BucketArgs a) {
     const int capi = s_cap[lane];
     const int nt1 = a.n_tiles + 1;
     const int Tc = own ? T : 0;
-    // every thread, between barriers: stages to memory, then counters advanced and cleared
+    // every thread, between barriers: stages to memory (entries of steps < kh from the front of
+    // the bucket up, the others from its back down, positions by ballots), counters advanced
     auto flush = [&]() {
         for (int t = wave; t < 64; t += W) {
             const int n = s_fill[t] < BK_STAGE ? s_fill[t] : BK_STAGE;
-            const int g0 = s_cnt[t], cp = s_cap[t];
+            const int f0 = s_cnt[t], b0 = s_cntb[t], cp = s_cap[t];
             int* d = a.bk + s_base[t];
-            for (int j = lane; j < n; j += 64)
-                if (g0 + j < cp) d[g0 + j] = s_stage[t * BK_SP + j];
+            int nf = 0, nbk = 0;  // wavefront-uniform
+            for (int j0 = 0; j0 < n; j0 += 64) {
+                const int j = j0 + lane;
+                const int x = j < n ? s_stage[t * BK_SP + j] : 0;
+                const bool back = j < n && (x & 31) >= a.kh, front = j < n && !back;
+                const unsigned long long mf = __ballot(front), mb = __ballot(back);
+                const unsigned long long below = (1ull << lane) - 1ull;
+                if (front) {
+                    const int pos = f0 + nf + __popcll(mf & below);
+                    if (pos < cp) d[pos] = x;
+                }
+                if (back) {
+                    const int pos = cp - 1 - (b0 + nbk + __popcll(mb & below));
+                    if (pos >= 0) d[pos] = x;
+                }
+                nf += __popcll(mf);
+                nbk += __popcll(mb);
+            }
+            if (lane == 0) {
+                s_cnt[t] = f0 + nf;
+                s_cntb[t] = b0 + nbk;
+            }
         }
         __syncthreads();
-        if (tid < 64) {
-            s_cnt[tid] += s_fill[tid] < BK_STAGE ? s_fill[tid] : BK_STAGE;
-            s_fill[tid] = 0;
-        }
+        if (tid < 64) s_fill[tid] = 0;
         if (tid == 0) s_flush = 0;
         __syncthreads();
     };
@@ -1327,15 +1420,16 @@ __global__ void __launch_bounds__(BK_THREADS, 8) tile_bucket(BucketArgs a) {
                 }
                 if (tot > 0) {
                     int slot = atomicAdd(&s_fill[lane], tot);
-                    const int over = slot + tot - BK_STAGE;
-                    int g = over > 0 ? atomicAdd(&s_cnt[lane], over < tot ? over : tot) : 0;
                     if (slot + tot > BK_STAGE - W * BK_UNROLL) s_flush = 1;
                     auto put = [&](int entry) {
                         if (slot < BK_STAGE) {
                             s_stage[lane * BK_SP + slot] = entry;
-                        } else {  // past the capacity: dropped, counted, flagged below
-                            if (g < capi) dst[g] = entry;
-                            g++;
+                        } else if ((entry & 31) < a.kh) {  // past a full stage: straight to memory
+                            const int g = atomicAdd(&s_cnt[lane], 1);
+                            if (g < capi) dst[g] = entry;  // past the capacity: dropped, flagged below
+                        } else {
+                            const int g = capi - 1 - atomicAdd(&s_cntb[lane], 1);
+                            if (g >= 0) dst[g] = entry;
                         }
                         slot++;
                     };
@@ -1360,12 +1454,13 @@ __global__ void __launch_bounds__(BK_THREADS, 8) tile_bucket(BucketArgs a) {
     }
     flush();
     if (tid < 64 && own) {
-        int c = s_cnt[lane];
-        if (c > capi) {
+        int c = s_cnt[lane], cb = s_cntb[lane];
+        if (c + cb > capi) {  // cannot happen: the capacities bound every block
             atomicOr(a.err, 2);
-            c = capi;
+            c = c < capi ? c : capi;
+            cb = cb < capi - c ? cb : capi - c;
         }
-        a.bk_cnt[((size_t)par * a.B + b) * a.n_tiles + T] = c;
+        reinterpret_cast<int2*>(a.bk_cnt)[((size_t)par * a.B + b) * a.n_tiles + T] = make_int2(c, cb);
     }
 }
 
@@ -1458,8 +1553,12 @@ namespace {
 
 struct Shape {
     int ut, pt, chunk, grid, K, ipt, src;
-    int pl;  // plastic pulls in effect (requested, and the engine has the plastic pair table)
+    int pl;     // plastic pulls in effect (requested, and the engine has the plastic pair table)
+    int parts;  // tile engine: a block runs as 1 part or as 2 (half the arrival columns in LDS)
 };
+
+// Steps of the first part of a block of K steps.
+int part_kh(const Shape& s, int K) { return s.parts == 2 && K >= 2 ? (K + 1) / 2 : K; }
 
 int resolve(const fz_engine* e, const fz_tune* t, Shape* s) {
     s->ut = (t && t->upd_threads) ? t->upd_threads : 256;
@@ -1471,6 +1570,8 @@ int resolve(const fz_engine* e, const fz_tune* t, Shape* s) {
     if (s->ipt != 1 && s->ipt != 4) return fail(FZ_E_ARG, "tile_stage %d: must be 1 or 4", s->ipt);
     s->src = t ? t->tile_source : 0;
     s->pl = 0;
+    s->parts = (t && t->block_parts) ? t->block_parts : 1;
+    if (s->parts != 1 && s->parts != 2) return fail(FZ_E_ARG, "block_parts %d: must be 1 or 2", s->parts);
     if (t && t->plastic_pull != 0 && t->plastic_pull != 1)
         return fail(FZ_E_ARG, "plastic_pull %d: must be 0 or 1", t->plastic_pull);
     if (t && t->plastic_pull && e->mode == 2 && e->n_pl > 0 && e->ptp && e->pk) s->pl = 1;
@@ -1485,8 +1586,9 @@ int resolve(const fz_engine* e, const fz_tune* t, Shape* s) {
         if (s->ut <= 0 || e->ts % s->ut || (npt != 1 && npt != 2 && npt != 4 && npt != 8))
             return fail(FZ_E_ARG, "tile engine: upd_threads %d must divide the tile (%d neurons) into 1, 2, 4 or 8",
                         s->ut, e->ts);
-        // K arrival columns and the hit-mask column per neuron, plus the segment-table staging
-        const size_t lds = (size_t)(s->K + 1) * e->ts * 4 + (s->src != 1 ? (size_t)3 * s->ipt * s->ut * 4 : 0);
+        // the first part's arrival columns and the hit-mask column per neuron, plus the
+        // segment-table staging
+        const size_t lds = (size_t)(part_kh(*s, s->K) + 1) * e->ts * 4 + (s->src != 1 ? (size_t)3 * s->ipt * s->ut * 4 : 0);
         if (lds > e->lds_max - TILE_STATIC_LDS)
             return fail(FZ_E_ARG, "tile engine: %zu B of LDS for %d steps x %d neurons exceeds %zu", lds, s->K,
                         e->ts, e->lds_max);
@@ -1589,6 +1691,7 @@ TileArgs make_tile(const fz_engine* e, const fz_inputs* in, int K, int KP, int k
     a.nb = e->st.B;
     a.K = K;
     a.KP = KP;
+    a.kh = K;  // one part; the caller sets the shape's
     a.kb = kb;
     a.first = first;
     a.TS = e->ts;
@@ -1675,11 +1778,12 @@ int tile_fmt(const fz_engine* e) { return e->pk ? 2 : (e->w_bits == 16 ? 0 : 1);
 // LDS staging of the segment-table source (segment, prefix end, step per staged entry).
 size_t tile_staging(const Shape& s) { return s.src != 1 ? (size_t)3 * s.ipt * s.ut * 4 : 0; }
 
-// Row stride of the LDS accumulator: K arrival columns and the hit-mask column, rounded up to
-// odd (no bank conflicts in the step loop) when that fits the LDS budget.
+// Row stride of the LDS accumulator: the first part's arrival columns and the hit-mask column,
+// rounded up to odd (no bank conflicts in the step loop) when that fits the LDS budget.
 int tile_kp(const fz_engine* e, const Shape& s, int K) {
-    const int odd = (K + 1) | 1;
-    return (size_t)odd * e->ts * 4 + tile_staging(s) <= e->lds_max - TILE_STATIC_LDS ? odd : K + 1;
+    const int kh = part_kh(s, K);
+    const int odd = (kh + 1) | 1;
+    return (size_t)odd * e->ts * 4 + tile_staging(s) <= e->lds_max - TILE_STATIC_LDS ? odd : kh + 1;
 }
 
 size_t tile_lds(const fz_engine* e, const Shape& s, int K) {
@@ -1950,16 +2054,16 @@ int ensure_buckets(fz_engine* e) {
     e->bk = nullptr;
     e->bk_cnt = nullptr;
     e->bk_B = 0;
-    // two parity buffers of B x stride 4-byte entries
+    // two parity buffers of B x stride 4-byte entries; (front, back) counts per bucket
     const size_t ne = 2 * (size_t)e->st.B * (size_t)(e->bk_stride > 0 ? e->bk_stride : 1);
     if (hipMalloc(&e->bk, sizeof(int) * ne) != hipSuccess ||
-        hipMalloc(&e->bk_cnt, sizeof(int) * 2 * (size_t)e->st.B * (size_t)e->n_tiles) != hipSuccess) {
+        hipMalloc(&e->bk_cnt, sizeof(int) * 4 * (size_t)e->st.B * (size_t)e->n_tiles) != hipSuccess) {
         (void)hipGetLastError();
         (void)hipFree(e->bk);
         e->bk = nullptr;
         return fail(FZ_E_HIP, "allocating %zu tile-bucket entries failed", ne);
     }
-    HIPCHK(hipMemset(e->bk_cnt, 0, sizeof(int) * 2 * (size_t)e->st.B * (size_t)e->n_tiles));
+    HIPCHK(hipMemset(e->bk_cnt, 0, sizeof(int) * 4 * (size_t)e->st.B * (size_t)e->n_tiles));
     e->bk_B = e->st.B;
     return FZ_OK;
 }
@@ -1977,6 +2081,7 @@ BucketArgs make_bucket(const fz_engine* e, const fz_inputs* in, int K, int kb, i
     a.N = e->n;
     a.B = e->st.B;
     a.K = K;
+    a.kh = K;  // one part; the caller sets the shape's
     a.kb = kb;
     a.nd = e->c.n_delay;
     a.M = 3 * e->c.n_delay;
@@ -2024,6 +2129,7 @@ int enqueue(fz_engine* e, const fz_inputs* in, int steps, const unsigned* hits, 
             const int K = (steps - kb) < s.K ? (steps - kb) : s.K;
             if (s.src == 1) {  // timed with the push timers: it is the spike expansion
                 BucketArgs ba = make_bucket(e, in, K, kb, blk);
+                ba.kh = part_kh(s, K);
                 if (s.pl) ba.ptp = e->ptp;
                 if ((rc = mark(ev_push))) return rc;
                 void* bargs[] = {&ba};
@@ -2032,6 +2138,7 @@ int enqueue(fz_engine* e, const fz_inputs* in, int steps, const unsigned* hits, 
                 if ((rc = mark(ev_push))) return rc;
             }
             TileArgs ta = make_tile(e, in, K, tile_kp(e, s, K), kb, blk, kb == 0, steps, hits, raster);
+            ta.kh = part_kh(s, K);
             if (s.src == 2) ta.tp = nullptr;
             if (s.pl) ta.ptp = e->ptp;
             void* args[] = {&ta};
@@ -2516,6 +2623,7 @@ int fz_graph_create(fz_engine* e, const fz_inputs* in, int steps, const fz_tune*
                 hipGraphNode_t bnode = nullptr;
                 if (s.src == 1) {
                     BucketArgs ba = make_bucket(e, in, K, kb, blk);
+                    ba.kh = part_kh(s, K);
                     ba.b0 = c.b0;
                     if (s.pl) ba.ptp = e->ptp;
                     void* bargs[] = {&ba};
@@ -2539,6 +2647,7 @@ int fz_graph_create(fz_engine* e, const fz_inputs* in, int steps, const fz_tune*
                         return bail("hipGraphAddKernelNode(bucket)", err);
                 }
                 TileArgs ta = make_tile(e, in, K, tile_kp(e, s, K), kb, blk, kb == 0, steps, nullptr, nullptr);
+                ta.kh = part_kh(s, K);
                 if (s.src == 2) ta.tp = nullptr;
                 if (s.pl) ta.ptp = e->ptp;
                 ta.b0 = c.b0;

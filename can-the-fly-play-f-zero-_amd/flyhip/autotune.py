@@ -109,18 +109,21 @@ def tune(bb, duration_ms: float = 1000.0 / 60.0988, reps: int = 2, use_cache: bo
     _time(bb, {}, steps, 1, snap)
     if bb.engine == "tile":
         # full grid (small): threads per workgroup (1, 2, 4 or 8 neurons per thread) x block length
-        # x staging depth x segment source (table lookups or buckets) x, with plastic synapses,
-        # plastic rows pushed or pulled; shapes over the LDS budget are skipped
+        # x blocks in one or two parts x segment source (table lookups, with staging depth 1 or 4,
+        # or buckets) x, with plastic synapses, plastic rows pushed or pulled; shapes over the LDS
+        # budget are skipped
         best_t, cur = float("inf"), None
         pulls = (0, 1) if len(bb.plastic_pos) else (0,)
         for ut in (bb.tile // k for k in (8, 4, 2, 1)):
             if not 64 <= ut <= 1024:
                 continue
             for ks in sorted({bb.n_delay, max(1, bb.n_delay // 2)}):
-                for stage in (1, 4):
-                    for src in (0, 1):
+                for parts in (1, 2):
+                    for src, stage in ((0, 1), (0, 4), (1, 1)):
                         for pull in pulls:
                             trial = {"upd_threads": ut, "block_steps": ks, "tile_stage": stage, "tile_source": src}
+                            if parts == 2:
+                                trial["block_parts"] = 2
                             if pull:
                                 trial["plastic_pull"] = 1
                             t = _time(bb, trial, steps, reps, snap)

@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define FZ_ABI_VERSION 2
+#define FZ_ABI_VERSION 3
 
 #define FZ_OK 0
 #define FZ_E_ARG (-1)      /* invalid argument (null pointer, size out of range) */
@@ -96,6 +96,11 @@ typedef struct fz_tune {
                         * are global atomics per plastic edge, pulls cost a table load per spike
                         * and tile.  1 needs packed edges and a plastic CSR sorted by target within
                         * rows (checked by fz_set_tiles); otherwise plastic rows are pushed. */
+    int block_parts;   /* tile engine: 1 (or 0) = a block's arrivals of all its steps sit in LDS at
+                        * once; 2 = the block runs as two parts (steps [0, ceil(K/2)) and the rest),
+                        * each part's arrivals pulled into LDS in turn while the neuron state stays
+                        * in registers: half the LDS per workgroup, so more workgroups per CU.  The
+                        * same results either way. */
 } fz_tune;
 
 int fz_version(void);

@@ -53,6 +53,7 @@ def test_b256_full_connectome_uneven_episodes_match_oracle(full_fly):
     gb.set_input(idx, rates[0])
     shape = dict(autotune.tune(gb, use_cache=False, reps=1))
     shape["tile_source"] = 1                                      # the bucket source: two graph chains
+    shape["block_parts"] = 2                                      # blocks in two parts (half the LDS)
     gb.set_tune(**shape)
     ob = OracleBrain(fly.weights, p, batch=B, seed=11)
 

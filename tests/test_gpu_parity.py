@@ -291,6 +291,12 @@ def test_extreme_launch_shapes_are_exact(small_fly, shape):
                                         (768, dict(upd_threads=384, tile_source=1)),
                                         (1024, dict(upd_threads=256, tile_source=2)),
                                         (2048, dict(upd_threads=512, block_steps=9, tile_stage=4, tile_source=2)),
+                                        (1024, dict(upd_threads=512, tile_source=1, block_parts=2)),
+                                        (512, dict(upd_threads=64, block_steps=5, tile_source=1, block_parts=2)),
+                                        (768, dict(upd_threads=384, block_steps=9, tile_source=1, block_parts=2)),
+                                        (1024, dict(upd_threads=256, tile_stage=4, block_parts=2)),
+                                        (256, dict(upd_threads=256, block_steps=7, block_parts=2)),
+                                        (2048, dict(upd_threads=512, block_steps=9, tile_source=2, block_parts=2)),
                                         (640, dict(upd_threads=640, tile_stage=4))])
 def test_tile_engine_shapes_are_exact(small_fly, tile, shape):
     # shapes that the LDS budget admits, incl. 8 neurons per thread, 5-step blocks and tiles
@@ -343,16 +349,17 @@ def test_full_connectome_raster_1000_steps_and_one_second_of_frames(full_fly, en
     _assert_state_equal(gb, ob)
 
 
+@pytest.mark.parametrize("parts", [1, 2])
 @pytest.mark.parametrize("pull", [0, 1])
 @pytest.mark.parametrize("source", [0, 1, 2])
-def test_tile_sources_give_identical_rasters_and_plastic_arrivals(small_fly, source, pull):
+def test_tile_sources_give_identical_rasters_and_plastic_arrivals(small_fly, source, pull, parts):
     # per-tile buckets versus segment-table lookups: same rasters, counts and state as the
     # oracle, with plastic synapses (pushed through the ring) and a retired fly
     fly = small_fly
     rng = np.random.default_rng(13)
     pos = np.sort(rng.choice(fly.weights.nnz, 3000, replace=False))
     gb, ob = _pair(fly, batch=3, seed=4, plastic_pos=pos, engine="tile", tile=1024)
-    gb.set_tune(upd_threads=512, tile_source=source, plastic_pull=pull)
+    gb.set_tune(upd_threads=512, tile_source=source, plastic_pull=pull, block_parts=parts)
     gb.pw[1] *= 1.5
     ob.pw[1] *= 1.5
     gb.alive[2] = 0
@@ -372,9 +379,10 @@ def test_tile_sources_give_identical_rasters_and_plastic_arrivals(small_fly, sou
     np.testing.assert_array_equal(_np(gb.g)[:2], ob.g[:2])
 
 
+@pytest.mark.parametrize("parts", [1, 2])
 @pytest.mark.parametrize("pull", [0, 1])
 @pytest.mark.parametrize("source", [0, 1, 2])
-def test_plastic_arrivals_carry_the_weights_of_the_window_they_were_emitted_in(small_fly, source, pull):
+def test_plastic_arrivals_carry_the_weights_of_the_window_they_were_emitted_in(small_fly, source, pull, parts):
     # With plastic pulls the tile engine pulls a plastic synapse at its target when the arrival
     # falls in the window the spike was emitted in, and pushes it at spike time otherwise;
     # without, every plastic row is pushed at spike time.  Windows of 2 to 166 steps,
@@ -385,7 +393,7 @@ def test_plastic_arrivals_carry_the_weights_of_the_window_they_were_emitted_in(s
     rng = np.random.default_rng(17)
     pos = np.sort(rng.choice(fly.weights.nnz, 6000, replace=False))
     gb, ob = _pair(fly, batch=2, seed=9, plastic_pos=pos, engine="tile", tile=1024)
-    gb.set_tune(upd_threads=512, tile_source=source, plastic_pull=pull)
+    gb.set_tune(upd_threads=512, tile_source=source, plastic_pull=pull, block_parts=parts)
     base = ob.pw.copy()
     total = 0
     for wi, steps in enumerate([40, 5, 3, 17, 18, 19, 2, 166, 36, 9, 30, 166]):
@@ -400,9 +408,10 @@ def test_plastic_arrivals_carry_the_weights_of_the_window_they_were_emitted_in(s
     _assert_state_equal(gb, ob)
 
 
+@pytest.mark.parametrize("parts", [1, 2])
 @pytest.mark.parametrize("t_ref", [0.5, 1.8])
 @pytest.mark.parametrize("source", [0, 1, 2])
-def test_tile_engine_refractory_shorter_than_or_equal_to_a_block(small_fly, t_ref, source):
+def test_tile_engine_refractory_shorter_than_or_equal_to_a_block(small_fly, t_ref, source, parts):
     # 18-step blocks: with t_ref = 0.5 ms (5 steps) a neuron can fire several times in one block
     # (the general step loop); with t_ref = 1.8 ms (18 steps) at most once, at the limit of the
     # once-per-block loop
@@ -413,7 +422,7 @@ def test_tile_engine_refractory_shorter_than_or_equal_to_a_block(small_fly, t_re
     idx, rate = fly_inputs(fly)
     for b in (gb, ob):
         b.set_input(idx, rate)
-    gb.set_tune(upd_threads=512, block_steps=18, tile_source=source)
+    gb.set_tune(upd_threads=512, block_steps=18, tile_source=source, block_parts=parts)
     steps = 360
     cg, rg = gb.run_steps(steps, raster=True)
     co, ro = ob.run_steps(steps, raster=True)
