@@ -633,7 +633,8 @@ __device__ __forceinline__ int once_steps(const int* row, int K, float& u_io, fl
 // in the segment table for every spike of the fly.
 // PL: plastic synapses are pulled here, with the fly's own weights, for spikes that arrive in
 // the window they were emitted in (a.ptp); only the others are pushed at spike time.
-template <typename WT, int NPT, int IPT, bool BK, bool PL>
+// PARTS: 2 runs each block as two parts (half the arrival columns in LDS), 1 as one.
+template <typename WT, int NPT, int IPT, bool BK, bool PL, int PARTS>
 __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
     extern __shared__ __attribute__((aligned(16))) int lds[];
     __shared__ int s_off[KMAX + 1];
@@ -762,13 +763,16 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
             // entries nf of the front part (first part), then nbk of the back part (second part)
             const int nf = k0 == 0 ? bcf : 0, nbk = k1 == a.K ? bcb : 0;
             const int total = nf + nbk, boff = bcap - bcb - nf;  // entry e >= nf sits at e + boff
-            if constexpr (!PL && std::is_same<WT, PackedEdge>::value && BK_PIECE == 2) {
+            if constexpr (std::is_same<WT, PackedEdge>::value && BK_PIECE == 2) {
                 // Pieces hold at most two edges (tile segments are short: ~1 edge per (row, tile) on a
                 // FlyWire-shaped graph), so each thread takes whole pieces: no scan and no search.
                 // Rounds of BK_ROUND entries per thread: every entry load first, then both edges of
                 // every piece (the second from the next packed edge, a valid address even past the
-                // last fixed edge: the packed array ends with one padding entry), then the LDS
-                // atomics -- two memory round trips per round and no wait in between.
+                // last fixed edge: the packed array ends with one padding entry, which plastic pieces
+                // also read), then the LDS atomics -- two memory round trips per round and no wait in
+                // between.  PL: a round with plastic pieces (first edge nnz + q) then loads their
+                // targets and the fly's weights of this window.
+                const int nnzi = (int)a.nnz;
                 for (int e0 = tid; e0 < total; e0 += BK_ROUND * TPB) {
                     int v[BK_ROUND];
 #pragma unroll
@@ -779,21 +783,50 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
                     int x0[BK_ROUND], x1[BK_ROUND];
 #pragma unroll
                     for (int r = 0; r < BK_ROUND; r++) {
-                        x0[r] = a.pk[v[r] >> 7];
-                        x1[r] = a.pk[(v[r] >> 7) + 1];
+                        const int f = v[r] >> 7;
+                        const bool fx = !PL || f < nnzi;
+                        x0[r] = a.pk[fx ? f : nnzi];
+                        x1[r] = a.pk[fx ? f + 1 : nnzi];
+                    }
+                    int l0[BK_ROUND], w0[BK_ROUND], l1[BK_ROUND], w1[BK_ROUND];
+#pragma unroll
+                    for (int r = 0; r < BK_ROUND; r++) {
+                        l0[r] = x0[r] & lmask;
+                        w0[r] = x0[r] >> a.lbits;
+                        l1[r] = x1[r] & lmask;
+                        w1[r] = (v[r] & (1 << 5)) ? x1[r] >> a.lbits : 0;
+                    }
+                    if constexpr (PL) {
+                        bool pr[BK_ROUND], anyp = false;
+#pragma unroll
+                        for (int r = 0; r < BK_ROUND; r++) {
+                            pr[r] = e0 + r * TPB < total && (v[r] >> 7) >= nnzi;
+                            anyp = anyp || pr[r];
+                        }
+                        if (__ballot(anyp)) {
+#pragma unroll
+                            for (int r = 0; r < BK_ROUND; r++) {
+                                if (!pr[r]) continue;
+                                const int q = (v[r] >> 7) - nnzi;
+                                l0[r] = a.pl_post[q] - i0;
+                                w0[r] = (int)rintf(pwf[q] * a.pw_qscale);
+                                if (v[r] & (1 << 5)) {
+                                    l1[r] = a.pl_post[q + 1] - i0;
+                                    w1[r] = (int)rintf(pwf[q + 1] * a.pw_qscale);
+                                }
+                            }
+                        }
                     }
 #pragma unroll
                     for (int r = 0; r < BK_ROUND; r++) {
                         if (e0 + r * TPB >= total) continue;
                         const int col = (v[r] & 31) - k0;
-                        const int v0 = x0[r] >> a.lbits, v1 = x1[r] >> a.lbits;
-                        if (v0 != 0) atomicAdd(&acc[__umul24((unsigned)(x0[r] & lmask), (unsigned)KP) + col], v0);
-                        if ((v[r] & (1 << 5)) && v1 != 0)
-                            atomicAdd(&acc[__umul24((unsigned)(x1[r] & lmask), (unsigned)KP) + col], v1);
+                        if (w0[r] != 0) atomicAdd(&acc[__umul24((unsigned)l0[r], (unsigned)KP) + col], w0[r]);
+                        if (w1[r] != 0) atomicAdd(&acc[__umul24((unsigned)l1[r], (unsigned)KP) + col], w1[r]);
                     }
                 }
             } else {
-                // The general form (plastic edges, unpacked formats): the two entry loads and the
+                // The general form (unpacked formats): the two entry loads and the
                 // first edge of each piece are issued unconditionally from clamped, valid indices and
                 // masked afterwards, so they are in flight together; the further edges of a piece are
                 // loaded on demand.
@@ -970,7 +1003,7 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
     // A block runs as one part, or as two (steps [0, kh) and [kh, K)) when its arrivals of all K
     // steps would not leave LDS for enough workgroups per CU: acc then holds one part's columns
     // (and the hit masks, read into registers before the first part's steps).
-    const int KH = a.kh;
+    const int KH = PARTS == 2 ? a.kh : a.K;
     pull(0, KH);
 
     // Poisson hits of the tile's input neurons for all K steps, as K-bit masks in column KH of
@@ -1055,9 +1088,9 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
                     int rend;
                     // the block lengths of dt = 0.1 ms and 0.25 ms (n_delay 18 and 7), and the parts
                     // of the former, unrolled
-                    if (kn == 18) rend = once_steps<18>(row, kn, u, g, r0, bias, hits, a.c);
-                    else if (kn == 9) rend = once_steps<9>(row, kn, u, g, r0, bias, hits, a.c);
-                    else if (kn == 7) rend = once_steps<7>(row, kn, u, g, r0, bias, hits, a.c);
+                    if (PARTS == 1 && kn == 18) rend = once_steps<18>(row, kn, u, g, r0, bias, hits, a.c);
+                    else if (PARTS == 1 && kn == 7) rend = once_steps<7>(row, kn, u, g, r0, bias, hits, a.c);
+                    else if (PARTS == 2 && kn == 9) rend = once_steps<9>(row, kn, u, g, r0, bias, hits, a.c);
                     else rend = once_steps<0>(row, kn, u, g, r0, bias, hits, a.c);
                     if (rend != r0) masks[p] |= 1u << (rend - 1 - a.c.n_ref + k0);
                     rr[p] = rend + k0;
@@ -1087,13 +1120,15 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
         }
     };
     steps(0, KH);
-    if (KH < a.K) {  // the second part: acc cleared and refilled with the arrivals of [KH, K)
-        __syncthreads();
-        for (int q = tid; q < (KP * TS) >> 2; q += TPB) reinterpret_cast<int4*>(acc)[q] = make_int4(0, 0, 0, 0);
-        __syncthreads();
-        pull(KH, a.K);
-        __syncthreads();
-        steps(KH, a.K);
+    if constexpr (PARTS == 2) {
+        if (KH < a.K) {  // the second part: acc cleared and refilled with the arrivals of [KH, K)
+            __syncthreads();
+            for (int q = tid; q < (KP * TS) >> 2; q += TPB) reinterpret_cast<int4*>(acc)[q] = make_int4(0, 0, 0, 0);
+            __syncthreads();
+            pull(KH, a.K);
+            __syncthreads();
+            steps(KH, a.K);
+        }
     }
     // Spikes per block step (LDS), then one global atomic per step reserves their places in the
     // per-step lists; its latency overlaps the state stores that follow.
@@ -1276,6 +1311,9 @@ struct BucketArgs {
 // window, which lif_tile pulls with the fly's own weights.
 template <bool T16, bool PL>
 __global__ void __launch_bounds__(BK_THREADS, 8) tile_bucket(BucketArgs a) {
+    // spikes per batch: with plastic pieces fewer, so that the kernel keeps to its 64 VGPRs
+    // without spilling
+    constexpr int BU = PL ? BK_UNROLL / 2 : BK_UNROLL;
     __shared__ int s_off[KMAX + 1];
     __shared__ int s_cnt[64];          // entries written to the front of each tile's bucket
     __shared__ int s_cntb[64];         // entries written to its back
@@ -1322,12 +1360,20 @@ __global__ void __launch_bounds__(BK_THREADS, 8) tile_bucket(BucketArgs a) {
     const int nt1 = a.n_tiles + 1;
     const int Tc = own ? T : 0;
     // every thread, between barriers: stages to memory (entries of steps < kh from the front of
-    // the bucket up, the others from its back down, positions by ballots), counters advanced
+    // the bucket up, the others from its back down, positions by ballots; a block of one part has
+    // only front entries, copied as they are), counters advanced
+    const bool split = a.kh < a.K;
     auto flush = [&]() {
         for (int t = wave; t < 64; t += W) {
             const int n = s_fill[t] < BK_STAGE ? s_fill[t] : BK_STAGE;
             const int f0 = s_cnt[t], b0 = s_cntb[t], cp = s_cap[t];
             int* d = a.bk + s_base[t];
+            if (!split) {
+                for (int j = lane; j < n; j += 64)
+                    if (f0 + j < cp) d[f0 + j] = s_stage[t * BK_SP + j];
+                if (lane == 0) s_cnt[t] = f0 + n;
+                continue;
+            }
             int nf = 0, nbk = 0;  // wavefront-uniform
             for (int j0 = 0; j0 < n; j0 += 64) {
                 const int j = j0 + lane;
@@ -1365,27 +1411,27 @@ __global__ void __launch_bounds__(BK_THREADS, 8) tile_bucket(BucketArgs a) {
             s_id[q] = (unsigned)id < (unsigned)a.N ? id << 5 | k : -1;  // id < 2^22 (table < 2^31 entries)
         }
         __syncthreads();
-        for (int r0 = 0; r0 < nc; r0 += W * BK_UNROLL) {
-            const int q0 = r0 + wave * BK_UNROLL;
+        for (int r0 = 0; r0 < nc; r0 += W * BU) {
+            const int q0 = r0 + wave * BU;
             if (q0 < nc) {
-                // The BK_UNROLL spikes are the same for every lane of the wavefront (lane = tile):
+                // The BU spikes are the same for every lane of the wavefront (lane = tile):
                 // their ids are made scalar, so the row addresses are scalar arithmetic and each
                 // lane's table reads are one scalar base plus its fixed tile offset, all in flight
                 // together.  Past the chunk end a lane rereads a valid spike and masks the result.
-                int lo[BK_UNROLL], len[BK_UNROLL], kk[BK_UNROLL], id[BK_UNROLL];
-                bool use[BK_UNROLL];
+                int lo[BU], len[BU], kk[BU], id[BU];
+                bool use[BU];
 #pragma unroll
-                for (int u = 0; u < BK_UNROLL; u++) {
+                for (int u = 0; u < BU; u++) {
                     const int q = q0 + u;
                     const int v = __builtin_amdgcn_readfirstlane(s_id[q < nc ? q : q0]);
                     use[u] = own && q < nc && v >= 0;
                     id[u] = v >= 0 ? v >> 5 : 0;
                     kk[u] = v & 31;
                 }
-                int l16[BK_UNROLL], h16[BK_UNROLL], r0_[BK_UNROLL];
-                int2 pq[PL ? BK_UNROLL : 1];
+                int l16[BU], h16[BU], r0_[BU];
+                int2 pq[PL ? BU : 1];
 #pragma unroll
-                for (int u = 0; u < BK_UNROLL; u++) {  // every load first ...
+                for (int u = 0; u < BU; u++) {  // every load first ...
                     if constexpr (PL) pq[u] = a.ptp[(unsigned)id[u] * (unsigned)a.n_tiles + (unsigned)Tc];
                     if constexpr (T16) {
                         const uint16_t* row = a.tb16 + (unsigned)id[u] * (unsigned)nt1;  // < 2^31 (checked)
@@ -1400,7 +1446,7 @@ __global__ void __launch_bounds__(BK_THREADS, 8) tile_bucket(BucketArgs a) {
                     }
                 }
 #pragma unroll
-                for (int u = 0; u < BK_UNROLL; u++) {  // ... then the arithmetic
+                for (int u = 0; u < BU; u++) {  // ... then the arithmetic
                     lo[u] = r0_[u] + l16[u];
                     len[u] = use[u] && h16[u] > l16[u] ? h16[u] - l16[u] : 0;
                 }
@@ -1408,11 +1454,11 @@ __global__ void __launch_bounds__(BK_THREADS, 8) tile_bucket(BucketArgs a) {
                 // reservation per lane for the batch; what does not fit takes bucket positions now
                 int tot = 0;
 #pragma unroll
-                for (int u = 0; u < BK_UNROLL; u++) tot += (int)(((unsigned)len[u] + (BK_PIECE - 1)) / BK_PIECE);
-                int plo[PL ? BK_UNROLL : 1], plen[PL ? BK_UNROLL : 1];
+                for (int u = 0; u < BU; u++) tot += (int)(((unsigned)len[u] + (BK_PIECE - 1)) / BK_PIECE);
+                int plo[PL ? BU : 1], plen[PL ? BU : 1];
                 if constexpr (PL) {
 #pragma unroll
-                    for (int u = 0; u < BK_UNROLL; u++) {  // emitted at window step kb - n_delay + k
+                    for (int u = 0; u < BU; u++) {  // emitted at window step kb - n_delay + k
                         plo[u] = (int)a.nnz + pq[u].x;
                         plen[u] = use[u] && a.kb + kk[u] - nd >= 0 ? pq[u].y : 0;
                         tot += (int)(((unsigned)plen[u] + (BK_PIECE - 1)) / BK_PIECE);
@@ -1420,16 +1466,23 @@ __global__ void __launch_bounds__(BK_THREADS, 8) tile_bucket(BucketArgs a) {
                 }
                 if (tot > 0) {
                     int slot = atomicAdd(&s_fill[lane], tot);
-                    if (slot + tot > BK_STAGE - W * BK_UNROLL) s_flush = 1;
+                    if (slot + tot > BK_STAGE - W * BU) s_flush = 1;
+                    // past a full stage: straight to memory (one front reservation for the batch when
+                    // the block has one part, else one per entry and part)
+                    const int over = slot + tot - BK_STAGE;
+                    int g = !split && over > 0 ? atomicAdd(&s_cnt[lane], over < tot ? over : tot) : 0;
                     auto put = [&](int entry) {
                         if (slot < BK_STAGE) {
                             s_stage[lane * BK_SP + slot] = entry;
-                        } else if ((entry & 31) < a.kh) {  // past a full stage: straight to memory
-                            const int g = atomicAdd(&s_cnt[lane], 1);
+                        } else if (!split) {
                             if (g < capi) dst[g] = entry;  // past the capacity: dropped, flagged below
+                            g++;
+                        } else if ((entry & 31) < a.kh) {
+                            const int gf = atomicAdd(&s_cnt[lane], 1);
+                            if (gf < capi) dst[gf] = entry;
                         } else {
-                            const int g = capi - 1 - atomicAdd(&s_cntb[lane], 1);
-                            if (g >= 0) dst[g] = entry;
+                            const int gb = capi - 1 - atomicAdd(&s_cntb[lane], 1);
+                            if (gb >= 0) dst[gb] = entry;
                         }
                         slot++;
                     };
@@ -1439,11 +1492,11 @@ __global__ void __launch_bounds__(BK_THREADS, 8) tile_bucket(BucketArgs a) {
                             put(((first + c) << 7) | (((n - c < BK_PIECE ? n - c : BK_PIECE) - 1) << 5) | k);
                     };
 #pragma unroll
-                    for (int u = 0; u < BK_UNROLL; u++)
+                    for (int u = 0; u < BU; u++)
                         if (len[u] > 0) pieces(lo[u], len[u], kk[u]);
                     if constexpr (PL) {
 #pragma unroll
-                        for (int u = 0; u < BK_UNROLL; u++)
+                        for (int u = 0; u < BU; u++)
                             if (plen[u] > 0) pieces(plo[u], plen[u], kk[u]);
                     }
                 }
@@ -1572,6 +1625,7 @@ int resolve(const fz_engine* e, const fz_tune* t, Shape* s) {
     s->pl = 0;
     s->parts = (t && t->block_parts) ? t->block_parts : 1;
     if (s->parts != 1 && s->parts != 2) return fail(FZ_E_ARG, "block_parts %d: must be 1 or 2", s->parts);
+    if (!(e->mode == 2 && e->pk)) s->parts = 1;  // two parts are compiled for packed edges
     if (t && t->plastic_pull != 0 && t->plastic_pull != 1)
         return fail(FZ_E_ARG, "plastic_pull %d: must be 0 or 1", t->plastic_pull);
     if (t && t->plastic_pull && e->mode == 2 && e->n_pl > 0 && e->ptp && e->pk) s->pl = 1;
@@ -1741,37 +1795,41 @@ TileArgs make_tile(const fz_engine* e, const fz_inputs* in, int K, int KP, int k
     return a;
 }
 
-template <typename WT, int IPT, bool BK, bool PL>
+template <typename WT, int IPT, bool BK, bool PL, int PARTS>
 void* tile_kernel_npt(int npt) {
     switch (npt) {
-        case 1: return reinterpret_cast<void*>(&lif_tile<WT, 1, IPT, BK, PL>);
-        case 2: return reinterpret_cast<void*>(&lif_tile<WT, 2, IPT, BK, PL>);
-        case 4: return reinterpret_cast<void*>(&lif_tile<WT, 4, IPT, BK, PL>);
-        default: return reinterpret_cast<void*>(&lif_tile<WT, 8, IPT, BK, PL>);
+        case 1: return reinterpret_cast<void*>(&lif_tile<WT, 1, IPT, BK, PL, PARTS>);
+        case 2: return reinterpret_cast<void*>(&lif_tile<WT, 2, IPT, BK, PL, PARTS>);
+        case 4: return reinterpret_cast<void*>(&lif_tile<WT, 4, IPT, BK, PL, PARTS>);
+        default: return reinterpret_cast<void*>(&lif_tile<WT, 8, IPT, BK, PL, PARTS>);
     }
 }
 
-template <typename WT, bool PL>
+// (the staging depth only shapes the segment-table source: the bucket source has one kernel)
+template <typename WT, bool PL, int PARTS>
 void* tile_kernel_src(int npt, int ipt, int src) {
-    if (src == 1) return ipt == 1 ? tile_kernel_npt<WT, 1, true, PL>(npt) : tile_kernel_npt<WT, 4, true, PL>(npt);
-    return ipt == 1 ? tile_kernel_npt<WT, 1, false, PL>(npt) : tile_kernel_npt<WT, 4, false, PL>(npt);
+    if (src == 1) return tile_kernel_npt<WT, 1, true, PL, PARTS>(npt);
+    return ipt == 1 ? tile_kernel_npt<WT, 1, false, PL, PARTS>(npt) : tile_kernel_npt<WT, 4, false, PL, PARTS>(npt);
 }
 
 // Plastic pulls are compiled for packed edges only (the format of every connectome whose tile
 // indices fit beside the weights); with the others plastic rows are pushed.
+// Blocks in two parts likewise (with the other formats a block runs as one part).
 template <typename WT>
-void* tile_kernel_fmt(int npt, int ipt, int src, int pl) {
-    if constexpr (std::is_same<WT, PackedEdge>::value)
-        if (pl) return tile_kernel_src<WT, true>(npt, ipt, src);
-    return tile_kernel_src<WT, false>(npt, ipt, src);
+void* tile_kernel_fmt(int npt, int ipt, int src, int pl, int parts) {
+    if constexpr (std::is_same<WT, PackedEdge>::value) {
+        if (parts == 2) return pl ? tile_kernel_src<WT, true, 2>(npt, ipt, src) : tile_kernel_src<WT, false, 2>(npt, ipt, src);
+        if (pl) return tile_kernel_src<WT, true, 1>(npt, ipt, src);
+    }
+    return tile_kernel_src<WT, false, 1>(npt, ipt, src);
 }
 
 // The lif_tile instantiation for an edge format (0: int16 col/w, 1: int32 col/w, 2: packed),
 // neurons per thread, staging depth, segment source and plastic pulls.
-void* tile_kernel(int fmt, int npt, int ipt, int src, int pl) {
-    if (fmt == 2) return tile_kernel_fmt<PackedEdge>(npt, ipt, src, pl);
-    if (fmt == 0) return tile_kernel_fmt<int16_t>(npt, ipt, src, 0);
-    return tile_kernel_fmt<int32_t>(npt, ipt, src, 0);
+void* tile_kernel(int fmt, int npt, int ipt, int src, int pl, int parts) {
+    if (fmt == 2) return tile_kernel_fmt<PackedEdge>(npt, ipt, src, pl, parts);
+    if (fmt == 0) return tile_kernel_fmt<int16_t>(npt, ipt, src, 0, 1);
+    return tile_kernel_fmt<int32_t>(npt, ipt, src, 0, 1);
 }
 int tile_fmt(const fz_engine* e) { return e->pk ? 2 : (e->w_bits == 16 ? 0 : 1); }
 
@@ -2123,7 +2181,7 @@ int enqueue(fz_engine* e, const fz_inputs* in, int steps, const unsigned* hits, 
     if (e->mode == 2) {
         const dim3 tgrid(e->n_tiles, e->st.B);
         const dim3 bgrid((e->n_tiles + 63) / 64, e->st.B);
-        void* fn = tile_kernel(tile_fmt(e), e->ts / s.ut, s.ipt, s.src, s.pl);
+        void* fn = tile_kernel(tile_fmt(e), e->ts / s.ut, s.ipt, s.src, s.pl, s.parts);
         if (s.src == 1 && (rc = ensure_buckets(e))) return rc;
         while (kb < steps) {
             const int K = (steps - kb) < s.K ? (steps - kb) : s.K;
@@ -2453,16 +2511,17 @@ int fz_set_tiles(fz_engine* e, int tile, int n_tiles, const int* tb, const int* 
         for (int npt : {1, 2, 4, 8})
             for (int ipt : {1, 4})
                 for (int src : {0, 1})
-                    for (int pl : {0, 1}) {
-                        hipError_t r = hipFuncSetAttribute(tile_kernel(fmt, npt, ipt, src, pl),
-                                                           hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                           (int)(e->lds_max - TILE_STATIC_LDS));
-                        if (r != hipSuccess) {
-                            (void)hipGetLastError();  // do not leave a sticky error for the caller's next HIP call
-                            return fail(FZ_E_HIP, "hipFuncSetAttribute(max dynamic LDS %zu) failed: %s",
-                                        e->lds_max - TILE_STATIC_LDS, hipGetErrorString(r));
+                    for (int pl : {0, 1})
+                        for (int parts : {1, 2}) {
+                            hipError_t r = hipFuncSetAttribute(tile_kernel(fmt, npt, ipt, src, pl, parts),
+                                                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                               (int)(e->lds_max - TILE_STATIC_LDS));
+                            if (r != hipSuccess) {
+                                (void)hipGetLastError();  // no sticky error for the caller's next HIP call
+                                return fail(FZ_E_HIP, "hipFuncSetAttribute(max dynamic LDS %zu) failed: %s",
+                                            e->lds_max - TILE_STATIC_LDS, hipGetErrorString(r));
+                            }
                         }
-                    }
     int rc = validate_tiles(e, tb, n_tiles, tile);
     if (rc) return rc;
     // from here on a failure leaves the engine unbound, in push mode, until set up again
@@ -2593,7 +2652,7 @@ int fz_graph_create(fz_engine* e, const fz_inputs* in, int steps, const fz_tune*
     int kb = 0, blk = 0;
     std::vector<hipGraphNode_t> lasts;  // tile engine: the last node of each chain
     if (e->mode == 2) {
-        void* fn = tile_kernel(tile_fmt(e), e->ts / s.ut, s.ipt, s.src, s.pl);
+        void* fn = tile_kernel(tile_fmt(e), e->ts / s.ut, s.ipt, s.src, s.pl, s.parts);
         if (s.src == 1 && (rc = ensure_buckets(e))) {
             (void)hipGraphDestroy(gr->graph);
             delete gr;
