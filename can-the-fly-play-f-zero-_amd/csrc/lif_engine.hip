@@ -638,6 +638,7 @@ template <typename WT, int NPT, int IPT, bool BK, bool PL, int PARTS>
 __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
     extern __shared__ __attribute__((aligned(16))) int lds[];
     __shared__ int s_off[KMAX + 1];
+    __shared__ int s_off2[KMAX + 1];
     __shared__ int s_lcnt[KMAX];
     __shared__ int s_wsum[MAX_WAVES + 1];
     const int TPB = blockDim.x, tid = threadIdx.x, lane = tid & 63;
@@ -721,7 +722,23 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
         bcb = c.y < bcap - bcf ? c.y : bcap - bcf;
         bkp4 = a.bk + ((size_t)par * a.B + b) * a.bk_stride + bko0;
         __syncthreads();
-    } else {
+    }
+    // Sources 0 and 2: the first TPB * IPT list entries of the block's first part are loaded
+    // speculatively, X0 positions of every source step (thread slot q = tid * IPT + r takes step
+    // q / X0, position q % X0, from a valid, clamped address), together with the list lengths, so
+    // the two loads overlap; entries past a list's length are masked once the lengths are known,
+    // and lists longer than X0 are finished by the general loop.
+    const int KH0 = PARTS == 2 ? a.kh : a.K;  // steps of the first part
+    int spec_ids[IPT];
+    if constexpr (!BK) {
+        const int X0 = TPB * IPT / KH0;
+#pragma unroll
+        for (int r = 0; r < IPT; r++) {
+            const int q = tid * IPT + r;
+            const int kq = q / X0 < KH0 ? q / X0 : KH0 - 1;
+            const int pos = q - (q / X0) * X0 < a.N ? q - (q / X0) * X0 : a.N - 1;
+            spec_ids[r] = idx_b[(size_t)((lt0 + kq - nd + M) % M) * a.N + pos];
+        }
         source_offsets(cnt_b, t0, lt0, a.K, nd, M, a.N, s_off);
     }
     // the ext-slot loads, also from valid addresses (a neuron without a slot reads slot 0); the
@@ -865,29 +882,58 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
                 }
             }
         } else {
-            // The list entries of source steps k0 .. k1 - 1 in chunks of TPB * IPT; each thread owns
-            // IPT consecutive entries so that all of its list and table loads are in flight
+            // The list entries of source steps k0 .. k1 - 1 in chunks of TPB * IPT: first the first X
+            // positions of every step's list (for the first part loaded at set-up, beside the list
+            // lengths), then the rest of the longer lists (offsets s_off2 over what is left of each).
+            // Each thread owns IPT entries so that all of its list and table loads are in flight
             // together; segments (any length) are staged with their prefix sums and the edges split
             // evenly over the threads.
-            const int lo = s_off[k0], hi = s_off[k1];
-            for (int base = lo; base < hi; base += TPB * IPT) {
+            const int kn = k1 - k0, X = TPB * IPT / kn;
+            if (tid == 0) {
+                int run = 0;
+                for (int k = 0; k < kn; k++) {
+                    s_off2[k] = run;
+                    const int c = s_off[k0 + k + 1] - s_off[k0 + k];
+                    run += c > X ? c - X : 0;
+                }
+                s_off2[kn] = run;
+            }
+            __syncthreads();
+            const int rest = s_off2[kn];
+            for (int base = -TPB * IPT; base < rest; base += TPB * IPT) {  // base < 0: the first X of each
                 int seg[IPT], len[IPT], kk[IPT];
                 int sum = 0;
                 int ids[IPT];
                 // loads from clamped, valid positions (results masked afterwards) so that the IPT
                 // list reads, then the IPT table reads, of a thread are in flight together
                 bool ok[IPT];
-                size_t li_[IPT];
+                if (base < 0) {
 #pragma unroll
-                for (int r = 0; r < IPT; r++) {  // list positions first (LDS only) ...
-                    const int e = base + tid * IPT + r;
-                    ok[r] = e < hi;
-                    const int ec = ok[r] ? e : base;
-                    kk[r] = entry_step(s_off, a.K, ec);
-                    li_[r] = (size_t)((lt0 + kk[r] - nd + M) % M) * a.N + (ec - s_off[kk[r]]);
+                    for (int r = 0; r < IPT; r++) {
+                        const int q = tid * IPT + r, kq = q / X;
+                        kk[r] = k0 + (kq < kn ? kq : kn - 1);
+                        ok[r] = kq < kn && q - kq * X < s_off[kk[r] + 1] - s_off[kk[r]];
+                        if (k0 == 0 && X == TPB * IPT / KH0) {
+                            ids[r] = spec_ids[r];
+                        } else {
+                            const int pos = q - kq * X < a.N ? q - kq * X : a.N - 1;
+                            ids[r] = idx_b[(size_t)((lt0 + kk[r] - nd + M) % M) * a.N + pos];
+                        }
+                    }
+                } else {
+                    size_t li_[IPT];
+#pragma unroll
+                    for (int r = 0; r < IPT; r++) {  // list positions first (LDS only) ...
+                        const int e = base + tid * IPT + r;
+                        ok[r] = e < rest;
+                        const int ec = ok[r] ? e : base;
+                        const int kq = entry_step(s_off2, kn, ec);
+                        kk[r] = k0 + kq;
+                        li_[r] = (size_t)((lt0 + kk[r] - nd + M) % M) * a.N + (X + ec - s_off2[kq]);
+                    }
+#pragma unroll
+                    for (int r = 0; r < IPT; r++) ids[r] = idx_b[li_[r]];  // ... then all list loads at once
                 }
-#pragma unroll
-                for (int r = 0; r < IPT; r++) ids[r] = idx_b[li_[r]];  // ... then all list loads at once
                 const unsigned nt1 = (unsigned)(a.n_tiles + 1);
                 int2 pq[PL ? IPT : 1];  // plastic segments, loaded beside the fixed ones
                 if constexpr (PL) {
