@@ -69,7 +69,7 @@ constexpr int KMAX = 32;          // steps per block held in registers
 constexpr int MAX_THREADS = 1024;
 constexpr int MAX_WAVES = MAX_THREADS / 64;
 constexpr int QMIN = 64;          // smallest push chunk (sizes the chunk-start table)
-constexpr int BK_PIECE = 2;  // edges per tile-bucket entry at most (longer segments are split)
+constexpr int BK_PIECE = 4;  // edges per tile-bucket entry at most (longer segments are split)
 constexpr int BK_ROUND = 4;  // tile-bucket entries per thread and round in lif_tile
 constexpr size_t TILE_STATIC_LDS = 1024;  // room left for lif_tile's static shared arrays
 constexpr size_t TP_MAX_BYTES = 1ull << 30;  // largest (first edge, length) pair table kept
@@ -780,15 +780,16 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
             // entries nf of the front part (first part), then nbk of the back part (second part)
             const int nf = k0 == 0 ? bcf : 0, nbk = k1 == a.K ? bcb : 0;
             const int total = nf + nbk, boff = bcap - bcb - nf;  // entry e >= nf sits at e + boff
-            if constexpr (std::is_same<WT, PackedEdge>::value && BK_PIECE == 2) {
-                // Pieces hold at most two edges (tile segments are short: ~1 edge per (row, tile) on a
-                // FlyWire-shaped graph), so each thread takes whole pieces: no scan and no search.
-                // Rounds of BK_ROUND entries per thread: every entry load first, then both edges of
-                // every piece (the second from the next packed edge, a valid address even past the
-                // last fixed edge: the packed array ends with one padding entry, which plastic pieces
-                // also read), then the LDS atomics -- two memory round trips per round and no wait in
-                // between.  PL: a round with plastic pieces (first edge nnz + q) then loads their
-                // targets and the fly's weights of this window.
+            if constexpr (std::is_same<WT, PackedEdge>::value) {
+                // Pieces hold at most BK_PIECE edges (tile segments are short: ~1 edge per (row, tile)
+                // on a FlyWire-shaped graph), so each thread takes whole pieces: no scan and no search.
+                // Rounds of BK_ROUND entries per thread: every entry load first, then all BK_PIECE
+                // edges of every piece from consecutive packed edges (valid addresses even past the
+                // last fixed edge: the packed array ends with BK_PIECE - 1 padding entries, which
+                // plastic pieces also read; edges past the piece's length are masked), then the LDS
+                // atomics -- two memory round trips per round and no wait in between.  PL: a round
+                // with plastic pieces (first edge nnz + q) then loads their targets and the fly's
+                // weights of this window.
                 const int nnzi = (int)a.nnz;
                 for (int e0 = tid; e0 < total; e0 += BK_ROUND * TPB) {
                     int v[BK_ROUND];
@@ -797,21 +798,23 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
                         const int e = e0 + r * TPB < total ? e0 + r * TPB : e0;
                         v[r] = bkp4[e < nf ? e : e + boff];  // first edge << 7 | (len - 1) << 5 | step
                     }
-                    int x0[BK_ROUND], x1[BK_ROUND];
+                    int x[BK_ROUND][BK_PIECE];
 #pragma unroll
                     for (int r = 0; r < BK_ROUND; r++) {
                         const int f = v[r] >> 7;
                         const bool fx = !PL || f < nnzi;
-                        x0[r] = a.pk[fx ? f : nnzi];
-                        x1[r] = a.pk[fx ? f + 1 : nnzi];
+#pragma unroll
+                        for (int j = 0; j < BK_PIECE; j++) x[r][j] = a.pk[fx ? f + j : nnzi];
                     }
-                    int l0[BK_ROUND], w0[BK_ROUND], l1[BK_ROUND], w1[BK_ROUND];
+                    int l[BK_ROUND][BK_PIECE], w[BK_ROUND][BK_PIECE];
 #pragma unroll
                     for (int r = 0; r < BK_ROUND; r++) {
-                        l0[r] = x0[r] & lmask;
-                        w0[r] = x0[r] >> a.lbits;
-                        l1[r] = x1[r] & lmask;
-                        w1[r] = (v[r] & (1 << 5)) ? x1[r] >> a.lbits : 0;
+                        const int len = ((v[r] >> 5) & 3) + 1;
+#pragma unroll
+                        for (int j = 0; j < BK_PIECE; j++) {
+                            l[r][j] = x[r][j] & lmask;
+                            w[r][j] = j < len ? x[r][j] >> a.lbits : 0;
+                        }
                     }
                     if constexpr (PL) {
                         bool pr[BK_ROUND], anyp = false;
@@ -824,12 +827,12 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
 #pragma unroll
                             for (int r = 0; r < BK_ROUND; r++) {
                                 if (!pr[r]) continue;
-                                const int q = (v[r] >> 7) - nnzi;
-                                l0[r] = a.pl_post[q] - i0;
-                                w0[r] = (int)rintf(pwf[q] * a.pw_qscale);
-                                if (v[r] & (1 << 5)) {
-                                    l1[r] = a.pl_post[q + 1] - i0;
-                                    w1[r] = (int)rintf(pwf[q + 1] * a.pw_qscale);
+                                const int q = (v[r] >> 7) - nnzi, len = ((v[r] >> 5) & 3) + 1;
+#pragma unroll
+                                for (int j = 0; j < BK_PIECE; j++) {
+                                    if (j >= len) continue;
+                                    l[r][j] = a.pl_post[q + j] - i0;
+                                    w[r][j] = (int)rintf(pwf[q + j] * a.pw_qscale);
                                 }
                             }
                         }
@@ -838,8 +841,9 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
                     for (int r = 0; r < BK_ROUND; r++) {
                         if (e0 + r * TPB >= total) continue;
                         const int col = (v[r] & 31) - k0;
-                        if (w0[r] != 0) atomicAdd(&acc[__umul24((unsigned)l0[r], (unsigned)KP) + col], w0[r]);
-                        if (w1[r] != 0) atomicAdd(&acc[__umul24((unsigned)l1[r], (unsigned)KP) + col], w1[r]);
+#pragma unroll
+                        for (int j = 0; j < BK_PIECE; j++)
+                            if (w[r][j] != 0) atomicAdd(&acc[__umul24((unsigned)l[r][j], (unsigned)KP) + col], w[r][j]);
                     }
                 }
             } else {
@@ -2498,10 +2502,10 @@ int pack_edges(fz_engine* e, int tile) {
     while ((1 << lbits) < tile) lbits++;
     int* pk = nullptr;
     int* over = nullptr;
-    // one padding entry (zero) after the last edge: the tile kernel reads the edge after a
-    // piece's first one unconditionally
-    if (hipMalloc(&pk, ((size_t)e->nnz + 1) * sizeof(int)) != hipSuccess ||
-        hipMemset(pk + e->nnz, 0, sizeof(int)) != hipSuccess) {
+    // BK_PIECE - 1 padding entries (zero) after the last edge: the tile kernel reads the edges
+    // after a piece's first one unconditionally
+    if (hipMalloc(&pk, ((size_t)e->nnz + BK_PIECE) * sizeof(int)) != hipSuccess ||
+        hipMemset(pk + e->nnz, 0, BK_PIECE * sizeof(int)) != hipSuccess) {
         (void)hipGetLastError();
         (void)hipFree(pk);
         return fail(FZ_E_HIP, "allocating %lld packed edges failed", e->nnz);
