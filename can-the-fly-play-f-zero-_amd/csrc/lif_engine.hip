@@ -69,7 +69,7 @@ constexpr int KMAX = 32;          // steps per block held in registers
 constexpr int MAX_THREADS = 1024;
 constexpr int MAX_WAVES = MAX_THREADS / 64;
 constexpr int QMIN = 64;          // smallest push chunk (sizes the chunk-start table)
-constexpr int BK_PIECE = 4;  // edges per tile-bucket entry at most (longer segments are split)
+constexpr int BK_PIECE = 2;  // edges per tile-bucket entry at most (longer segments are split)
 constexpr int BK_ROUND = 4;  // tile-bucket entries per thread and round in lif_tile
 constexpr size_t TILE_STATIC_LDS = 1024;  // room left for lif_tile's static shared arrays
 constexpr size_t TP_MAX_BYTES = 1ull << 30;  // largest (first edge, length) pair table kept
@@ -638,7 +638,6 @@ template <typename WT, int NPT, int IPT, bool BK, bool PL, int PARTS>
 __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
     extern __shared__ __attribute__((aligned(16))) int lds[];
     __shared__ int s_off[KMAX + 1];
-    __shared__ int s_off2[KMAX + 1];
     __shared__ int s_lcnt[KMAX];
     __shared__ int s_wsum[MAX_WAVES + 1];
     const int TPB = blockDim.x, tid = threadIdx.x, lane = tid & 63;
@@ -722,23 +721,7 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
         bcb = c.y < bcap - bcf ? c.y : bcap - bcf;
         bkp4 = a.bk + ((size_t)par * a.B + b) * a.bk_stride + bko0;
         __syncthreads();
-    }
-    // Sources 0 and 2: the first TPB * IPT list entries of the block's first part are loaded
-    // speculatively, X0 positions of every source step (thread slot q = tid * IPT + r takes step
-    // q / X0, position q % X0, from a valid, clamped address), together with the list lengths, so
-    // the two loads overlap; entries past a list's length are masked once the lengths are known,
-    // and lists longer than X0 are finished by the general loop.
-    const int KH0 = PARTS == 2 ? a.kh : a.K;  // steps of the first part
-    int spec_ids[IPT];
-    if constexpr (!BK) {
-        const int X0 = TPB * IPT / KH0;
-#pragma unroll
-        for (int r = 0; r < IPT; r++) {
-            const int q = tid * IPT + r;
-            const int kq = q / X0 < KH0 ? q / X0 : KH0 - 1;
-            const int pos = q - (q / X0) * X0 < a.N ? q - (q / X0) * X0 : a.N - 1;
-            spec_ids[r] = idx_b[(size_t)((lt0 + kq - nd + M) % M) * a.N + pos];
-        }
+    } else {
         source_offsets(cnt_b, t0, lt0, a.K, nd, M, a.N, s_off);
     }
     // the ext-slot loads, also from valid addresses (a neuron without a slot reads slot 0); the
@@ -780,16 +763,15 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
             // entries nf of the front part (first part), then nbk of the back part (second part)
             const int nf = k0 == 0 ? bcf : 0, nbk = k1 == a.K ? bcb : 0;
             const int total = nf + nbk, boff = bcap - bcb - nf;  // entry e >= nf sits at e + boff
-            if constexpr (std::is_same<WT, PackedEdge>::value) {
-                // Pieces hold at most BK_PIECE edges (tile segments are short: ~1 edge per (row, tile)
-                // on a FlyWire-shaped graph), so each thread takes whole pieces: no scan and no search.
-                // Rounds of BK_ROUND entries per thread: every entry load first, then all BK_PIECE
-                // edges of every piece from consecutive packed edges (valid addresses even past the
-                // last fixed edge: the packed array ends with BK_PIECE - 1 padding entries, which
-                // plastic pieces also read; edges past the piece's length are masked), then the LDS
-                // atomics -- two memory round trips per round and no wait in between.  PL: a round
-                // with plastic pieces (first edge nnz + q) then loads their targets and the fly's
-                // weights of this window.
+            if constexpr (std::is_same<WT, PackedEdge>::value && BK_PIECE == 2) {
+                // Pieces hold at most two edges (tile segments are short: ~1 edge per (row, tile) on a
+                // FlyWire-shaped graph), so each thread takes whole pieces: no scan and no search.
+                // Rounds of BK_ROUND entries per thread: every entry load first, then both edges of
+                // every piece (the second from the next packed edge, a valid address even past the
+                // last fixed edge: the packed array ends with one padding entry, which plastic pieces
+                // also read), then the LDS atomics -- two memory round trips per round and no wait in
+                // between.  PL: a round with plastic pieces (first edge nnz + q) then loads their
+                // targets and the fly's weights of this window.
                 const int nnzi = (int)a.nnz;
                 for (int e0 = tid; e0 < total; e0 += BK_ROUND * TPB) {
                     int v[BK_ROUND];
@@ -798,23 +780,21 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
                         const int e = e0 + r * TPB < total ? e0 + r * TPB : e0;
                         v[r] = bkp4[e < nf ? e : e + boff];  // first edge << 7 | (len - 1) << 5 | step
                     }
-                    int x[BK_ROUND][BK_PIECE];
+                    int x0[BK_ROUND], x1[BK_ROUND];
 #pragma unroll
                     for (int r = 0; r < BK_ROUND; r++) {
                         const int f = v[r] >> 7;
                         const bool fx = !PL || f < nnzi;
-#pragma unroll
-                        for (int j = 0; j < BK_PIECE; j++) x[r][j] = a.pk[fx ? f + j : nnzi];
+                        x0[r] = a.pk[fx ? f : nnzi];
+                        x1[r] = a.pk[fx ? f + 1 : nnzi];
                     }
-                    int l[BK_ROUND][BK_PIECE], w[BK_ROUND][BK_PIECE];
+                    int l0[BK_ROUND], w0[BK_ROUND], l1[BK_ROUND], w1[BK_ROUND];
 #pragma unroll
                     for (int r = 0; r < BK_ROUND; r++) {
-                        const int len = ((v[r] >> 5) & 3) + 1;
-#pragma unroll
-                        for (int j = 0; j < BK_PIECE; j++) {
-                            l[r][j] = x[r][j] & lmask;
-                            w[r][j] = j < len ? x[r][j] >> a.lbits : 0;
-                        }
+                        l0[r] = x0[r] & lmask;
+                        w0[r] = x0[r] >> a.lbits;
+                        l1[r] = x1[r] & lmask;
+                        w1[r] = (v[r] & (1 << 5)) ? x1[r] >> a.lbits : 0;
                     }
                     if constexpr (PL) {
                         bool pr[BK_ROUND], anyp = false;
@@ -827,12 +807,12 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
 #pragma unroll
                             for (int r = 0; r < BK_ROUND; r++) {
                                 if (!pr[r]) continue;
-                                const int q = (v[r] >> 7) - nnzi, len = ((v[r] >> 5) & 3) + 1;
-#pragma unroll
-                                for (int j = 0; j < BK_PIECE; j++) {
-                                    if (j >= len) continue;
-                                    l[r][j] = a.pl_post[q + j] - i0;
-                                    w[r][j] = (int)rintf(pwf[q + j] * a.pw_qscale);
+                                const int q = (v[r] >> 7) - nnzi;
+                                l0[r] = a.pl_post[q] - i0;
+                                w0[r] = (int)rintf(pwf[q] * a.pw_qscale);
+                                if (v[r] & (1 << 5)) {
+                                    l1[r] = a.pl_post[q + 1] - i0;
+                                    w1[r] = (int)rintf(pwf[q + 1] * a.pw_qscale);
                                 }
                             }
                         }
@@ -841,9 +821,8 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
                     for (int r = 0; r < BK_ROUND; r++) {
                         if (e0 + r * TPB >= total) continue;
                         const int col = (v[r] & 31) - k0;
-#pragma unroll
-                        for (int j = 0; j < BK_PIECE; j++)
-                            if (w[r][j] != 0) atomicAdd(&acc[__umul24((unsigned)l[r][j], (unsigned)KP) + col], w[r][j]);
+                        if (w0[r] != 0) atomicAdd(&acc[__umul24((unsigned)l0[r], (unsigned)KP) + col], w0[r]);
+                        if (w1[r] != 0) atomicAdd(&acc[__umul24((unsigned)l1[r], (unsigned)KP) + col], w1[r]);
                     }
                 }
             } else {
@@ -886,58 +865,29 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
                 }
             }
         } else {
-            // The list entries of source steps k0 .. k1 - 1 in chunks of TPB * IPT: first the first X
-            // positions of every step's list (for the first part loaded at set-up, beside the list
-            // lengths), then the rest of the longer lists (offsets s_off2 over what is left of each).
-            // Each thread owns IPT entries so that all of its list and table loads are in flight
+            // The list entries of source steps k0 .. k1 - 1 in chunks of TPB * IPT; each thread owns
+            // IPT consecutive entries so that all of its list and table loads are in flight
             // together; segments (any length) are staged with their prefix sums and the edges split
             // evenly over the threads.
-            const int kn = k1 - k0, X = TPB * IPT / kn;
-            if (tid == 0) {
-                int run = 0;
-                for (int k = 0; k < kn; k++) {
-                    s_off2[k] = run;
-                    const int c = s_off[k0 + k + 1] - s_off[k0 + k];
-                    run += c > X ? c - X : 0;
-                }
-                s_off2[kn] = run;
-            }
-            __syncthreads();
-            const int rest = s_off2[kn];
-            for (int base = -TPB * IPT; base < rest; base += TPB * IPT) {  // base < 0: the first X of each
+            const int lo = s_off[k0], hi = s_off[k1];
+            for (int base = lo; base < hi; base += TPB * IPT) {
                 int seg[IPT], len[IPT], kk[IPT];
                 int sum = 0;
                 int ids[IPT];
                 // loads from clamped, valid positions (results masked afterwards) so that the IPT
                 // list reads, then the IPT table reads, of a thread are in flight together
                 bool ok[IPT];
-                if (base < 0) {
+                size_t li_[IPT];
 #pragma unroll
-                    for (int r = 0; r < IPT; r++) {
-                        const int q = tid * IPT + r, kq = q / X;
-                        kk[r] = k0 + (kq < kn ? kq : kn - 1);
-                        ok[r] = kq < kn && q - kq * X < s_off[kk[r] + 1] - s_off[kk[r]];
-                        if (k0 == 0 && X == TPB * IPT / KH0) {
-                            ids[r] = spec_ids[r];
-                        } else {
-                            const int pos = q - kq * X < a.N ? q - kq * X : a.N - 1;
-                            ids[r] = idx_b[(size_t)((lt0 + kk[r] - nd + M) % M) * a.N + pos];
-                        }
-                    }
-                } else {
-                    size_t li_[IPT];
-#pragma unroll
-                    for (int r = 0; r < IPT; r++) {  // list positions first (LDS only) ...
-                        const int e = base + tid * IPT + r;
-                        ok[r] = e < rest;
-                        const int ec = ok[r] ? e : base;
-                        const int kq = entry_step(s_off2, kn, ec);
-                        kk[r] = k0 + kq;
-                        li_[r] = (size_t)((lt0 + kk[r] - nd + M) % M) * a.N + (X + ec - s_off2[kq]);
-                    }
-#pragma unroll
-                    for (int r = 0; r < IPT; r++) ids[r] = idx_b[li_[r]];  // ... then all list loads at once
+                for (int r = 0; r < IPT; r++) {  // list positions first (LDS only) ...
+                    const int e = base + tid * IPT + r;
+                    ok[r] = e < hi;
+                    const int ec = ok[r] ? e : base;
+                    kk[r] = entry_step(s_off, a.K, ec);
+                    li_[r] = (size_t)((lt0 + kk[r] - nd + M) % M) * a.N + (ec - s_off[kk[r]]);
                 }
+#pragma unroll
+                for (int r = 0; r < IPT; r++) ids[r] = idx_b[li_[r]];  // ... then all list loads at once
                 const unsigned nt1 = (unsigned)(a.n_tiles + 1);
                 int2 pq[PL ? IPT : 1];  // plastic segments, loaded beside the fixed ones
                 if constexpr (PL) {
@@ -2502,10 +2452,10 @@ int pack_edges(fz_engine* e, int tile) {
     while ((1 << lbits) < tile) lbits++;
     int* pk = nullptr;
     int* over = nullptr;
-    // BK_PIECE - 1 padding entries (zero) after the last edge: the tile kernel reads the edges
-    // after a piece's first one unconditionally
-    if (hipMalloc(&pk, ((size_t)e->nnz + BK_PIECE) * sizeof(int)) != hipSuccess ||
-        hipMemset(pk + e->nnz, 0, BK_PIECE * sizeof(int)) != hipSuccess) {
+    // one padding entry (zero) after the last edge: the tile kernel reads the edge after a
+    // piece's first one unconditionally
+    if (hipMalloc(&pk, ((size_t)e->nnz + 1) * sizeof(int)) != hipSuccess ||
+        hipMemset(pk + e->nnz, 0, sizeof(int)) != hipSuccess) {
         (void)hipGetLastError();
         (void)hipFree(pk);
         return fail(FZ_E_HIP, "allocating %lld packed edges failed", e->nnz);
