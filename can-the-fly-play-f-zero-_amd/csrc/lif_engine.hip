@@ -1272,7 +1272,11 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
 // spike of the fly up in the segment table.  Grid (ceil(n_tiles / 64), B) workgroups, one per
 // (tile group g, fly): lane t of every wavefront owns tile 64 g + t; the wavefronts split the
 // spikes, staged through LDS in chunks; positions come from LDS counters, so a bucket's order
-// depends on timing while its contents (and the integer sums made from them) do not.
+// depends on timing while its contents (and the integer sums made from them) do not.  A group
+// of at most 32 tiles (the last one, when n_tiles is not a multiple of 64) would leave most
+// lanes idle in every pass, so there lane so * tw + t takes tile t of the so-th of 64 / tw
+// spikes (tw = the group's tile count rounded up to a power of two): at 136 tiles the third
+// group's 8 tiles cost an eighth of a full group instead of as much.
 constexpr int BK_CHUNK = 2048;
 constexpr int BK_UNROLL = 8;         // spikes per batch: table loads in flight per lane
 constexpr int BK_THREADS = 1024;
@@ -1313,23 +1317,30 @@ template <bool T16, bool PL>
 __global__ void __launch_bounds__(BK_THREADS, 8) tile_bucket(BucketArgs a) {
     // spikes per batch: with plastic pieces fewer, so that the kernel keeps to its 64 VGPRs
     // without spilling
-    constexpr int BU = PL ? BK_UNROLL / 2 : BK_UNROLL;
+    constexpr int BUS = PL ? BK_UNROLL / 2 : BK_UNROLL;
     __shared__ int s_off[KMAX + 1];
     __shared__ int s_cnt[64];          // entries written to the front of each tile's bucket
     __shared__ int s_cntb[64];         // entries written to its back
-    __shared__ int s_fill[64];         // entries reserved in each tile's stage since the last flush
+    __shared__ int s_fill[64];         // entries reserved in each lane's stage since the last flush
     __shared__ int s_cap[64];          // each tile's bucket capacity
     __shared__ long long s_base[64];   // each tile's bucket in the arena
     __shared__ int s_flush;
     __shared__ int s_id[BK_CHUNK];
-    __shared__ int s_stage[64 * BK_SP];
+    __shared__ int s_stage[64 * BK_SP];  // one stage row per lane (the lane's tile: lane & (tw - 1))
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, W = blockDim.x >> 6;
     // the tile groups of one fly run on one XCD: they read the same spike lists, and the
     // table-row spans of neighbouring groups share cache lines
     int grp, b;
     xcd_order(gridDim.x, gridDim.y, &grp, &b);
     b += a.b0;
-    const int T = grp * 64 + lane;
+    // tiles in this group (1 .. 64); lanes per spike tw (64 unless the group has at most 32
+    // tiles: then its count rounded up to a power of two) and spikes per lane pass spl = 64 / tw
+    const int w = a.n_tiles - grp * 64 < 64 ? a.n_tiles - grp * 64 : 64;
+    int lsh = 6;
+    while (lsh > 0 && (1 << (lsh - 1)) >= w) lsh--;
+    const int tw = 1 << lsh, spl = 64 >> lsh;
+    const int tl = lane & (tw - 1), so = lane >> lsh;  // this lane's tile in the group, its spike slot
+    const int T = grp * 64 + tl;
     if (a.alive && !a.alive[b]) return;
     const long long t0 = a.clock[0] + a.kb;
     const int M = a.M, nd = a.nd;
@@ -1339,45 +1350,71 @@ __global__ void __launch_bounds__(BK_THREADS, 8) tile_bucket(BucketArgs a) {
     // buffers alternate with the block parity, so the bucket of block j can be written while
     // the tile kernel of block j - 1 reads the other one
     const int par = (int)((a.clock[2] + a.blk) & 1);
-    const bool own = T < a.n_tiles;
-    if (tid < 64) {
+    const bool own = tl < w;
+    if (tid < 64) {  // per tile of the group (tile index tid)
         long long base = 0, cap = 0;
-        if (own) {
-            base = ((long long)par * a.B + b) * a.bk_stride + a.bk_off[T];
-            cap = a.bk_off[T + 1] - a.bk_off[T];
+        if (tid < w) {
+            const int Ti = grp * 64 + tid;
+            base = ((long long)par * a.B + b) * a.bk_stride + a.bk_off[Ti];
+            cap = a.bk_off[Ti + 1] - a.bk_off[Ti];
         }
-        s_base[lane] = base;
-        s_cap[lane] = cap < 0x7fffffff ? (int)cap : 0x7fffffff;
-        s_cnt[lane] = 0;
-        s_cntb[lane] = 0;
-        s_fill[lane] = 0;
+        s_base[tid] = base;
+        s_cap[tid] = cap < 0x7fffffff ? (int)cap : 0x7fffffff;
+        s_cnt[tid] = 0;
+        s_cntb[tid] = 0;
+        s_fill[tid] = 0;
     }
     if (tid == 0) s_flush = 0;
     source_offsets(cnt_b, t0, lt0, a.K, nd, M, a.N, s_off);
     const int total = s_off[a.K];
-    int* const dst = a.bk + s_base[lane];  // this lane's bucket: entry p at dst[p]
-    const int capi = s_cap[lane];
+    int* const dst = a.bk + s_base[tl];  // this lane's bucket: entry p at dst[p]
+    const int capi = s_cap[tl];
     const int nt1 = a.n_tiles + 1;
     const int Tc = own ? T : 0;
-    // every thread, between barriers: stages to memory (entries of steps < kh from the front of
-    // the bucket up, the others from its back down, positions by ballots; a block of one part has
-    // only front entries, copied as they are), counters advanced
+    // every thread, between barriers: stage rows to memory (entries of steps < kh from the front
+    // of the bucket up, the others from its back down, positions by ballots; a block of one part
+    // has only front entries, copied as they are), counters advanced.  With one row per tile
+    // (tw = 64) nothing else moves a tile's counters during a flush; with several, each row
+    // reserves its run of the tile's front and back with LDS atomics first.
     const bool split = a.kh < a.K;
     auto flush = [&]() {
-        for (int t = wave; t < 64; t += W) {
-            const int n = s_fill[t] < BK_STAGE ? s_fill[t] : BK_STAGE;
-            const int f0 = s_cnt[t], b0 = s_cntb[t], cp = s_cap[t];
+        for (int r = wave; r < 64; r += W) {
+            const int n = s_fill[r] < BK_STAGE ? s_fill[r] : BK_STAGE;
+            if (n == 0) continue;  // wavefront-uniform
+            const int t = r & (tw - 1), cp = s_cap[t];
+            const int* row = s_stage + r * BK_SP;
             int* d = a.bk + s_base[t];
             if (!split) {
+                int f0 = s_cnt[t];
+                if (spl > 1) {
+                    int g0 = 0;
+                    if (lane == 0) g0 = atomicAdd(&s_cnt[t], n);
+                    f0 = __shfl(g0, 0);
+                }
                 for (int j = lane; j < n; j += 64)
-                    if (f0 + j < cp) d[f0 + j] = s_stage[t * BK_SP + j];
-                if (lane == 0) s_cnt[t] = f0 + n;
+                    if (f0 + j < cp) d[f0 + j] = row[j];
+                if (spl == 1 && lane == 0) s_cnt[t] = f0 + n;
                 continue;
+            }
+            int f0 = s_cnt[t], b0 = s_cntb[t];
+            if (spl > 1) {
+                int nfr = 0;  // front entries of the row (wavefront-uniform)
+                for (int j0 = 0; j0 < n; j0 += 64) {
+                    const int j = j0 + lane;
+                    nfr += __popcll(__ballot(j < n && (row[j < n ? j : 0] & 31) < a.kh));
+                }
+                int g0 = 0, g1 = 0;
+                if (lane == 0) {
+                    g0 = atomicAdd(&s_cnt[t], nfr);
+                    g1 = atomicAdd(&s_cntb[t], n - nfr);
+                }
+                f0 = __shfl(g0, 0);
+                b0 = __shfl(g1, 0);
             }
             int nf = 0, nbk = 0;  // wavefront-uniform
             for (int j0 = 0; j0 < n; j0 += 64) {
                 const int j = j0 + lane;
-                const int x = j < n ? s_stage[t * BK_SP + j] : 0;
+                const int x = j < n ? row[j] : 0;
                 const bool back = j < n && (x & 31) >= a.kh, front = j < n && !back;
                 const unsigned long long mf = __ballot(front), mb = __ballot(back);
                 const unsigned long long below = (1ull << lane) - 1ull;
@@ -1392,7 +1429,7 @@ __global__ void __launch_bounds__(BK_THREADS, 8) tile_bucket(BucketArgs a) {
                 nf += __popcll(mf);
                 nbk += __popcll(mb);
             }
-            if (lane == 0) {
+            if (spl == 1 && lane == 0) {
                 s_cnt[t] = f0 + nf;
                 s_cntb[t] = b0 + nbk;
             }
@@ -1411,10 +1448,13 @@ __global__ void __launch_bounds__(BK_THREADS, 8) tile_bucket(BucketArgs a) {
             s_id[q] = (unsigned)id < (unsigned)a.N ? id << 5 | k : -1;  // id < 2^22 (table < 2^31 entries)
         }
         __syncthreads();
-        for (int r0 = 0; r0 < nc; r0 += W * BU) {
-            const int q0 = r0 + wave * BU;
-            if (q0 < nc) {
-                // The BU spikes are the same for every lane of the wavefront (lane = tile):
+        // One batch of BU spikes per lane: spikes q0 + u (every lane of the wavefront the same
+        // spike) when a lane pass covers one spike (SCALAR), else q0 + u * spl + so.
+        auto batch = [&](const int q0, auto scalar) {
+            constexpr bool SCALAR = decltype(scalar)::value;
+            constexpr int BU = SCALAR ? BUS : BUS / 2;  // per-lane ids take more registers
+            {
+                // SCALAR: the BU spikes are the same for every lane of the wavefront (lane = tile):
                 // their ids are made scalar, so the row addresses are scalar arithmetic and each
                 // lane's table reads are one scalar base plus its fixed tile offset, all in flight
                 // together.  Past the chunk end a lane rereads a valid spike and masks the result.
@@ -1422,8 +1462,9 @@ __global__ void __launch_bounds__(BK_THREADS, 8) tile_bucket(BucketArgs a) {
                 bool use[BU];
 #pragma unroll
                 for (int u = 0; u < BU; u++) {
-                    const int q = q0 + u;
-                    const int v = __builtin_amdgcn_readfirstlane(s_id[q < nc ? q : q0]);
+                    const int q = SCALAR ? q0 + u : q0 + u * spl + so;
+                    const int sv = s_id[q < nc ? q : q0];
+                    const int v = SCALAR ? __builtin_amdgcn_readfirstlane(sv) : sv;
                     use[u] = own && q < nc && v >= 0;
                     id[u] = v >= 0 ? v >> 5 : 0;
                     kk[u] = v & 31;
@@ -1465,12 +1506,13 @@ __global__ void __launch_bounds__(BK_THREADS, 8) tile_bucket(BucketArgs a) {
                     }
                 }
                 if (tot > 0) {
+                    // each stage row gains at most one batch of one wavefront per round
                     int slot = atomicAdd(&s_fill[lane], tot);
-                    if (slot + tot > BK_STAGE - W * BU) s_flush = 1;
+                    if (slot + tot > BK_STAGE - W * BUS) s_flush = 1;
                     // past a full stage: straight to memory (one front reservation for the batch when
                     // the block has one part, else one per entry and part)
                     const int over = slot + tot - BK_STAGE;
-                    int g = !split && over > 0 ? atomicAdd(&s_cnt[lane], over < tot ? over : tot) : 0;
+                    int g = !split && over > 0 ? atomicAdd(&s_cnt[tl], over < tot ? over : tot) : 0;
                     auto put = [&](int entry) {
                         if (slot < BK_STAGE) {
                             s_stage[lane * BK_SP + slot] = entry;
@@ -1478,10 +1520,10 @@ __global__ void __launch_bounds__(BK_THREADS, 8) tile_bucket(BucketArgs a) {
                             if (g < capi) dst[g] = entry;  // past the capacity: dropped, flagged below
                             g++;
                         } else if ((entry & 31) < a.kh) {
-                            const int gf = atomicAdd(&s_cnt[lane], 1);
+                            const int gf = atomicAdd(&s_cnt[tl], 1);
                             if (gf < capi) dst[gf] = entry;
                         } else {
-                            const int gb = capi - 1 - atomicAdd(&s_cntb[lane], 1);
+                            const int gb = capi - 1 - atomicAdd(&s_cntb[tl], 1);
                             if (gb >= 0) dst[gb] = entry;
                         }
                         slot++;
@@ -1501,19 +1543,28 @@ __global__ void __launch_bounds__(BK_THREADS, 8) tile_bucket(BucketArgs a) {
                     }
                 }
             }
+        };
+        const int bu = spl == 1 ? BUS : BUS / 2;  // spikes per lane and round
+        for (int r0 = 0; r0 < nc; r0 += W * bu * spl) {
+            const int q0 = r0 + wave * bu * spl;
+            if (q0 < nc) {
+                if (spl == 1) batch(q0, std::true_type{});
+                else batch(q0, std::false_type{});
+            }
             __syncthreads();
             if (s_flush) flush();
         }
     }
     flush();
-    if (tid < 64 && own) {
-        int c = s_cnt[lane], cb = s_cntb[lane];
-        if (c + cb > capi) {  // cannot happen: the capacities bound every block
+    if (tid < w) {  // per tile of the group (tile index tid)
+        int c = s_cnt[tid], cb = s_cntb[tid];
+        const int cp = s_cap[tid];
+        if (c + cb > cp) {  // cannot happen: the capacities bound every block
             atomicOr(a.err, 2);
-            c = c < capi ? c : capi;
-            cb = cb < capi - c ? cb : capi - c;
+            c = c < cp ? c : cp;
+            cb = cb < cp - c ? cb : cp - c;
         }
-        reinterpret_cast<int2*>(a.bk_cnt)[((size_t)par * a.B + b) * a.n_tiles + T] = make_int2(c, cb);
+        reinterpret_cast<int2*>(a.bk_cnt)[((size_t)par * a.B + b) * a.n_tiles + grp * 64 + tid] = make_int2(c, cb);
     }
 }
 
