@@ -117,6 +117,47 @@ __device__ __forceinline__ int step_slot(long long t, int m) {
     return (int)(((hi % mm) * p32 + lo % mm) % mm);     // < m^2 + m < 2^32
 }
 
+// Division by a divisor that is fixed for a launch (tile count, ring lengths, block length),
+// as one multiply-high and two shifts instead of the ~12-instruction reciprocal sequence the
+// compiler emits for a runtime divisor (on uniform operands the multiply and shifts are scalar
+// instructions).  Granlund and Montgomery, "Division by invariant integers using
+// multiplication" (PLDI 1994), the unsigned round-up multiplier: with l = ceil(log2 d),
+// m = floor(2^32 (2^l - d) / d) + 1, t = mulhi(n, m), n / d = (t + ((n - t) >> min(l, 1))) >>
+// max(l - 1, 0), exact for every 32-bit n and 1 <= d < 2^31.  `div32_for` builds it on the host.
+struct Div32 {
+    unsigned m;
+    unsigned s1, s2;
+    unsigned d;
+};
+
+inline Div32 div32_for(unsigned d) {
+    unsigned l = 0;
+    while (l < 32 && (1ull << l) < d) l++;
+    Div32 r;
+    r.m = (unsigned)((((1ull << 32) * ((1ull << l) - d)) / d) + 1);
+    r.s1 = l < 1 ? l : 1;
+    r.s2 = l > 1 ? l - 1 : 0;
+    r.d = d;
+    return r;
+}
+
+__device__ __forceinline__ unsigned div32(unsigned n, const Div32& v) {
+    const unsigned t = __umulhi(n, v.m);
+    return (t + ((n - t) >> v.s1)) >> v.s2;
+}
+
+// step_slot with the ring length's divider: the common case (0 <= t < 2^32) takes no division.
+__device__ __forceinline__ int step_slot(long long t, const Div32& v) {
+    if (t >= 0 && ((unsigned long long)t >> 32) == 0) {
+        const unsigned lo = (unsigned)t;
+        return (int)(lo - div32(lo, v) * v.d);
+    }
+    return step_slot(t, (int)v.d);
+}
+
+// x mod m for 0 <= x < 2 m: a ring slot a few steps past a slot, without a division.
+__device__ __forceinline__ int wrap2(int x, int m) { return x >= m ? x - m : x; }
+
 struct UpdArgs {
     const long long* clock;
     int N, B, K, kb, blk, first, chunk;
@@ -494,6 +535,7 @@ struct TileArgs {
     long long bk_stride;
     const int* bk_cnt;     // [2][B][n_tiles][2] front and back entries of each bucket
     int b0, nb;            // this launch covers flies [b0, b0 + nb) (grid n_tiles x nb)
+    Div32 dv_tiles, dv_M, dv_ring, dv_K;  // dividers by n_tiles, M, ring_slots and K
     Consts c;
 };
 
@@ -513,6 +555,18 @@ __device__ __forceinline__ void xcd_order(int n_tiles, int B, int* T, int* b) {
     *T = L - *b * n_tiles;
 }
 
+// The same map with the tile count's divider (no division instruction sequence).
+__device__ __forceinline__ void xcd_order(const Div32& dt, int B, int* T, int* b) {
+    const int n_tiles = (int)dt.d;
+    const int total = n_tiles * B;
+    const int p = blockIdx.y * gridDim.x + blockIdx.x;
+    const int x = p % NXCD, i = p / NXCD;
+    const int q = total / NXCD, r = total % NXCD;
+    const int L = x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+    *b = (int)div32((unsigned)L, dt);
+    *T = L - *b * n_tiles;
+}
+
 // Offsets of the per-step spike lists a block's arrivals come from (the spikes of steps
 // t0 + k - n_delay, k < K): s_off[k] = entries of the source steps before k, s_off[K] = all.
 // The first wavefront's lanes load the K list lengths together and scan them with shuffles (K <=
@@ -524,7 +578,7 @@ __device__ __forceinline__ void source_offsets(const int* cnt_b, long long t0, i
         const int lane = threadIdx.x;
         int c = 0;
         if (lane < K && t0 + lane - nd >= 0) {
-            c = cnt_b[(lt0 + lane - nd + M) % M];
+            c = cnt_b[wrap2(lt0 + lane - nd + M, M)];  // lane < K <= nd: below 2 M
             c = c < N ? c : N;
         }
         int x = c;
@@ -628,6 +682,17 @@ __device__ __forceinline__ int once_steps(const int* row, int K, float& u_io, fl
     return rend;
 }
 
+// Zeroes the LDS accumulator, KP x TPB x NPT ints (a multiple of 4: TPB is a multiple of 64),
+// with 16-byte stores: thread tid stores int4 number tid + j TPB for j < ceil(KP NPT / 4), a
+// trip count without a division.
+__device__ __forceinline__ void clear_acc(int* acc, int KP, int NPT, int TPB, int tid) {
+    const int nq = (KP * NPT * TPB) >> 2, per = (KP * NPT + 3) >> 2;
+    for (int j = 0; j < per; j++) {
+        const int q = tid + j * TPB;
+        if (q < nq) reinterpret_cast<int4*>(acc)[q] = make_int4(0, 0, 0, 0);
+    }
+}
+
 // IPT: spike-list entries staged per thread (more loads in flight, more LDS).
 // BK: segments come from the tile's bucket (filled by tile_bucket) instead of being looked up
 // in the segment table for every spike of the fly.
@@ -652,7 +717,7 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
     int* st_end = st_seg + TPB * IPT;    // [TPB * IPT] inclusive prefix of segment lengths
     int* st_k = st_end + TPB * IPT;      // [TPB * IPT]
     int b, T;
-    xcd_order(a.n_tiles, a.nb, &T, &b);
+    xcd_order(a.dv_tiles, a.nb, &T, &b);
     b += a.b0;
     const int i0 = T * TS;
     const size_t bn = (size_t)b * a.N;
@@ -688,11 +753,11 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
     const long long t0 = clk0 + a.kb;
     const uint32_t step0 = clk1 + (uint32_t)a.kb;
     const int nd = a.nd, M = a.M;
-    const int lt0 = step_slot(t0, M);                      // list slot of step t0
-    const int rs0 = step_slot(t0, a.ring_slots);
+    const int lt0 = step_slot(t0, a.dv_M);                 // list slot of step t0
+    const int rs0 = step_slot(t0, a.dv_ring);
     int* cnt_b = a.spk_cnt + (size_t)b * M;
     int* idx_b = a.spk_idx + (size_t)b * M * a.N;
-    if (T == 0 && tid < nd) cnt_b[(lt0 + a.K + tid) % M] = 0;   // lists of the next n_delay steps
+    if (T == 0 && tid < nd) cnt_b[wrap2(lt0 + a.K + tid, M)] = 0;  // lists of the next n_delay steps (K <= nd)
     if (!live) {                                                // retired fly
         if (a.first)
             for (int p = 0; p < NPT; p++) {
@@ -703,7 +768,7 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
     }
     if (tid < KMAX) s_lcnt[tid] = 0;  // spikes per block step (read after the step loop)
     // KP * TS is a multiple of 4 ints (TS is a multiple of 64): 16-byte stores
-    for (int q = tid; q < (KP * TS) >> 2; q += TPB) reinterpret_cast<int4*>(acc)[q] = make_int4(0, 0, 0, 0);
+    clear_acc(acc, KP, NPT, TPB, tid);
     // per-step keys of the Poisson draw (read after the next barrier)
     __shared__ uint32_t s_sk[KMAX];
     const uint32_t hb = (uint32_t)b + a.fly_base;  // the fly's number in the draw
@@ -884,7 +949,7 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
                     ok[r] = e < hi;
                     const int ec = ok[r] ? e : base;
                     kk[r] = entry_step(s_off, a.K, ec);
-                    li_[r] = (size_t)((lt0 + kk[r] - nd + M) % M) * a.N + (ec - s_off[kk[r]]);
+                    li_[r] = (size_t)wrap2(lt0 + kk[r] - nd + M, M) * a.N + (ec - s_off[kk[r]]);
                 }
 #pragma unroll
                 for (int r = 0; r < IPT; r++) ids[r] = idx_b[li_[r]];  // ... then all list loads at once
@@ -1026,8 +1091,8 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
         const int payload = a.hits ? j_[p] : __float_as_int(pp_[p]);
         const int lbase = p * TPB + (tid & ~63);
         // pairs x = q * K + k, x = lane, lane + 64, ...: one division, then increments
-        int q = lane / a.K, k = lane - q * a.K;
-        const int dq = 64 / a.K, dk = 64 - dq * a.K;
+        int q = (int)div32((unsigned)lane, a.dv_K), k = lane - q * a.K;
+        const int dq = (int)div32(64u, a.dv_K), dk = 64 - dq * a.K;
         for (int x0 = 0; x0 < ne * a.K; x0 += 64) {  // uniform trip count: every lane permutes
             const int sl = __builtin_amdgcn_ds_bpermute((q < ne ? q : 0) << 2, src);
             const int pl = __builtin_amdgcn_ds_bpermute(sl << 2, payload);
@@ -1123,7 +1188,7 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
     if constexpr (PARTS == 2) {
         if (KH < a.K) {  // the second part: acc cleared and refilled with the arrivals of [KH, K)
             __syncthreads();
-            for (int q = tid; q < (KP * TS) >> 2; q += TPB) reinterpret_cast<int4*>(acc)[q] = make_int4(0, 0, 0, 0);
+            clear_acc(acc, KP, NPT, TPB, tid);
             __syncthreads();
             pull(KH, a.K);
             __syncthreads();
@@ -1137,7 +1202,7 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
         for (uint32_t m = masks[p]; m; m &= m - 1) atomicAdd(&s_lcnt[__ffs(m) - 1], 1);
     __syncthreads();
     int lpos = 0;
-    if (tid < a.K && s_lcnt[tid] > 0) lpos = atomicAdd(&cnt_b[(lt0 + tid) % M], s_lcnt[tid]);
+    if (tid < a.K && s_lcnt[tid] > 0) lpos = atomicAdd(&cnt_b[wrap2(lt0 + tid, M)], s_lcnt[tid]);
     // state and counts (a window's later blocks add theirs with no-return atomics: no load waits)
 #pragma unroll
     for (int p = 0; p < NPT; p++) {
@@ -1176,7 +1241,7 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
         for (uint32_t m = masks[p]; m; m &= m - 1) {
             const int k = __ffs(m) - 1;
             const int pos = atomicAdd(&s_lcnt[k], 1);
-            if (pos < a.N) idx_b[(size_t)((lt0 + k) % M) * a.N + pos] = i;
+            if (pos < a.N) idx_b[(size_t)wrap2(lt0 + k, M) * a.N + pos] = i;
         }
     }
     if (a.pl_indptr) {
@@ -1301,6 +1366,7 @@ struct BucketArgs {
     const unsigned char* alive;
     int* err;
     int b0;  // this launch covers flies [b0, b0 + gridDim.y)
+    Div32 dv_M, dv_grp;  // dividers by M and by the tile-group count (gridDim.x)
 };
 
 // T16: read the uint16 copy of the segment table (offsets from the row start).  At most 64
@@ -1331,7 +1397,7 @@ __global__ void __launch_bounds__(BK_THREADS, 8) tile_bucket(BucketArgs a) {
     // the tile groups of one fly run on one XCD: they read the same spike lists, and the
     // table-row spans of neighbouring groups share cache lines
     int grp, b;
-    xcd_order(gridDim.x, gridDim.y, &grp, &b);
+    xcd_order(a.dv_grp, gridDim.y, &grp, &b);
     b += a.b0;
     // tiles in this group (1 .. 64); lanes per spike tw (64 unless the group has at most 32
     // tiles: then its count rounded up to a power of two) and spikes per lane pass spl = 64 / tw
@@ -1344,7 +1410,7 @@ __global__ void __launch_bounds__(BK_THREADS, 8) tile_bucket(BucketArgs a) {
     if (a.alive && !a.alive[b]) return;
     const long long t0 = a.clock[0] + a.kb;
     const int M = a.M, nd = a.nd;
-    const int lt0 = step_slot(t0, M);
+    const int lt0 = step_slot(t0, a.dv_M);
     const int* cnt_b = a.spk_cnt + (size_t)b * M;
     const int* idx_b = a.spk_idx + (size_t)b * M * a.N;
     // buffers alternate with the block parity, so the bucket of block j can be written while
@@ -1444,7 +1510,7 @@ __global__ void __launch_bounds__(BK_THREADS, 8) tile_bucket(BucketArgs a) {
         for (int q = tid; q < nc; q += blockDim.x) {  // stage the chunk: id << 5 | step, or -1
             const int e = c0 + q;
             const int k = entry_step(s_off, a.K, e);
-            const int id = idx_b[(size_t)((lt0 + k - nd + M) % M) * a.N + (e - s_off[k])];
+            const int id = idx_b[(size_t)wrap2(lt0 + k - nd + M, M) * a.N + (e - s_off[k])];  // k < K <= nd
             s_id[q] = (unsigned)id < (unsigned)a.N ? id << 5 | k : -1;  // id < 2^22 (table < 2^31 entries)
         }
         __syncthreads();
@@ -1842,6 +1908,10 @@ TileArgs make_tile(const fz_engine* e, const fz_inputs* in, int K, int KP, int k
     a.ptg_idx = e->n_pl > 0 ? e->ptg_idx : nullptr;
     a.spk_idx = e->st.spk_idx;
     a.spk_cnt = e->st.spk_cnt;
+    a.dv_tiles = div32_for((unsigned)(e->n_tiles > 0 ? e->n_tiles : 1));
+    a.dv_M = div32_for((unsigned)a.M);
+    a.dv_ring = div32_for((unsigned)(a.ring_slots > 0 ? a.ring_slots : 1));
+    a.dv_K = div32_for((unsigned)K);
     a.c = e->c;
     return a;
 }
@@ -2208,6 +2278,8 @@ BucketArgs make_bucket(const fz_engine* e, const fz_inputs* in, int K, int kb, i
     a.bk_cnt = e->bk_cnt;
     a.alive = in->alive;
     a.err = e->err;
+    a.dv_M = div32_for((unsigned)a.M);
+    a.dv_grp = div32_for((unsigned)((e->n_tiles + 63) / 64));  // the launch's gridDim.x
     return a;
 }
 
