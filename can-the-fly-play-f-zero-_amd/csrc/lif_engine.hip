@@ -634,8 +634,10 @@ __device__ __forceinline__ int block_incl_scan(int v, int* wsum, int* total) {
 // counter carried as the block step rend at which it runs out; see the kernel).  KC > 0 is the
 // block length known at compile time: the loop unrolls, the LDS reads take immediate offsets
 // and the step index is a constant in every compare.  The float operations and their order are
-// those of the runtime loop (KC = 0), so the two give the same bits.
-template <int KC>
+// those of the runtime loop (KC = 0), so the two give the same bits.  KICK = false drops the
+// Poisson kick (bit test, add and select) for a wavefront none of whose neurons has a hit in
+// the block: without a hit the select keeps u as it is, so the result is the same.
+template <int KC, bool KICK = true>
 __device__ __forceinline__ int once_steps(const int* row, int K, float& u_io, float& g_io, int rend, float bias,
                                           uint32_t hits, const Consts& c) {
     float u = u_io, g = g_io;
@@ -664,8 +666,10 @@ __device__ __forceinline__ int once_steps(const int* row, int K, float& u_io, fl
             g = g * c.decay_syn + inj;
             const float u_int = u + (g + bias - u) * c.decay_m;
             u = held ? u : u_int;
-            const float u_kick = u + c.kick;
-            u = ((hits >> k) & 1u) ? u_kick : u;
+            if constexpr (KICK) {
+                const float u_kick = u + c.kick;
+                u = ((hits >> k) & 1u) ? u_kick : u;
+            }
             const bool held_next = k + 1 - c.n_ref < rel;  // k + 1 < rend
             const bool ge = u >= c.u_th;
             const bool sp = ge & !held_next;
@@ -1153,9 +1157,19 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
                     int rend;
                     // the block lengths of dt = 0.1 ms and 0.25 ms (n_delay 18 and 7), and the parts
                     // of the former, unrolled
-                    if (PARTS == 1 && kn == 18) rend = once_steps<18>(row, kn, u, g, r0, bias, hits, a.c);
-                    else if (PARTS == 1 && kn == 7) rend = once_steps<7>(row, kn, u, g, r0, bias, hits, a.c);
-                    else if (PARTS == 2 && kn == 9) rend = once_steps<9>(row, kn, u, g, r0, bias, hits, a.c);
+                    // (a wavefront whose neurons have no Poisson hit in these steps takes the
+                    // unrolled loop without the kick)
+                    const uint32_t hp = kn < 32 ? hits & ((1u << kn) - 1u) : hits;
+                    const bool kick = __ballot(hp != 0u) != 0ull;
+                    if (PARTS == 1 && kn == 18)
+                        rend = kick ? once_steps<18>(row, kn, u, g, r0, bias, hits, a.c)
+                                    : once_steps<18, false>(row, kn, u, g, r0, bias, hits, a.c);
+                    else if (PARTS == 1 && kn == 7)
+                        rend = kick ? once_steps<7>(row, kn, u, g, r0, bias, hits, a.c)
+                                    : once_steps<7, false>(row, kn, u, g, r0, bias, hits, a.c);
+                    else if (PARTS == 2 && kn == 9)
+                        rend = kick ? once_steps<9>(row, kn, u, g, r0, bias, hits, a.c)
+                                    : once_steps<9, false>(row, kn, u, g, r0, bias, hits, a.c);
                     else rend = once_steps<0>(row, kn, u, g, r0, bias, hits, a.c);
                     if (rend != r0) masks[p] |= 1u << (rend - 1 - a.c.n_ref + k0);
                     rr[p] = rend + k0;
