@@ -74,7 +74,7 @@ constexpr int BK_ROUND = 4;  // tile-bucket entries per thread and round in lif_
 constexpr size_t TILE_STATIC_LDS = 1024;  // room left for lif_tile's static shared arrays
 constexpr size_t TP_MAX_BYTES = 1ull << 30;  // largest (first edge, length) pair table kept
 constexpr int PL_SERIAL = 2;  // plastic rows up to this length are pushed by their own thread
-constexpr int UNROLL = 4;         // tile engine: edge loads in flight per thread
+constexpr int UNROLL = 2;         // tile engine: edge loads in flight per thread (each also pays a search)
 
 struct Consts {
     float decay_syn, decay_m, u_th, u_reset, kick, in_scale, pw_qscale;
