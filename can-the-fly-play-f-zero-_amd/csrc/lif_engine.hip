@@ -2846,7 +2846,10 @@ int fz_graph_create(fz_engine* e, const fz_inputs* in, int steps, const fz_tune*
                     kp.sharedMemBytes = 0;
                     kp.kernelParams = bargs;
                     kp.extra = nullptr;
-                    hipGraphNode_t d = dep >= 0 ? c.tiles[dep] : nullptr;
+                    // the second chain starts after the first chain's first prologue, so from then on
+                    // one chain's prologue runs beside the other's tile kernel instead of in step
+                    hipGraphNode_t d = dep >= 0 ? c.tiles[dep]
+                                                : (&c != &chains[0] && blk == 0 && !gr->bnode.empty() ? gr->bnode[0] : nullptr);
                     if ((err = hipGraphAddKernelNode(&bnode, gr->graph, d ? &d : nullptr, d ? 1 : 0, &kp)) != hipSuccess)
                         return bail("hipGraphAddKernelNode(bucket)", err);
                     gr->bnode.push_back(bnode);

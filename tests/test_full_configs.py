@@ -40,7 +40,8 @@ def _respawn_oracle(ob, flies):
         ob.ring[b] = 0
 
 
-def test_b256_full_connectome_uneven_episodes_match_oracle(full_fly):
+@pytest.mark.parametrize("parts", [1, 2])
+def test_b256_full_connectome_uneven_episodes_match_oracle(full_fly, parts):
     fly, p, B = full_fly, LIFParams(), 256
     sample = np.array([0, 127, 128, 255])
     rng = np.random.default_rng(5)
@@ -53,7 +54,7 @@ def test_b256_full_connectome_uneven_episodes_match_oracle(full_fly):
     gb.set_input(idx, rates[0])
     shape = dict(autotune.tune(gb, use_cache=False, reps=1))
     shape["tile_source"] = 1                                      # the bucket source: two graph chains
-    shape["block_parts"] = 2                                      # blocks in two parts (half the LDS)
+    shape["block_parts"] = parts                                  # blocks in one part, or two (half the LDS)
     gb.set_tune(**shape)
     ob = OracleBrain(fly.weights, p, batch=B, seed=11)
 
