@@ -936,8 +936,8 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
         } else {
             // The list entries of source steps k0 .. k1 - 1 in chunks of TPB * IPT; each thread owns
             // IPT consecutive entries so that all of its list and table loads are in flight
-            // together; segments (any length) are staged with their prefix sums and the edges split
-            // evenly over the threads.
+            // together; each wavefront stages its segments (any length) with their prefix sums and
+            // splits their edges evenly over its lanes (no workgroup barrier in the chunk loop).
             const int lo = s_off[k0], hi = s_off[k1];
             for (int base = lo; base < hi; base += TPB * IPT) {
                 int seg[IPT], len[IPT], kk[IPT];
@@ -1010,8 +1010,20 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
                     sum = 0;
 #pragma unroll
                     for (int r = 0; r < IPT; r++) sum += len[r];
-                    int tot;
-                    int run = block_incl_scan(sum, s_wsum, &tot) - sum;
+                    // Each wavefront stages its own 64 x IPT segments with their prefix sums (a
+                    // shuffle scan) in its own region of the staging arrays and spreads their edges
+                    // over its own 64 lanes: no workgroup barrier, a search over 64 x IPT entries.
+                    // The region is rewritten by the next chunk only after this wavefront's reads
+                    // (LDS operations of a wavefront complete in order).
+                    int x = sum;
+#pragma unroll
+                    for (int o = 1; o < 64; o <<= 1) {
+                        const int y = __shfl_up(x, o);
+                        if (lane >= o) x += y;
+                    }
+                    const int tot = __shfl(x, 63);
+                    int run = x - sum;
+                    const int wb = (tid & ~63) * IPT;
 #pragma unroll
                     for (int r = 0; r < IPT; r++) {
                         run += len[r];
@@ -1019,24 +1031,26 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
                         st_end[tid * IPT + r] = run;
                         st_k[tid * IPT + r] = kk[r] - k0;  // acc column
                     }
-                    __syncthreads();
-                    const int ns = TPB * IPT;
-                    for (int q0 = tid; q0 < tot; q0 += TPB * UNROLL) {
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    const int ns = 64 * IPT;
+                    for (int q0 = lane; q0 < tot; q0 += 64 * UNROLL) {
                         int ek[UNROLL], slot[UNROLL];
 #pragma unroll
                         for (int u = 0; u < UNROLL; u++) {
-                            const int q = q0 + u * TPB;
+                            const int q = q0 + u * 64;
                             ek[u] = -1;
                             slot[u] = 0;
                             if (q < tot) {
-                                int l = 0, h = ns - 1;  // first j with st_end[j] > q
+                                int l = 0, h = ns - 1;  // first j with st_end[wb + j] > q
                                 while (l < h) {
                                     const int mid = (l + h) >> 1;
-                                    if (st_end[mid] > q) h = mid;
+                                    if (st_end[wb + mid] > q) h = mid;
                                     else l = mid + 1;
                                 }
-                                ek[u] = st_seg[l] + (q - (l ? st_end[l - 1] : 0));
-                                slot[u] = st_k[l];
+                                ek[u] = st_seg[wb + l] + (q - (l ? st_end[wb + l - 1] : 0));
+                                slot[u] = st_k[wb + l];
                             }
                         }
                         int loc[UNROLL], val[UNROLL];
@@ -1049,7 +1063,9 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
                         for (int u = 0; u < UNROLL; u++)
                             if (val[u] != 0) atomicAdd(&acc[__umul24(loc[u], KP) + slot[u]], val[u]);
                     }
-                    __syncthreads();
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 }
             }
         }
