@@ -1747,14 +1747,6 @@ struct fz_engine {
 struct fz_graph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    // Tile-engine windows keep their kernel nodes with the arguments they were made with, so
-    // that fz_graph_set_inputs can point the instantiated graph at other input buffers.
-    bool rebindable = false;
-    fz_inputs in{};
-    std::vector<hipGraphNode_t> tnode, bnode;
-    std::vector<TileArgs> targ;
-    std::vector<BucketArgs> barg;
-    std::vector<hipKernelNodeParams> tkp, bkp;  // kernelParams set at each update
 };
 
 namespace {
@@ -2862,15 +2854,9 @@ int fz_graph_create(fz_engine* e, const fz_inputs* in, int steps, const fz_tune*
                     kp.sharedMemBytes = 0;
                     kp.kernelParams = bargs;
                     kp.extra = nullptr;
-                    // the second chain starts after the first chain's first prologue, so from then on
-                    // one chain's prologue runs beside the other's tile kernel instead of in step
-                    hipGraphNode_t d = dep >= 0 ? c.tiles[dep]
-                                                : (&c != &chains[0] && blk == 0 && !gr->bnode.empty() ? gr->bnode[0] : nullptr);
+                    hipGraphNode_t d = dep >= 0 ? c.tiles[dep] : nullptr;
                     if ((err = hipGraphAddKernelNode(&bnode, gr->graph, d ? &d : nullptr, d ? 1 : 0, &kp)) != hipSuccess)
                         return bail("hipGraphAddKernelNode(bucket)", err);
-                    gr->bnode.push_back(bnode);
-                    gr->barg.push_back(ba);
-                    gr->bkp.push_back(kp);
                 }
                 TileArgs ta = make_tile(e, in, K, tile_kp(e, s, K), kb, blk, kb == 0, steps, nullptr, nullptr);
                 ta.kh = part_kh(s, K);
@@ -2893,9 +2879,6 @@ int fz_graph_create(fz_engine* e, const fz_inputs* in, int steps, const fz_tune*
                 hipGraphNode_t tnode;
                 if ((err = hipGraphAddKernelNode(&tnode, gr->graph, nd ? deps : nullptr, nd, &kp)) != hipSuccess)
                     return bail("hipGraphAddKernelNode(tile)", err);
-                gr->tnode.push_back(tnode);
-                gr->targ.push_back(ta);
-                gr->tkp.push_back(kp);
                 c.prev = tnode;
                 c.tiles.push_back(tnode);
                 c.tile_end.push_back(kb + K);
@@ -2943,58 +2926,7 @@ int fz_graph_create(fz_engine* e, const fz_inputs* in, int steps, const fz_tune*
     }
     err = hipGraphInstantiate(&gr->exec, gr->graph, nullptr, nullptr, 0);
     if (err != hipSuccess) return bail("hipGraphInstantiate", err);
-    gr->rebindable = e->mode == 2;
-    gr->in = *in;
     *out = gr;
-    return FZ_OK;
-}
-
-int fz_graph_set_inputs(fz_graph* gr, const fz_inputs* in) {
-    if (!gr || !gr->exec) return fail(FZ_E_ARG, "null graph");
-    if (!in) return fail(FZ_E_ARG, "null inputs");
-    if (!gr->rebindable)
-        return fail(FZ_E_ARG, "fz_graph_set_inputs: only tile-engine windows can be pointed at other inputs; "
-                    "create a new graph");
-    const fz_inputs& g = gr->in;
-    if (in->n_ext != g.n_ext || (in->n_ext > 0 && in->ext_idx != g.ext_idx) || in->seed != g.seed ||
-        in->fly_base != g.fly_base)
-        return fail(FZ_E_ARG, "fz_graph_set_inputs: the ext slots, seed and fly numbering are part of the graph; "
-                    "create a new graph");
-    if (in->n_ext > 0 && (!in->p_ext || !in->bias_ext))
-        return fail(FZ_E_ARG, "n_ext > 0 needs ext_idx, p_ext and bias_ext");
-    if ((g.pw != nullptr) != (in->pw != nullptr)) return fail(FZ_E_ARG, "fz_graph_set_inputs: pw must stay (non-)null");
-    // every argument copy is updated before the first node, so a failure leaves the host copies
-    // describing the new inputs and the graph partly updated: the caller gets an error either way
-    for (size_t i = 0; i < gr->tnode.size(); i++) {
-        TileArgs& a = gr->targ[i];
-        if (a.n_ext > 0) {
-            a.p_ext = in->p_ext;
-            a.bias_ext = in->bias_ext;
-        }
-        if (a.pw) a.pw = in->pw;
-        a.alive = in->alive;
-        void* args[] = {&a};
-        hipKernelNodeParams kp = gr->tkp[i];
-        kp.kernelParams = args;
-        const hipError_t r = hipGraphExecKernelNodeSetParams(gr->exec, gr->tnode[i], &kp);
-        if (r != hipSuccess) {
-            (void)hipGetLastError();
-            return fail(FZ_E_HIP, "hipGraphExecKernelNodeSetParams(tile) failed: %s", hipGetErrorString(r));
-        }
-    }
-    for (size_t i = 0; i < gr->bnode.size(); i++) {
-        BucketArgs& a = gr->barg[i];
-        a.alive = in->alive;
-        void* args[] = {&a};
-        hipKernelNodeParams kp = gr->bkp[i];
-        kp.kernelParams = args;
-        const hipError_t r = hipGraphExecKernelNodeSetParams(gr->exec, gr->bnode[i], &kp);
-        if (r != hipSuccess) {
-            (void)hipGetLastError();
-            return fail(FZ_E_HIP, "hipGraphExecKernelNodeSetParams(bucket) failed: %s", hipGetErrorString(r));
-        }
-    }
-    gr->in = *in;
     return FZ_OK;
 }
 

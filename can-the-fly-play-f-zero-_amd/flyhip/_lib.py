@@ -66,7 +66,6 @@ SIGNATURES = {
     "fz_graph_create": (c_int, [c_void_p, ctypes.POINTER(FzInputs), c_int, ctypes.POINTER(FzTune),
                                 ctypes.POINTER(c_void_p)]),
     "fz_graph_launch": (c_int, [c_void_p, c_void_p]),
-    "fz_graph_set_inputs": (c_int, [c_void_p, ctypes.POINTER(FzInputs)]),
     "fz_graph_destroy": (c_int, [c_void_p]),
     "fz_propagate": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     "fz_check": (c_int, [c_void_p, c_void_p]),

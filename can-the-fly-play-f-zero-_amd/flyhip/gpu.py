@@ -11,9 +11,7 @@ deliberate:
 * weights are integer synapse counts accumulated in an int32 ring (``flyhip.params``); the
   ring is ``ring`` [B, n_delay, n] int32, and ``pending`` is a read-only mV view of it;
 * ``run`` replays one HIP graph per window length (kernels read inputs from device memory, so
-  in-place edits of ``p_ext``/``bias_ext``/``pw`` between frames are seen; with the tile engine
-  these attributes may also be replaced by other tensors of the same shape between windows,
-  e.g. each frame's staged rates, and the graph is pointed at them without a copy);
+  in-place edits of ``p_ext``/``bias_ext``/``pw`` between frames are seen);
 * ``alive`` (uint8 [B]) retires flies: a retired fly is skipped (counts 0) until reset and
   revived, which is how flies with uneven episode lengths share one batch;
 * no ``B * n < 2**24`` limit (flyzero/gpu.py:168): the spike work list is 64-bit addressed.
@@ -180,7 +178,6 @@ class BatchBrain:
         _lib.check(lib.fz_bind(h, ctypes.byref(self._st)), "fz_bind")
         self.tune = _lib.FzTune()  # all zero: engine defaults
         self._graphs: dict = {}
-        self._graph_bufs: dict = {}  # input buffers each cached window graph points at
 
     # ------------------------------------------------------------------ clock
     @property
@@ -350,15 +347,8 @@ class BatchBrain:
         self._drop_graphs()
 
     def _graph(self, steps: int, inp: _lib.FzInputs):
-        # Tile-engine windows are pointed at the current input buffers (probabilities, bias,
-        # plastic weights, alive mask) in place, so a caller may hand over a different buffer
-        # every window (e.g. each frame's rates where they were staged) without a new capture
-        # or a copy; push-engine windows are keyed by those pointers instead.
-        bufs = (inp.p_ext, inp.bias_ext, inp.pw, inp.alive)
-        key = (steps, inp.n_ext, inp.ext_idx, inp.seed, inp.fly_base,
+        key = (steps, inp.n_ext, inp.ext_idx, inp.p_ext, inp.bias_ext, inp.pw, inp.alive, inp.seed, inp.fly_base,
                tuple(getattr(self.tune, f) for f, _ in _lib.FzTune._fields_))
-        if self.engine != "tile":
-            key = key + bufs
         gr = self._graphs.get(key)
         if gr is None:
             if len(self._graphs) > 16:
@@ -367,10 +357,6 @@ class BatchBrain:
             _lib.check(self._lib.fz_graph_create(self._h, ctypes.byref(inp), steps, ctypes.byref(self.tune),
                                                  ctypes.byref(h)), "fz_graph_create")
             gr = self._graphs[key] = h
-            self._graph_bufs[key] = bufs
-        elif self._graph_bufs.get(key) != bufs:
-            _lib.check(self._lib.fz_graph_set_inputs(gr, ctypes.byref(inp)), "fz_graph_set_inputs")
-            self._graph_bufs[key] = bufs
         return gr
 
     def _drop_graphs(self):
@@ -379,7 +365,6 @@ class BatchBrain:
         for h in self._graphs.values():
             self._lib.fz_graph_destroy(h)
         self._graphs = {}
-        self._graph_bufs = {}
 
     def close(self):
         if getattr(self, "_h", None) is not None:

@@ -171,16 +171,6 @@ int fz_run_timed(fz_engine* e, const fz_inputs* in, int steps, const fz_tune* tu
  * update them in place between replays. */
 int fz_graph_create(fz_engine* e, const fz_inputs* in, int steps, const fz_tune* tune, fz_graph** out);
 int fz_graph_launch(fz_graph* gr, void* stream);
-
-/* Tile engine: point an instantiated window graph at other input buffers (p_ext, bias_ext,
- * pw, alive) without capturing it again, e.g. the rates of the next frame where a producer has
- * already put them, instead of copying them into the buffers the graph was made with (no
- * counterpart in the reference, whose BatchBrain copies new rates into its own arrays,
- * flyzero/gpu.py:222-226).  The ext slots (n_ext, ext_idx), the seed and fly_base are part of
- * the graph and must be the ones it was made with.  Takes effect from the next
- * fz_graph_launch; replays already enqueued keep the buffers they were launched with.
- * FZ_E_ARG for push-engine windows (create a new graph there). */
-int fz_graph_set_inputs(fz_graph* gr, const fz_inputs* in);
 int fz_graph_destroy(fz_graph* gr);
 
 /* acc[col] += weight (fixed-point units) for every fixed edge of the n_list

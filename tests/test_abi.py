@@ -47,7 +47,6 @@ def test_argument_errors_are_codes_not_crashes():
     assert lib.fz_bind(None, None) == _lib.FZ_E_ARG
     assert lib.fz_run(None, None, 1, None, None, None, None) == _lib.FZ_E_ARG
     assert lib.fz_graph_launch(None, None) == _lib.FZ_E_ARG
-    assert lib.fz_graph_set_inputs(None, None) == _lib.FZ_E_ARG
     assert lib.fz_destroy(None) == _lib.FZ_OK
     with pytest.raises(ValueError):
         _lib.check(_lib.FZ_E_ARG, "probe")

@@ -178,35 +178,6 @@ def test_hash_rng_frames_graph_and_eager_match_oracle(small_fly, engine):
     assert gb.t == ob.t and gb.step_no == ob.step_no
 
 
-@pytest.mark.parametrize("source", [0, 1])
-def test_tile_window_graph_follows_new_input_buffers_without_a_new_capture(small_fly, source):
-    # every frame's rates staged up front in their own buffers and handed over by replacing
-    # p_ext (and alive, with the same contents in two buffers): the one window graph is pointed
-    # at each frame's buffers in place (fz_graph_set_inputs), the replays are queued without
-    # synchronisation, and every frame's counts equal the oracle's for that frame's rates
-    gb, ob = _pair(small_fly, batch=3, seed=21, engine="tile", tile=1024)
-    gb.set_tune(upd_threads=512, tile_source=source)
-    idx = np.r_[small_fly.input_idx, small_fly.drive_idx]
-    frames = 8
-    rates, staged = [], []
-    for f in range(frames):
-        rates.append(np.r_[smooth_field(small_fly.input_xy, f), np.float32(40.0 + 10.0 * f)])
-        gb.set_input(idx, rates[f])
-        staged.append(gb.p_ext.clone())
-    alive = [torch.tensor([1, 1, 0], dtype=torch.uint8, device="cuda") for _ in range(2)]
-    outs = []
-    for f in range(frames):
-        gb.p_ext = staged[f]
-        gb.alive = alive[f % 2]
-        outs.append(gb.run(FRAME_MS))
-    assert len(gb._graphs) == 1
-    ob.alive = np.array([1, 1, 0], np.uint8)
-    for f in range(frames):
-        ob.set_input(idx, rates[f])
-        np.testing.assert_array_equal(_np(outs[f]), ob.run(FRAME_MS), err_msg=f"frame {f}")
-    _assert_state_equal(gb, ob)
-
-
 @pytest.mark.parametrize("engine", ENGINES)
 def test_a_brain_of_a_slice_of_flies_draws_that_slice_s_noise(small_fly, engine):
     # fly_base: flies [2, 4) of a 4-fly batch in a brain of their own get the same Poisson draws
