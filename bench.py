@@ -241,12 +241,15 @@ def measure_line(fly, p, B, engine, tile, frames, warmup, torch, gain=None, driv
                 "spikes_per_step_per_fly": round(spikes / (kernel_frames * steps * B), 1)}
         if valu:
             roof["valu_issue_frac"] = valu["valu_share"]
-            roof["limit"] = (f"VALU issue: {valu['valu_share']:.0%} of the SIMD cycles "
-                             f"({valu['valu_instr']:.3g} wave64 VALU instructions per launch)")
-        elif B == 1 and engine == "tile":
+            roof["wait_frac"] = valu["wait_share"]
+        if B == 1 and engine == "tile":
             roof["limit"] = ("latency: each block is a chain of dependent memory round trips (spike list -> segment "
                              f"table -> edges -> LDS sums) on {-(-bb.n // tile)} workgroups, well under the HBM and "
                              "VALU rooflines")
+        elif valu:
+            roof["limit"] = (f"latency of the memory phases: waves wait in {valu['wait_share']:.0%} of their cycles; "
+                             f"VALU issue {valu['valu_share']:.0%} of the SIMD cycles "
+                             f"({valu['valu_instr']:.3g} wave64 VALU instructions per launch)")
         res["roofline"] = roof
     del bb, probs
     torch.cuda.empty_cache()

@@ -27,6 +27,7 @@
 // with |v| ~ 45 mV.  Must be compiled with -ffp-contract=off: the float update
 // is evaluated exactly as written so that it matches the CPU oracle bit for bit.
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdarg>
@@ -71,7 +72,7 @@ constexpr int MAX_WAVES = MAX_THREADS / 64;
 constexpr int QMIN = 64;          // smallest push chunk (sizes the chunk-start table)
 constexpr int BK_PIECE = 2;  // edges per tile-bucket entry at most (longer segments are split)
 constexpr int BK_ROUND = 4;  // tile-bucket entries per thread and round in lif_tile
-constexpr size_t TILE_STATIC_LDS = 1024;  // room left for lif_tile's static shared arrays
+constexpr size_t TILE_STATIC_LDS = 1536;  // room left for lif_tile's static shared arrays (at most 1,248 B)
 constexpr size_t TP_MAX_BYTES = 1ull << 30;  // largest (first edge, length) pair table kept
 constexpr int PL_SERIAL = 2;  // plastic rows up to this length are pushed by their own thread
 constexpr int UNROLL = 2;         // tile engine: edge loads in flight per thread (each also pays a search)
@@ -636,10 +637,12 @@ __device__ __forceinline__ int block_incl_scan(int v, int* wsum, int* total) {
 // and the step index is a constant in every compare.  The float operations and their order are
 // those of the runtime loop (KC = 0), so the two give the same bits.  KICK = false drops the
 // Poisson kick (bit test, add and select) for a wavefront none of whose neurons has a hit in
-// the block: without a hit the select keeps u as it is, so the result is the same.
+// the block: without a hit the select keeps u as it is, so the result is the same.  With KICK,
+// wh (wavefront-uniform) has bit k set when some lane of the wavefront has a hit at step k; the
+// kick of the other steps is skipped by a scalar branch, for the same reason.
 template <int KC, bool KICK = true>
 __device__ __forceinline__ int once_steps(const int* row, int K, float& u_io, float& g_io, int rend, float bias,
-                                          uint32_t hits, const Consts& c) {
+                                          uint32_t hits, uint32_t wh, const Consts& c) {
     float u = u_io, g = g_io;
     auto step = [&](int k) {
         const float inj = (float)row[k] * c.in_scale;
@@ -653,31 +656,49 @@ __device__ __forceinline__ int once_steps(const int* row, int K, float& u_io, fl
         rend = sp ? k + 1 + c.n_ref : rend;
     };
     if constexpr (KC > 0) {
-        // The same steps with two VALU operations fewer each: "held" (k < rend) is carried from
-        // the previous step, where it was computed for the spike test: k < rend_new is
-        // (k < rend_old) unless a spike just moved rend_new to k + n_ref > k (n_ref >= K >= 1);
-        // and rend is kept as rel = rend - n_ref, so the spike sets it to the inline
-        // constant k + 1 (no constant-bus move into a VGPR).
-        int rel = rend - c.n_ref;
-        bool held = 0 < rend;
+        // The same steps with fewer VALU operations each.  "held" (k < rend) is carried from the
+        // previous step, where it was computed for the spike test: k < rend_new is (k < rend_old)
+        // unless a spike just moved rend_new to k + n_ref > k (n_ref >= K >= 1).  After a spike
+        // the neuron is held to the end of the block (k' + 1 < k + 1 + n_ref for every k' < K),
+        // so "k + 1 < rend" is one compare against the incoming rend or'ed with the lane mask
+        // "has spiked", and the spike's step is kept by one select.  The empty asm hides the
+        // range of rend (a refractory count that fits a byte), which would otherwise turn the
+        // compare into a 16-bit one that cannot take the step as an inline constant.  The kick
+        // body holds an empty volatile asm so that it stays behind its scalar branch instead of
+        // being turned into selects that run on every step; the block's LDS row is read before
+        // the first step (those branches would otherwise hold each read back to its own step)
+        // and the reset and kick values sit in VGPRs once, not moved or loaded on every step.
+        asm("" : "+v"(rend));
+        float u_reset = c.u_reset, kick = c.kick;
+        asm("" : "+v"(u_reset));
+        asm("" : "+v"(kick));
+        int rv[KC];
+#pragma unroll
+        for (int k = 0; k < KC; k++) rv[k] = row[k];
+        bool held = 0 < rend, done = false;
+        int ks = 0;
 #pragma unroll
         for (int k = 0; k < KC; k++) {
-            const float inj = (float)row[k] * c.in_scale;
+            const float inj = (float)rv[k] * c.in_scale;
             g = g * c.decay_syn + inj;
             const float u_int = u + (g + bias - u) * c.decay_m;
             u = held ? u : u_int;
             if constexpr (KICK) {
-                const float u_kick = u + c.kick;
-                u = ((hits >> k) & 1u) ? u_kick : u;
+                if ((wh >> k) & 1u) {
+                    asm volatile("");
+                    const float u_kick = u + kick;
+                    u = ((hits >> k) & 1u) ? u_kick : u;
+                }
             }
-            const bool held_next = k + 1 - c.n_ref < rel;  // k + 1 < rend
+            const bool held_next = k + 1 < rend || done;
             const bool ge = u >= c.u_th;
-            const bool sp = ge & !held_next;
-            u = sp ? c.u_reset : u;
-            rel = sp ? k + 1 : rel;
-            held = held_next | ge;                          // = held_next || sp
+            const bool sp = ge && !held_next;
+            u = sp ? u_reset : u;
+            ks = sp ? k + 1 : ks;
+            done = done || sp;
+            held = held_next || ge;  // = held_next || sp
         }
-        rend = rel + c.n_ref;
+        rend = done ? ks + c.n_ref : rend;
     } else {
         for (int k = 0; k < K; k++) step(k);
     }
@@ -771,6 +792,9 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
         return;
     }
     if (tid < KMAX) s_lcnt[tid] = 0;  // spikes per block step (read after the step loop)
+    // steps with a Poisson hit in each wavefront's neurons (bit k), per neuron slot
+    __shared__ uint32_t s_wh[MAX_WAVES * NPT];
+    if (tid < MAX_WAVES * NPT) s_wh[tid] = 0u;
     // KP * TS is a multiple of 4 ints (TS is a multiple of 64): 16-byte stores
     clear_acc(acc, KP, NPT, TPB, tid);
     // per-step keys of the Poisson draw (read after the next barrier)
@@ -1125,7 +1149,10 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
                     const uint32_t h = mix32((hb * (uint32_t)a.N + (uint32_t)(i0 + lq)) ^ s_sk[k]);
                     hit = (float)(h >> 8) * (1.0f / 16777216.0f) < __int_as_float(pl);
                 }
-                if (hit) atomicOr(&acc[__umul24(lq, KP) + KH], 1 << k);
+                if (hit) {
+                    atomicOr(&acc[__umul24(lq, KP) + KH], 1 << k);
+                    atomicOr(&s_wh[(tid >> 6) * NPT + p], 1u << k);
+                }
             }
             q += dq;
             k += dk;
@@ -1174,19 +1201,19 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
                     // the block lengths of dt = 0.1 ms and 0.25 ms (n_delay 18 and 7), and the parts
                     // of the former, unrolled
                     // (a wavefront whose neurons have no Poisson hit in these steps takes the
-                    // unrolled loop without the kick)
-                    const uint32_t hp = kn < 32 ? hits & ((1u << kn) - 1u) : hits;
-                    const bool kick = __ballot(hp != 0u) != 0ull;
+                    // unrolled loop without the kick; the others skip the steps without a hit)
+                    const uint32_t wh = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_wh[(tid >> 6) * NPT + p]) >> k0;
+                    const bool kick = (kn < 32 ? wh & ((1u << kn) - 1u) : wh) != 0u;
                     if (PARTS == 1 && kn == 18)
-                        rend = kick ? once_steps<18>(row, kn, u, g, r0, bias, hits, a.c)
-                                    : once_steps<18, false>(row, kn, u, g, r0, bias, hits, a.c);
+                        rend = kick ? once_steps<18>(row, kn, u, g, r0, bias, hits, wh, a.c)
+                                    : once_steps<18, false>(row, kn, u, g, r0, bias, hits, wh, a.c);
                     else if (PARTS == 1 && kn == 7)
-                        rend = kick ? once_steps<7>(row, kn, u, g, r0, bias, hits, a.c)
-                                    : once_steps<7, false>(row, kn, u, g, r0, bias, hits, a.c);
+                        rend = kick ? once_steps<7>(row, kn, u, g, r0, bias, hits, wh, a.c)
+                                    : once_steps<7, false>(row, kn, u, g, r0, bias, hits, wh, a.c);
                     else if (PARTS == 2 && kn == 9)
-                        rend = kick ? once_steps<9>(row, kn, u, g, r0, bias, hits, a.c)
-                                    : once_steps<9, false>(row, kn, u, g, r0, bias, hits, a.c);
-                    else rend = once_steps<0>(row, kn, u, g, r0, bias, hits, a.c);
+                        rend = kick ? once_steps<9>(row, kn, u, g, r0, bias, hits, wh, a.c)
+                                    : once_steps<9, false>(row, kn, u, g, r0, bias, hits, wh, a.c);
+                    else rend = once_steps<0>(row, kn, u, g, r0, bias, hits, wh, a.c);
                     if (rend != r0) masks[p] |= 1u << (rend - 1 - a.c.n_ref + k0);
                     rr[p] = rend + k0;
                 } else {
@@ -2315,19 +2342,29 @@ BucketArgs make_bucket(const fz_engine* e, const fz_inputs* in, int K, int kb, i
 
 
 // One window as a sequence of kernel launches (the graph builder mirrors it).
-// With `ev_*` non-null, events bracket every kernel.
+// With `ev_*` non-null, every kernel is timed by a start and a stop event that the runtime
+// stamps from the kernel's own dispatch (hipExtLaunchKernel), so the pair measures the kernel
+// and not the event packets around it (about 2 us on a 15 us launch).
 int enqueue(fz_engine* e, const fz_inputs* in, int steps, const unsigned* hits, unsigned* raster,
             const Shape& s, hipStream_t stream, std::vector<hipEvent_t>* ev_upd, std::vector<hipEvent_t>* ev_push,
             const WorkList& wl) {
     const dim3 ugrid((e->n + s.ut - 1) / s.ut, e->st.B);
     const dim3 pgrid = push_grid(e, s);
     int kb = 0, blk = 0;
-    auto mark = [&](std::vector<hipEvent_t>* ev) -> int {
-        if (!ev) return FZ_OK;
-        hipEvent_t x;
-        HIPCHK(hipEventCreate(&x));
-        ev->push_back(x);
-        HIPCHK(hipEventRecord(x, stream));
+    auto launch = [&](const void* fn, dim3 grid, dim3 block, void** args, size_t lds,
+                      std::vector<hipEvent_t>* ev) -> int {
+        if (!ev) {  // untimed (and capturable): the plain launch
+            HIPCHK(hipLaunchKernel(fn, grid, block, args, lds, stream));
+            HIPCHK(hipGetLastError());
+            return FZ_OK;
+        }
+        hipEvent_t x[2];
+        for (auto& y : x) {
+            HIPCHK(hipEventCreate(&y));
+            ev->push_back(y);
+        }
+        HIPCHK(hipExtLaunchKernel(fn, grid, block, args, lds, stream, x[0], x[1], 0));
+        HIPCHK(hipGetLastError());
         return FZ_OK;
     };
     int rc;
@@ -2342,20 +2379,15 @@ int enqueue(fz_engine* e, const fz_inputs* in, int steps, const unsigned* hits, 
                 BucketArgs ba = make_bucket(e, in, K, kb, blk);
                 ba.kh = part_kh(s, K);
                 if (s.pl) ba.ptp = e->ptp;
-                if ((rc = mark(ev_push))) return rc;
                 void* bargs[] = {&ba};
-                HIPCHK(hipLaunchKernel(bucket_kernel(e, s.pl), bgrid, dim3(BK_THREADS), bargs, 0, stream));
-                HIPCHK(hipGetLastError());
-                if ((rc = mark(ev_push))) return rc;
+                if ((rc = launch(bucket_kernel(e, s.pl), bgrid, dim3(BK_THREADS), bargs, 0, ev_push))) return rc;
             }
             TileArgs ta = make_tile(e, in, K, tile_kp(e, s, K), kb, blk, kb == 0, steps, hits, raster);
             ta.kh = part_kh(s, K);
             if (s.src == 2) ta.tp = nullptr;
             if (s.pl) ta.ptp = e->ptp;
             void* args[] = {&ta};
-            if ((rc = mark(ev_upd))) return rc;
-            HIPCHK(hipLaunchKernel(fn, tgrid, dim3(s.ut), args, tile_lds(e, s, K), stream));
-            if ((rc = mark(ev_upd))) return rc;
+            if ((rc = launch(fn, tgrid, dim3(s.ut), args, tile_lds(e, s, K), ev_upd))) return rc;
             kb += K;
             blk++;
         }
@@ -2366,18 +2398,19 @@ int enqueue(fz_engine* e, const fz_inputs* in, int steps, const unsigned* hits, 
     while (kb < steps) {
         const int K = (steps - kb) < s.K ? (steps - kb) : s.K;
         UpdArgs ua = make_upd(e, in, s, K, kb, blk, kb == 0, hits, raster, wl);
-        if ((rc = mark(ev_upd))) return rc;
-        hipLaunchKernelGGL(lif_update, ugrid, dim3(s.ut), 0, stream, ua);
-        HIPCHK(hipGetLastError());
-        if ((rc = mark(ev_upd))) return rc;
-        if ((rc = mark(ev_push))) return rc;
+        void* uargs[] = {&ua};
+        if ((rc = launch(reinterpret_cast<const void*>(&lif_update), ugrid, dim3(s.ut), uargs, 0, ev_upd))) return rc;
         if (e->w_bits == 16) {
-            hipLaunchKernelGGL(lif_push<int16_t>, pgrid, dim3(s.pt), 0, stream, make_push<int16_t>(e, in, s, blk, wl));
+            auto pa = make_push<int16_t>(e, in, s, blk, wl);
+            void* pargs[] = {&pa};
+            if ((rc = launch(reinterpret_cast<const void*>(&lif_push<int16_t>), pgrid, dim3(s.pt), pargs, 0, ev_push)))
+                return rc;
         } else {
-            hipLaunchKernelGGL(lif_push<int32_t>, pgrid, dim3(s.pt), 0, stream, make_push<int32_t>(e, in, s, blk, wl));
+            auto pa = make_push<int32_t>(e, in, s, blk, wl);
+            void* pargs[] = {&pa};
+            if ((rc = launch(reinterpret_cast<const void*>(&lif_push<int32_t>), pgrid, dim3(s.pt), pargs, 0, ev_push)))
+                return rc;
         }
-        HIPCHK(hipGetLastError());
-        if ((rc = mark(ev_push))) return rc;
         kb += K;
         blk++;
     }
