@@ -393,6 +393,21 @@ __global__ void tile_pack(long long nnz, const int* col, const WT* w, int frac_b
     }
 }
 
+// Summed |packed weight| of each neuron's fixed input edges into s (64-bit, zeroed), then their
+// maximum into *mx: the bound under which two block steps' sums share one LDS word.
+__global__ void in_strength(long long nnz, const int* col, const int* pk, int lbits, unsigned long long* s) {
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < nnz;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int v = pk[e] >> lbits;
+        atomicAdd(&s[col[e]], (unsigned long long)(v < 0 ? -(long long)v : (long long)v));
+    }
+}
+__global__ void max_u64(int n, const unsigned long long* s, unsigned long long* mx) {
+    unsigned long long m = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) m = s[i] > m ? s[i] : m;
+    atomicMax(mx, m);
+}
+
 // Scatter fly blockIdx.y's spikes of the block (flyzero/gpu.py:98-116), one
 // edge chunk per workgroup iteration.
 template <typename WT>
@@ -546,17 +561,7 @@ struct TileArgs {
 // spike's segment-table row and its row of edges are fetched once per XCD instead of
 // by every XCD.  The map is a bijection onto [0, n_tiles * B) for any dispatch order.
 constexpr int NXCD = 8;
-__device__ __forceinline__ void xcd_order(int n_tiles, int B, int* T, int* b) {
-    const int total = n_tiles * B;
-    const int p = blockIdx.y * gridDim.x + blockIdx.x;
-    const int x = p % NXCD, i = p / NXCD;
-    const int q = total / NXCD, r = total % NXCD;
-    const int L = x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
-    *b = L / n_tiles;
-    *T = L - *b * n_tiles;
-}
-
-// The same map with the tile count's divider (no division instruction sequence).
+// (With the tile count's divider: no division instruction sequence.)
 __device__ __forceinline__ void xcd_order(const Div32& dt, int B, int* T, int* b) {
     const int n_tiles = (int)dt.d;
     const int total = n_tiles * B;
@@ -631,6 +636,21 @@ __device__ __forceinline__ int block_incl_scan(int v, int* wsum, int* total) {
     return x + wsum[wave];
 }
 
+// The arrival sum of block step k in a neuron's LDS row.  H16: two steps per word, step 2j in
+// the low half (biased by 0x8000, so that a negative addend never borrows from the high half)
+// and step 2j + 1 in the high half; exact while every partial sum stays inside int16, which the
+// host guarantees by allowing the mode only when each neuron's summed |input weight| is below
+// 32768.
+template <bool H16>
+__device__ __forceinline__ int arrivals(const int* row, int k) {
+    if constexpr (H16) {
+        const int w = row[k >> 1];
+        return (k & 1) ? (w >> 16) : (int)(short)(w ^ 0x8000);
+    } else {
+        return row[k];
+    }
+}
+
 // The step loop of a block for the case n_ref >= K (at most one spike per block, the refractory
 // counter carried as the block step rend at which it runs out; see the kernel).  KC > 0 is the
 // block length known at compile time: the loop unrolls, the LDS reads take immediate offsets
@@ -640,12 +660,12 @@ __device__ __forceinline__ int block_incl_scan(int v, int* wsum, int* total) {
 // the block: without a hit the select keeps u as it is, so the result is the same.  With KICK,
 // wh (wavefront-uniform) has bit k set when some lane of the wavefront has a hit at step k; the
 // kick of the other steps is skipped by a scalar branch, for the same reason.
-template <int KC, bool KICK = true>
+template <int KC, bool KICK = true, bool H16 = false>
 __device__ __forceinline__ int once_steps(const int* row, int K, float& u_io, float& g_io, int rend, float bias,
                                           uint32_t hits, uint32_t wh, const Consts& c) {
     float u = u_io, g = g_io;
     auto step = [&](int k) {
-        const float inj = (float)row[k] * c.in_scale;
+        const float inj = (float)arrivals<H16>(row, k) * c.in_scale;
         g = g * c.decay_syn + inj;
         const float u_int = u + (g + bias - u) * c.decay_m;
         u = k < rend ? u : u_int;
@@ -674,7 +694,7 @@ __device__ __forceinline__ int once_steps(const int* row, int K, float& u_io, fl
         asm("" : "+v"(kick));
         int rv[KC];
 #pragma unroll
-        for (int k = 0; k < KC; k++) rv[k] = row[k];
+        for (int k = 0; k < KC; k++) rv[k] = arrivals<H16>(row, k);
         bool held = 0 < rend, done = false;
         int ks = 0;
 #pragma unroll
@@ -707,14 +727,14 @@ __device__ __forceinline__ int once_steps(const int* row, int K, float& u_io, fl
     return rend;
 }
 
-// Zeroes the LDS accumulator, KP x TPB x NPT ints (a multiple of 4: TPB is a multiple of 64),
-// with 16-byte stores: thread tid stores int4 number tid + j TPB for j < ceil(KP NPT / 4), a
-// trip count without a division.
-__device__ __forceinline__ void clear_acc(int* acc, int KP, int NPT, int TPB, int tid) {
+// Fills the LDS accumulator, KP x TPB x NPT ints (a multiple of 4: TPB is a multiple of 64),
+// with v, in 16-byte stores: thread tid stores int4 number tid + j TPB for j < ceil(KP NPT / 4),
+// a trip count without a division.
+__device__ __forceinline__ void clear_acc(int* acc, int KP, int NPT, int TPB, int tid, int v = 0) {
     const int nq = (KP * NPT * TPB) >> 2, per = (KP * NPT + 3) >> 2;
     for (int j = 0; j < per; j++) {
         const int q = tid + j * TPB;
-        if (q < nq) reinterpret_cast<int4*>(acc)[q] = make_int4(0, 0, 0, 0);
+        if (q < nq) reinterpret_cast<int4*>(acc)[q] = make_int4(v, v, v, v);
     }
 }
 
@@ -724,7 +744,10 @@ __device__ __forceinline__ void clear_acc(int* acc, int KP, int NPT, int TPB, in
 // PL: plastic synapses are pulled here, with the fly's own weights, for spikes that arrive in
 // the window they were emitted in (a.ptp); only the others are pushed at spike time.
 // PARTS: 2 runs each block as two parts (half the arrival columns in LDS), 1 as one.
-template <typename WT, int NPT, int IPT, bool BK, bool PL, int PARTS>
+// H16: two block steps' arrival sums per LDS word (see arrivals): a row of ceil(K / 2) + 1
+// words instead of K + 1, so more workgroups fit a CU (bucket source, one part, no plastic
+// synapses; the host checks the int16 bound).
+template <typename WT, int NPT, int IPT, bool BK, bool PL, int PARTS, bool H16 = false>
 __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
     extern __shared__ __attribute__((aligned(16))) int lds[];
     __shared__ int s_off[KMAX + 1];
@@ -796,7 +819,7 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
     __shared__ uint32_t s_wh[MAX_WAVES * NPT];
     if (tid < MAX_WAVES * NPT) s_wh[tid] = 0u;
     // KP * TS is a multiple of 4 ints (TS is a multiple of 64): 16-byte stores
-    clear_acc(acc, KP, NPT, TPB, tid);
+    clear_acc(acc, KP, NPT, TPB, tid, H16 ? 0x8000 : 0);
     // per-step keys of the Poisson draw (read after the next barrier)
     __shared__ uint32_t s_sk[KMAX];
     const uint32_t hb = (uint32_t)b + a.fly_base;  // the fly's number in the draw
@@ -848,6 +871,11 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
             *loc = a.col[ek] - i0;
             *val = wval(w[ek], a.frac_bits);
         }
+    };
+    // adds v to the arrival sum of neuron loc (tile-local) at column col (block step - k0)
+    auto acc_add = [&](unsigned loc, int col, int v) {
+        if constexpr (H16) atomicAdd(&acc[__umul24(loc, (unsigned)KP) + (col >> 1)], (col & 1) ? (int)((unsigned)v << 16) : v);
+        else atomicAdd(&acc[__umul24(loc, (unsigned)KP) + col], v);
     };
     // The arrivals of block steps [k0, k1) into acc columns k - k0 (the LDS atomics of the last
     // round may still be in flight when it returns: a barrier orders them before the step loop).
@@ -914,8 +942,8 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
                     for (int r = 0; r < BK_ROUND; r++) {
                         if (e0 + r * TPB >= total) continue;
                         const int col = (v[r] & 31) - k0;
-                        if (w0[r] != 0) atomicAdd(&acc[__umul24((unsigned)l0[r], (unsigned)KP) + col], w0[r]);
-                        if (w1[r] != 0) atomicAdd(&acc[__umul24((unsigned)l1[r], (unsigned)KP) + col], w1[r]);
+                        if (w0[r] != 0) acc_add((unsigned)l0[r], col, w0[r]);
+                        if (w1[r] != 0) acc_add((unsigned)l1[r], col, w1[r]);
                     }
                 }
             } else {
@@ -953,7 +981,7 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
                         const int col = (v[r] & 31) - k0;
 #pragma unroll
                         for (int j = 0; j < BK_PIECE; j++)
-                            if (val[r][j] != 0) atomicAdd(&acc[__umul24(loc[r][j], KP) + col], val[r][j]);
+                            if (val[r][j] != 0) acc_add((unsigned)loc[r][j], col, val[r][j]);
                     }
                 }
             }
@@ -1085,7 +1113,7 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
                         }
 #pragma unroll
                         for (int u = 0; u < UNROLL; u++)
-                            if (val[u] != 0) atomicAdd(&acc[__umul24(loc[u], KP) + slot[u]], val[u]);
+                            if (val[u] != 0) acc_add((unsigned)loc[u], slot[u], val[u]);
                     }
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                     __builtin_amdgcn_wave_barrier();
@@ -1103,7 +1131,7 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
                 int* r = a.ring + ((size_t)b * a.ring_slots + (rs0 + k0 + kq) % a.ring_slots) * a.N + i;
                 const int v = *r;
                 if (v != 0) {
-                    atomicAdd(&acc[__umul24(i - i0, KP) + kq], v);
+                    acc_add((unsigned)(i - i0), kq, v);
                     *r = 0;
                 }
             }
@@ -1113,16 +1141,22 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
     // steps would not leave LDS for enough workgroups per CU: acc then holds one part's columns
     // (and the hit masks, read into registers before the first part's steps).
     const int KH = PARTS == 2 ? a.kh : a.K;
+    const int HC = H16 ? (a.K + 1) >> 1 : KH;  // the hit-mask column
     pull(0, KH);
 
-    // Poisson hits of the tile's input neurons for all K steps, as K-bit masks in column KH of
+    // Poisson hits of the tile's input neurons for all K steps, as K-bit masks in column HC of
     // acc.  Input neurons are scattered, so hashing inside the step loop would run the hash on
     // every lane of almost every wavefront; instead each wavefront spreads the (neuron, step)
     // pairs of its input neurons over its 64 lanes, in registers: a lane permutation lists the
     // input lanes in order (src[q] = lane of the q-th input neuron) and each pair fetches its
-    // neuron's payload from that lane.  No LDS staging and no barrier; the column-KH atomics
+    // neuron's payload from that lane.  No LDS staging and no barrier; the column-HC atomics
     // touch no word the pull adds into.  The draw is flyzero/gpu.py:66-67: lowbias32 of
     // (neuron, step, seed + fly), or the precomputed hit bits.
+    if constexpr (H16) {  // hit words start at zero (the fill put the arrival bias there); only
+                          // lanes of the owner's wavefront or them, after this store in order
+#pragma unroll
+        for (int p = 0; p < NPT; p++) acc[__umul24((unsigned)(p * TPB + tid), (unsigned)KP) + HC] = 0;
+    }
 #pragma unroll
     for (int p = 0; p < NPT; p++) {
         const bool want = i0 + p * TPB + tid < a.N && j_[p] >= 0 && (a.hits || pp_[p] > 0.f);
@@ -1150,7 +1184,7 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
                     hit = (float)(h >> 8) * (1.0f / 16777216.0f) < __int_as_float(pl);
                 }
                 if (hit) {
-                    atomicOr(&acc[__umul24(lq, KP) + KH], 1 << k);
+                    atomicOr(&acc[__umul24(lq, KP) + HC], 1 << k);
                     atomicOr(&s_wh[(tid >> 6) * NPT + p], 1u << k);
                 }
             }
@@ -1191,7 +1225,7 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
             const int li = p * TPB + tid, i = i0 + li;
             const int* row = acc + __umul24(li, KP);
             if (i < a.N) {
-                if (k0 == 0) hm[p] = (uint32_t)row[KH];
+                if (k0 == 0) hm[p] = (uint32_t)row[HC];
                 const uint32_t hits = hm[p] >> k0;
                 float u = uu[p], g = gg[p];
                 const float bias = j_[p] >= 0 ? bias_[p] : 0.f;
@@ -1205,22 +1239,22 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
                     const uint32_t wh = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_wh[(tid >> 6) * NPT + p]) >> k0;
                     const bool kick = (kn < 32 ? wh & ((1u << kn) - 1u) : wh) != 0u;
                     if (PARTS == 1 && kn == 18)
-                        rend = kick ? once_steps<18>(row, kn, u, g, r0, bias, hits, wh, a.c)
-                                    : once_steps<18, false>(row, kn, u, g, r0, bias, hits, wh, a.c);
+                        rend = kick ? once_steps<18, true, H16>(row, kn, u, g, r0, bias, hits, wh, a.c)
+                                    : once_steps<18, false, H16>(row, kn, u, g, r0, bias, hits, wh, a.c);
                     else if (PARTS == 1 && kn == 7)
-                        rend = kick ? once_steps<7>(row, kn, u, g, r0, bias, hits, wh, a.c)
-                                    : once_steps<7, false>(row, kn, u, g, r0, bias, hits, wh, a.c);
+                        rend = kick ? once_steps<7, true, H16>(row, kn, u, g, r0, bias, hits, wh, a.c)
+                                    : once_steps<7, false, H16>(row, kn, u, g, r0, bias, hits, wh, a.c);
                     else if (PARTS == 2 && kn == 9)
-                        rend = kick ? once_steps<9>(row, kn, u, g, r0, bias, hits, wh, a.c)
-                                    : once_steps<9, false>(row, kn, u, g, r0, bias, hits, wh, a.c);
-                    else rend = once_steps<0>(row, kn, u, g, r0, bias, hits, wh, a.c);
+                        rend = kick ? once_steps<9, true, H16>(row, kn, u, g, r0, bias, hits, wh, a.c)
+                                    : once_steps<9, false, H16>(row, kn, u, g, r0, bias, hits, wh, a.c);
+                    else rend = once_steps<0, true, H16>(row, kn, u, g, r0, bias, hits, wh, a.c);
                     if (rend != r0) masks[p] |= 1u << (rend - 1 - a.c.n_ref + k0);
                     rr[p] = rend + k0;
                 } else {
                     int r = rr[p];
                     uint32_t mask = 0;
                     for (int k = 0; k < kn; k++) {
-                        const float inj = (float)row[k] * a.c.in_scale;
+                        const float inj = (float)arrivals<H16>(row, k) * a.c.in_scale;
                         g = g * a.c.decay_syn + inj;
                         const bool refr = r > 0;
                         const float u_int = u + (g + bias - u) * a.c.decay_m;
@@ -1752,6 +1786,7 @@ struct fz_engine {
     int ts = 0, n_tiles = 0;
     int* pk = nullptr;  // packed fixed edges (null: the kernel reads col and w)
     int lbits = 0;
+    long long in_max = -1;  // largest summed |packed weight| into one neuron (-1: not known)
     // tile buckets (tile source 1): per-tile offsets in a fly's arena (entries), arena, counts
     uint16_t* tb16 = nullptr;  // segment table as uint16 offsets from each row's start
     int2* tp = nullptr;        // segment table as (first edge, length) pairs, when it fits TP_MAX_BYTES
@@ -1782,6 +1817,7 @@ struct Shape {
     int ut, pt, chunk, grid, K, ipt, src;
     int pl;     // plastic pulls in effect (requested, and the engine has the plastic pair table)
     int parts;  // tile engine: a block runs as 1 part or as 2 (half the arrival columns in LDS)
+    int h16;    // tile engine: two block steps' arrival sums per LDS word
 };
 
 // Steps of the first part of a block of K steps.
@@ -1808,6 +1844,14 @@ int resolve(const fz_engine* e, const fz_tune* t, Shape* s) {
     if (s->src == 1 && !e->bk4)
         return fail(FZ_E_ARG, "tile_source 1 needs fewer than 2^24 fixed edges (bucket entries hold a 24-bit edge "
                     "index); use tile_source 0");
+    const int ab = t ? t->acc_bits : 0;
+    if (ab != 0 && ab != 16 && ab != 32) return fail(FZ_E_ARG, "acc_bits %d: must be 0, 16 or 32", ab);
+    const bool h16_ok = e->mode == 2 && e->pk && s->src == 1 && s->parts == 1 && !s->pl && e->n_pl == 0 &&
+                        e->in_max >= 0 && e->in_max < 32768;
+    if (ab == 16 && !h16_ok)
+        return fail(FZ_E_ARG, "acc_bits 16 needs the tile engine with packed edges, tile_source 1, one-part blocks, no "
+                    "plastic synapses and every neuron's summed |input weight| below 32768 (%lld)", e->in_max);
+    s->h16 = (ab == 16 || (ab == 0 && h16_ok)) ? 1 : 0;
     if (e->mode == 2) {
         if (!(t && t->upd_threads)) s->ut = e->ts < 256 ? e->ts : 256;
         const int npt = s->ut > 0 ? e->ts / s->ut : 0;
@@ -2003,8 +2047,17 @@ void* tile_kernel_fmt(int npt, int ipt, int src, int pl, int parts) {
 }
 
 // The lif_tile instantiation for an edge format (0: int16 col/w, 1: int32 col/w, 2: packed),
-// neurons per thread, staging depth, segment source and plastic pulls.
-void* tile_kernel(int fmt, int npt, int ipt, int src, int pl, int parts) {
+// neurons per thread, staging depth, segment source, plastic pulls, parts and two steps per LDS
+// word (compiled for packed edges, the bucket source, one part and no plastic pulls).
+void* tile_kernel(int fmt, int npt, int ipt, int src, int pl, int parts, int h16) {
+    if (fmt == 2 && h16 && src == 1 && !pl && parts == 1) {
+        switch (npt) {
+            case 1: return reinterpret_cast<void*>(&lif_tile<PackedEdge, 1, 1, true, false, 1, true>);
+            case 2: return reinterpret_cast<void*>(&lif_tile<PackedEdge, 2, 1, true, false, 1, true>);
+            case 4: return reinterpret_cast<void*>(&lif_tile<PackedEdge, 4, 1, true, false, 1, true>);
+            default: return reinterpret_cast<void*>(&lif_tile<PackedEdge, 8, 1, true, false, 1, true>);
+        }
+    }
     if (fmt == 2) return tile_kernel_fmt<PackedEdge>(npt, ipt, src, pl, parts);
     if (fmt == 0) return tile_kernel_fmt<int16_t>(npt, ipt, src, 0, 1);
     return tile_kernel_fmt<int32_t>(npt, ipt, src, 0, 1);
@@ -2017,7 +2070,7 @@ size_t tile_staging(const Shape& s) { return s.src != 1 ? (size_t)3 * s.ipt * s.
 // Row stride of the LDS accumulator: the first part's arrival columns and the hit-mask column,
 // rounded up to odd (no bank conflicts in the step loop) when that fits the LDS budget.
 int tile_kp(const fz_engine* e, const Shape& s, int K) {
-    const int kh = part_kh(s, K);
+    const int kh = s.h16 ? (K + 1) / 2 : part_kh(s, K);  // arrival words per neuron
     const int odd = (kh + 1) | 1;
     return (size_t)odd * e->ts * 4 + tile_staging(s) <= e->lds_max - TILE_STATIC_LDS ? odd : kh + 1;
 }
@@ -2371,7 +2424,7 @@ int enqueue(fz_engine* e, const fz_inputs* in, int steps, const unsigned* hits, 
     if (e->mode == 2) {
         const dim3 tgrid(e->n_tiles, e->st.B);
         const dim3 bgrid((e->n_tiles + 63) / 64, e->st.B);
-        void* fn = tile_kernel(tile_fmt(e), e->ts / s.ut, s.ipt, s.src, s.pl, s.parts);
+        void* fn = tile_kernel(tile_fmt(e), e->ts / s.ut, s.ipt, s.src, s.pl, s.parts, s.h16);
         if (s.src == 1 && (rc = ensure_buckets(e))) return rc;
         while (kb < steps) {
             const int K = (steps - kb) < s.K ? (steps - kb) : s.K;
@@ -2633,6 +2686,7 @@ int pack_edges(fz_engine* e, int tile) {
     (void)hipFree(e->pk);
     e->pk = nullptr;
     e->lbits = 0;
+    e->in_max = -1;
     if (e->nnz == 0) return FZ_OK;
     int lbits = 0;
     while ((1 << lbits) < tile) lbits++;
@@ -2674,6 +2728,21 @@ int pack_edges(fz_engine* e, int tile) {
     }
     e->pk = pk;
     e->lbits = lbits;
+    // the input-strength bound (a failure here only leaves the two-steps-per-word mode off)
+    e->in_max = -1;
+    unsigned long long* ins = nullptr;
+    if (hipMalloc(&ins, sizeof(unsigned long long) * ((size_t)e->n + 1)) == hipSuccess &&
+        hipMemset(ins, 0, sizeof(unsigned long long) * ((size_t)e->n + 1)) == hipSuccess) {
+        hipLaunchKernelGGL(in_strength, dim3(grid), dim3(256), 0, 0, e->nnz, e->col, pk, lbits, ins);
+        const unsigned g2 = (unsigned)((e->n + 255) / 256 < 1024 ? (e->n + 255) / 256 : 1024);
+        hipLaunchKernelGGL(max_u64, dim3(g2), dim3(256), 0, 0, e->n, ins, ins + e->n);
+        unsigned long long mx = 0;
+        if (hipGetLastError() == hipSuccess &&
+            hipMemcpy(&mx, ins + e->n, sizeof(mx), hipMemcpyDeviceToHost) == hipSuccess)
+            e->in_max = mx < (1ull << 62) ? (long long)mx : (1ll << 62);
+    }
+    (void)hipGetLastError();
+    (void)hipFree(ins);
     return FZ_OK;
 }
 
@@ -2698,16 +2767,17 @@ int fz_set_tiles(fz_engine* e, int tile, int n_tiles, const int* tb, const int* 
             for (int ipt : {1, 4})
                 for (int src : {0, 1})
                     for (int pl : {0, 1})
-                        for (int parts : {1, 2}) {
-                            hipError_t r = hipFuncSetAttribute(tile_kernel(fmt, npt, ipt, src, pl, parts),
-                                                               hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                               (int)(e->lds_max - TILE_STATIC_LDS));
-                            if (r != hipSuccess) {
-                                (void)hipGetLastError();  // no sticky error for the caller's next HIP call
-                                return fail(FZ_E_HIP, "hipFuncSetAttribute(max dynamic LDS %zu) failed: %s",
-                                            e->lds_max - TILE_STATIC_LDS, hipGetErrorString(r));
+                        for (int parts : {1, 2})
+                            for (int h16 : {0, 1}) {
+                                hipError_t r = hipFuncSetAttribute(tile_kernel(fmt, npt, ipt, src, pl, parts, h16),
+                                                                   hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                                   (int)(e->lds_max - TILE_STATIC_LDS));
+                                if (r != hipSuccess) {
+                                    (void)hipGetLastError();  // no sticky error for the caller's next HIP call
+                                    return fail(FZ_E_HIP, "hipFuncSetAttribute(max dynamic LDS %zu) failed: %s",
+                                                e->lds_max - TILE_STATIC_LDS, hipGetErrorString(r));
+                                }
                             }
-                        }
     int rc = validate_tiles(e, tb, n_tiles, tile);
     if (rc) return rc;
     // from here on a failure leaves the engine unbound, in push mode, until set up again
@@ -2838,7 +2908,7 @@ int fz_graph_create(fz_engine* e, const fz_inputs* in, int steps, const fz_tune*
     int kb = 0, blk = 0;
     std::vector<hipGraphNode_t> lasts;  // tile engine: the last node of each chain
     if (e->mode == 2) {
-        void* fn = tile_kernel(tile_fmt(e), e->ts / s.ut, s.ipt, s.src, s.pl, s.parts);
+        void* fn = tile_kernel(tile_fmt(e), e->ts / s.ut, s.ipt, s.src, s.pl, s.parts, s.h16);
         if (s.src == 1 && (rc = ensure_buckets(e))) {
             (void)hipGraphDestroy(gr->graph);
             delete gr;

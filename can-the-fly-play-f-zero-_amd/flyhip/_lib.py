@@ -14,7 +14,7 @@ import torch  # noqa: F401  (loads the HIP runtime the library links against)
 
 from .build import LIB_PATH
 
-FZ_ABI_VERSION = 3
+FZ_ABI_VERSION = 4
 FZ_OK, FZ_E_ARG, FZ_E_STATE, FZ_E_HIP, FZ_E_OVERFLOW = 0, -1, -2, -3, -4
 
 c_void_p = ctypes.c_void_p
@@ -42,7 +42,8 @@ class FzInputs(ctypes.Structure):
 class FzTune(ctypes.Structure):
     _fields_ = [("upd_threads", c_int), ("push_threads", c_int), ("push_chunk", c_int),
                 ("push_grid", c_int), ("block_steps", c_int), ("tile_stage", c_int),
-                ("tile_source", c_int), ("plastic_pull", c_int), ("block_parts", c_int)]
+                ("tile_source", c_int), ("plastic_pull", c_int), ("block_parts", c_int),
+                ("acc_bits", c_int)]
 
 
 # name -> (restype, argtypes); every symbol declared in include/flyhip.h

@@ -341,9 +341,9 @@ class BatchBrain:
         _lib.check(self._lib.fz_check(self._h, _lib.stream_handle(self.device)), "fz_check")
 
     def set_tune(self, upd_threads=0, push_threads=0, push_chunk=0, push_grid=0, block_steps=0, tile_stage=0,
-                 tile_source=0, plastic_pull=0, block_parts=0):
+                 tile_source=0, plastic_pull=0, block_parts=0, acc_bits=0):
         self.tune = _lib.FzTune(upd_threads, push_threads, push_chunk, push_grid, block_steps, tile_stage,
-                                tile_source, plastic_pull, block_parts)
+                                tile_source, plastic_pull, block_parts, acc_bits)
         self._drop_graphs()
 
     def _graph(self, steps: int, inp: _lib.FzInputs):

@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define FZ_ABI_VERSION 3
+#define FZ_ABI_VERSION 4
 
 #define FZ_OK 0
 #define FZ_E_ARG (-1)      /* invalid argument (null pointer, size out of range) */
@@ -101,6 +101,13 @@ typedef struct fz_tune {
                         * each part's arrivals pulled into LDS in turn while the neuron state stays
                         * in registers: half the LDS per workgroup, so more workgroups per CU.  The
                         * same results either way. */
+    int acc_bits;      /* tile engine: bits per block step's arrival sum in LDS.  32 = one int per
+                        * step; 16 = two steps per int (16-bit sums, the low one biased): about half
+                        * the LDS per workgroup, so more workgroups per CU.  16 is allowed with
+                        * packed edges, tile_source 1, one-part blocks and no plastic synapses, when
+                        * every neuron's summed |input weight| (in the packed integer units) is
+                        * below 32768, which keeps every partial sum inside int16: the same results
+                        * either way.  0 = 16 where allowed, else 32. */
 } fz_tune;
 
 int fz_version(void);

@@ -40,8 +40,8 @@ def _respawn_oracle(ob, flies):
         ob.ring[b] = 0
 
 
-@pytest.mark.parametrize("parts", [1, 2])
-def test_b256_full_connectome_uneven_episodes_match_oracle(full_fly, parts):
+@pytest.mark.parametrize("parts,acc_bits", [(1, 0), (2, 0), (1, 32)])
+def test_b256_full_connectome_uneven_episodes_match_oracle(full_fly, parts, acc_bits):
     fly, p, B = full_fly, LIFParams(), 256
     sample = np.array([0, 127, 128, 255])
     rng = np.random.default_rng(5)
@@ -55,6 +55,7 @@ def test_b256_full_connectome_uneven_episodes_match_oracle(full_fly, parts):
     shape = dict(autotune.tune(gb, use_cache=False, reps=1))
     shape["tile_source"] = 1                                      # the bucket source: two graph chains
     shape["block_parts"] = parts                                  # blocks in one part, or two (half the LDS)
+    shape["acc_bits"] = acc_bits                                  # two 16-bit step sums per LDS word where allowed, or one int
     gb.set_tune(**shape)
     ob = OracleBrain(fly.weights, p, batch=B, seed=11)
 

@@ -297,7 +297,14 @@ def test_extreme_launch_shapes_are_exact(small_fly, shape):
                                         (1024, dict(upd_threads=256, tile_stage=4, block_parts=2)),
                                         (256, dict(upd_threads=256, block_steps=7, block_parts=2)),
                                         (2048, dict(upd_threads=512, block_steps=9, tile_source=2, block_parts=2)),
-                                        (640, dict(upd_threads=640, tile_stage=4))])
+                                        (640, dict(upd_threads=640, tile_stage=4)),
+                                        # one int per step's arrival sum, and two 16-bit sums per int
+                                        (1024, dict(upd_threads=256, tile_source=1, acc_bits=32)),
+                                        (512, dict(upd_threads=64, block_steps=5, tile_source=1, acc_bits=32)),
+                                        (256, dict(upd_threads=256, block_steps=7, tile_source=1, acc_bits=32)),
+                                        (768, dict(upd_threads=384, tile_source=1, acc_bits=16)),
+                                        (2048, dict(upd_threads=256, block_steps=7, tile_source=1, acc_bits=16)),
+                                        (1024, dict(upd_threads=1024, block_steps=5, tile_source=1, acc_bits=16))])
 def test_tile_engine_shapes_are_exact(small_fly, tile, shape):
     # shapes that the LDS budget admits, incl. 8 neurons per thread, 5-step blocks and tiles
     # that are not powers of two
@@ -309,6 +316,40 @@ def test_tile_engine_shapes_are_exact(small_fly, tile, shape):
         rate = np.r_[smooth_field(fly.input_xy, f), np.float32(60.0)]
         gb.set_input(idx, rate)
         ob.set_input(idx, rate)
+        np.testing.assert_array_equal(_np(gb.run(FRAME_MS)), ob.run(FRAME_MS))
+    _assert_state_equal(gb, ob)
+
+
+def test_two_sums_per_word_needs_its_bound_and_shape(small_fly):
+    # 16-bit step sums are refused where they are not compiled (two-part blocks, the table
+    # source) and where a neuron's summed |input weight| could leave int16; the automatic
+    # choice then keeps one int per step, and the results stay the oracle's
+    gb, _ = _pair(small_fly, batch=1, engine="tile", tile=1024)
+    for bad in (dict(tile_source=1, block_parts=2, acc_bits=16), dict(tile_source=0, acc_bits=16),
+                dict(tile_source=1, acc_bits=8)):
+        gb.set_tune(upd_threads=256, **bad)
+        with pytest.raises(ValueError, match="acc_bits"):
+            gb.run(FRAME_MS, graph=False)
+    rng = np.random.default_rng(17)
+    n, m = 3000, 90000
+    rows = np.r_[rng.integers(0, n, m), np.arange(1200)]
+    cols = np.r_[rng.integers(0, n, m), np.full(1200, 7)]  # neuron 7: 1,200 more inputs of weight 30
+    vals = np.r_[np.round(rng.random(m) * 12.0 - 4.0), np.full(1200, 30.0)].astype(np.float32)
+    w = sp.coo_matrix((vals, (rows, cols)), shape=(n, n)).tocsr()
+    w.eliminate_zeros()
+    assert np.abs(w[:, 7]).sum() >= 32768
+    p = LIFParams()
+    gb = BatchBrain(w, p, batch=2, seed=4, engine="tile", tile=512)
+    ob = OracleBrain(w, p, batch=2, seed=4)
+    idx = np.arange(0, n, 7)
+    rate = np.full(len(idx), 80.0, np.float32)
+    gb.set_tune(upd_threads=256, tile_source=1, acc_bits=16)
+    gb.set_input(idx, rate)
+    with pytest.raises(ValueError, match="32768"):
+        gb.run(FRAME_MS, graph=False)
+    gb.set_tune(upd_threads=256, tile_source=1)
+    ob.set_input(idx, rate)
+    for _ in range(2):
         np.testing.assert_array_equal(_np(gb.run(FRAME_MS)), ob.run(FRAME_MS))
     _assert_state_equal(gb, ob)
 
