@@ -1437,6 +1437,7 @@ constexpr int BK_CHUNK = 2048;
 constexpr int BK_UNROLL = 8;         // spikes per batch: table loads in flight per lane
 constexpr int BK_THREADS = 1024;
 constexpr int BK_STAGE = 256;        // entries staged in LDS per tile between flushes
+constexpr int BK_SYNC = 4;           // rounds between the barriers that may flush the stages
 constexpr int BK_SP = BK_STAGE + 1;  // odd row stride: the 64 lanes' rows start in different banks
 
 struct BucketArgs {
@@ -1701,15 +1702,21 @@ __global__ void __launch_bounds__(BK_THREADS, 8) tile_bucket(BucketArgs a) {
                 }
             }
         };
+        // A barrier (and a flush if a stage passed its mark) after every BK_SYNC rounds and
+        // after the chunk's last round, so that the wavefronts' rounds between them overlap
+        // instead of each round waiting for the slowest wavefront.  Between barriers a stage row
+        // can gain more than its room; such entries take bucket positions at once (put).
         const int bu = spl == 1 ? BUS : BUS / 2;  // spikes per lane and round
-        for (int r0 = 0; r0 < nc; r0 += W * bu * spl) {
+        for (int r0 = 0, rr = 1; r0 < nc; r0 += W * bu * spl, rr++) {
             const int q0 = r0 + wave * bu * spl;
             if (q0 < nc) {
                 if (spl == 1) batch(q0, std::true_type{});
                 else batch(q0, std::false_type{});
             }
-            __syncthreads();
-            if (s_flush) flush();
+            if (rr % BK_SYNC == 0 || r0 + W * bu * spl >= nc) {  // uniform
+                __syncthreads();
+                if (s_flush) flush();
+            }
         }
     }
     flush();
