@@ -1467,8 +1467,8 @@ struct BucketArgs {
 // Entries are staged per tile in LDS and written out in flushes, one tile per wavefront with
 // the lanes on consecutive entries: a tile gains only a few entries per batch, so writing
 // them straight to memory would cost one memory transaction per 4-byte entry.  A flush runs
-// after a round (every wavefront one batch) that left some stage more than half full; entries
-// past a full stage go straight to memory.
+// at the barrier after every BK_SYNC rounds (every wavefront one batch per round) when some
+// stage has passed its mark; entries past a full stage go straight to memory.
 // PL: also the plastic segments (entries nnz + first plastic edge) of the spikes emitted in this
 // window, which lif_tile pulls with the fly's own weights.
 template <bool T16, bool PL>
