@@ -143,9 +143,13 @@ def _assert_state_equal(gb, ob):
         np.testing.assert_array_equal(_np(gb.ring), ob.ring)
 
 
-@pytest.mark.parametrize("engine", ENGINES)
-def test_raster_identical_with_precomputed_hits(small_fly, engine):
+@pytest.mark.parametrize("engine,shape", [(e, {}) for e in ENGINES] +
+                         [("tile", dict(upd_threads=256, tile_source=1, acc_bits=16)),
+                          ("tile", dict(upd_threads=256, tile_source=1, acc_bits=32))])
+def test_raster_identical_with_precomputed_hits(small_fly, engine, shape):
     gb, ob = _pair(small_fly, batch=2, engine=engine)
+    if shape:
+        gb.set_tune(**shape)             # the bucket source with two 16-bit step sums per word, or one int
     steps = 400
     hits = poisson_hits(ob.p_ext[0], steps, 2, seed=3)
     cg, rg = gb.run_steps(steps, hits=torch.from_numpy(hits.view(np.int32)).cuda(), raster=True)
