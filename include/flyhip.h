@@ -167,9 +167,11 @@ int fz_run(fz_engine* e, const fz_inputs* in, int steps, const unsigned* hits, u
 int fz_run_static(fz_engine* e, const fz_inputs* in, int steps, const unsigned* hits, unsigned* raster,
                   const fz_tune* tune, void* stream);
 
-/* As fz_run, timing every kernel with HIP events on `stream`; waits for the
- * stream.  Out: summed milliseconds and launch counts of the update and push
- * kernels. */
+/* As fz_run, timing every kernel with a start and a stop HIP event that the
+ * runtime stamps from the kernel's own dispatch (hipExtLaunchKernel) on
+ * `stream`; waits for the stream.  Out: summed milliseconds and launch counts
+ * of the update kernels (the tile kernel with the tile engine) and of the push
+ * kernels (the bucket prologue with the tile engine's bucket source). */
 int fz_run_timed(fz_engine* e, const fz_inputs* in, int steps, const fz_tune* tune, void* stream,
                  float* upd_ms, int* upd_launches, float* push_ms, int* push_launches);
 
