@@ -1153,7 +1153,8 @@ __global__ void __launch_bounds__(MAX_THREADS) lif_tile(TileArgs a) {
     // touch no word the pull adds into.  The draw is flyzero/gpu.py:66-67: lowbias32 of
     // (neuron, step, seed + fly), or the precomputed hit bits.
     if constexpr (H16) {  // hit words start at zero (the fill put the arrival bias there); only
-                          // lanes of the owner's wavefront or them, after this store in order
+                          // lanes of the owning wavefront set their bits, after this store (a
+                          // wavefront's LDS operations complete in order)
 #pragma unroll
         for (int p = 0; p < NPT; p++) acc[__umul24((unsigned)(p * TPB + tid), (unsigned)KP) + HC] = 0;
     }
